@@ -1,0 +1,158 @@
+"""Throughput of the per-step particle hot path on one MI355X.
+
+Workload (default): 1,000,000 beads of 3 types in a 2,000 A periodic box, 6 tabulated
+WCA+DH pair tables (1,024 knots), a 256^3 FP32 membrane grid sampled tricubically,
+Langevin (BAOAB, gamma = 1/ps), cutoff 30 A + 5 A skin, rebuild period 20.
+
+A step is SimManager's per-step sequence (rebuild policy, forces, PMF, integrator)
+over the whole system. Inputs are resident in HBM before the timed region; timing
+uses HIP events on the stream the library launches on. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "mars-md_amd"))
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+
+
+def algorithmic_bytes(spec) -> dict:
+    """Bytes a step of this method must move (per particle-step), as accounted for
+    the workload: state S_state, grid S_field, and the rebuild share
+    (S_rebuild + 8 P_half) / K. Returns the figures and the per-launch bytes of the
+    fused step kernel (S_state + S_field per particle)."""
+    import numpy as np
+
+    n = spec.n
+    langevin = spec.params.integrator == 1
+    s_state = 52.0 if langevin else 28.0
+    s_topo = 0.0
+    if spec.bonds is not None:
+        s_topo += 8.0 * len(spec.bonds) / n
+    if spec.angles is not None:
+        s_topo += 12.0 * len(spec.angles) / n
+    s_field = sum(g.values.size * 4 for g in spec.grids) / n
+    L = np.prod(spec.box.size)
+    rho = n / L
+    p_half = 0.5 * rho * 4.0 / 3.0 * np.pi * spec.params.pairlist_cutoff ** 3
+    s_rebuild = 144.0
+    K = spec.params.rebuild_period
+    b_alg = s_state + s_topo + s_field + (s_rebuild + 8.0 * p_half) / K
+    return {"s_state": s_state, "s_topo": s_topo, "s_field": s_field, "p_half": p_half,
+            "b_alg_per_particle_step": b_alg, "step_kernel_bytes_per_launch": (s_state + s_topo + s_field) * n}
+
+
+def cpu_baseline(spec, steps: int):
+    """The oracle (single-threaded C++ restatement of the reference functors) on the
+    same workload: one untimed step (pairlist build), then `steps` timed steps."""
+    from oracle import pyoracle
+
+    sim = pyoracle.Sim(spec)
+    sim.run(1)
+    t0 = time.perf_counter()
+    sim.run(steps, first_step=1)
+    dt = time.perf_counter() - t0
+    return {"value": spec.n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps after the step-1 pairlist build, {spec.n} beads, same workload; "
+                      f"host {platform.processor() or platform.machine()} nproc={os.cpu_count()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=15)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import marsmd
+    from marsmd import configs
+
+    spec = configs.config2(n=args.n, grid_n=args.grid)
+    patch = marsmd.Patch(spec)
+    stream = torch.cuda.current_stream()
+    K = spec.params.rebuild_period
+    patch.run(args.warmup)
+    torch.cuda.synchronize()
+
+    # timed region: `steps` steps in one run() call, bracketed by HIP events on the
+    # stream the library launches on
+    first = args.warmup
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    patch.run(args.steps, first_step=first)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = e0.elapsed_time(e1)
+    ms_step = total_ms / args.steps
+    # per-launch duration of the fused step kernel: a further 2K steps with events
+    # around each step; steps without a rebuild are a single kernel launch
+    first2 = first + args.steps
+    nprobe = 2 * K
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(nprobe + 1)]
+    ev[0].record(stream)
+    for s in range(nprobe):
+        patch.run(1, first_step=first2 + s)
+        ev[s + 1].record(stream)
+    torch.cuda.synchronize()
+    per = np.array([ev[s].elapsed_time(ev[s + 1]) for s in range(nprobe)])
+    steps_abs = np.arange(first2 + 1, first2 + nprobe + 1)
+    plain = (steps_abs - 1) % K != 0
+    value = spec.n * args.steps / (total_ms / 1e3)
+    acc = algorithmic_bytes(spec)
+    t_kernel = float(np.mean(per[plain])) * 1e-3
+    achieved = acc["step_kernel_bytes_per_launch"] / t_kernel
+    out = {
+        "metric": "particle-steps/s at 1M beads",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded)",
+        "config": {"workload": spec.name, "beads": spec.n, "box_A": spec.box.size[0], "types": len(spec.types),
+                   "pair_tables": len(spec.nb_tables), "grid": f"{args.grid}^3 tricubic", "integrator": "langevin",
+                   "cutoff_A": spec.params.cutoff, "skin_A": spec.params.pairlist_cutoff - spec.params.cutoff,
+                   "rebuild_period": K},
+        "hbm_roofline_frac_whole_step": value * acc["b_alg_per_particle_step"] / HBM_PEAK,
+        "b_alg_per_particle_step": acc["b_alg_per_particle_step"],
+        "roofline": {"bound": "hbm", "kernel": "k_step (fused pair force + PMF + BAOAB)",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "avg_launch_us": t_kernel * 1e6,
+                     "rebuild_step_ms": float(np.mean(per[~plain])) if (~plain).any() else None},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(spec, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

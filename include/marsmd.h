@@ -1,0 +1,403 @@
+/*
+ * marsmd.h — C ABI of the MI355X (gfx950) implementation of MARS-MD's per-step
+ * particle hot path: Z-order keys + stable radix sort, the cell-stencil pairlist,
+ * tabulated nonbonded / bond / angle forces, grid-potential (PMF) sampling and the
+ * Brownian (BD) and Langevin (BAOAB) integrators with the Philox counter RNG.
+ *
+ * Conventions (all entry points):
+ *   - Every function returns marsmd_status (0 = success). On failure
+ *     marsmd_last_error() returns a thread-local message; the C++ mirror rethrows it
+ *     as MARS::Exception, as the reference's MARS_Exception does.
+ *   - Array pointers are DEVICE pointers unless the parameter is documented "host".
+ *   - Vector3 arrays are 16-byte {x, y, z, t} records (the reference's alignas(16)
+ *     Vector3_t<float>, src/Types/Vector3.h:41-56); ForceEnergy.t carries energy.
+ *   - int2 pair arrays are {x, y} with x < y in the caller's (canonical) indices.
+ *   - Launches are ordered on the hipStream_t passed (NULL = legacy default stream)
+ *     and never synchronise implicitly, except where a function returns a host value.
+ *   - The caller owns every buffer it passes; scratch belongs to opaque handles.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative to the
+ * reference tree).
+ */
+#ifndef MARSMD_H_
+#define MARSMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int marsmd_status;
+typedef void* marsmd_stream; /* hipStream_t */
+
+#define MARSMD_OK 0
+#define MARSMD_ERR_VALUE 1   /* ExceptionType::ValueError   (src/MARSException.h:103-130) */
+#define MARSMD_ERR_RUNTIME 2 /* ExceptionType::RuntimeError                            */
+#define MARSMD_ERR_DEVICE 3  /* a HIP runtime call failed (CUDA_CHECK, KernelHelper.cuh:170) */
+#define MARSMD_ERR_CAPACITY 4 /* pairlist capacity ceiling (ZOrderPairlist.cpp:292-335) */
+
+#define MARSMD_MAX_TYPES 16
+#define MARSMD_MAX_PMF_TERMS 32
+
+/* Pair-term bits of a pair tag (src/Interactions/Nonbonded/Pairwise.h:19-45). */
+#define MARSMD_PAIR_TERM_NONE 0u
+#define MARSMD_PAIR_TERM_COULOMB (1u << 0)
+#define MARSMD_PAIR_TERM_DEBYE_HUCKEL (1u << 1)
+#define MARSMD_PAIR_TERM_ONCK (1u << 2)
+#define MARSMD_PAIR_TERM_GAUSSIAN (1u << 3)
+#define MARSMD_PAIR_TERM_SOFTCORE (1u << 4)
+#define MARSMD_PAIR_TERM_TABULATED (1u << 5)
+#define MARSMD_PAIR_TERM_MASK 0xFFu
+#define MARSMD_PAIR_TABLE_SHIFT 8
+
+/* Integrators (IntegratorType) and grid enums (src/Types/GridTerm.h:26-36). */
+#define MARSMD_INTEGRATOR_BROWNIAN 0
+#define MARSMD_INTEGRATOR_LANGEVIN 1
+#define MARSMD_INTERP_LINEAR 1
+#define MARSMD_INTERP_CUBIC 3
+#define MARSMD_BC_DIRICHLET 0
+#define MARSMD_BC_NEUMANN 1
+#define MARSMD_BC_PERIODIC 2
+
+/* Particle flag FLAG_RB_ATTACHED (src/Objects/DeviceParticle.h): not integrated. */
+#define MARSMD_FLAG_RB_ATTACHED (1u << 2)
+
+/* InteractionForm::Tabulated as stored in the bonded `forms` arrays. */
+#define MARSMD_FORM_TABULATED 1
+
+/* ------------------------------------------------------------------ POD views */
+
+/* Orthorhombic PeriodicBox (src/System/PeriodicBox.h:16-209, 445-461). A zero
+ * periodic flag marks an open axis. Triclinic boxes are not part of this path. */
+typedef struct marsmd_box {
+    float size[3];
+    float origin[3];
+    int32_t periodic[3];
+} marsmd_box;
+
+/* TabulatedPotential (src/Interactions/TabulatedPotential.h:8-13); pot is a device
+ * pointer to `size` knots. */
+typedef struct marsmd_table {
+    const float* pot;
+    float step_inv;
+    uint32_t size;
+    float start;
+    int32_t is_periodic;
+} marsmd_table;
+
+/* ExclusionView (src/Interactions/DeviceExclusions.h:22-75): symmetric CSR over
+ * particles [0, num_particles), plus an optional per-particle rigid-body id. */
+typedef struct marsmd_exclusion_view {
+    const int32_t* offsets;
+    const int32_t* excluded;
+    uint64_t num_particles;
+    const int32_t* rigid_body_id;
+} marsmd_exclusion_view;
+
+/* GridTerm (src/Types/GridTerm.h:55-69). */
+typedef struct marsmd_grid_term {
+    int32_t grid_id;
+    float scale;
+    float scale_slope;
+    int32_t boundary_condition;
+} marsmd_grid_term;
+
+/* BaseGridView<float> (src/Types/BaseGridDevice.h:24-30, 105+). basis and
+ * basis_inv are column-major 3x3 (cols[0] = first basis vector, as Matrix3_t).
+ * Layout: value(ix,iy,iz) = data[iz + iy*nz + ix*ny*nz]. */
+typedef struct marsmd_grid_view {
+    const float* data;
+    float origin[3];
+    float basis[9];
+    float basis_inv[9];
+    uint64_t dims[3];
+    int32_t grid_id;
+    int32_t boundary_condition;
+} marsmd_grid_view;
+
+/* ParticleTypeView (src/Objects/DeviceParticle.h:49-65), held BY VALUE (host
+ * struct, passed to kernels as kernel arguments). */
+typedef struct marsmd_particle_types {
+    int32_t num_types;
+    float mass[MARSMD_MAX_TYPES];
+    float charge[MARSMD_MAX_TYPES];
+    float diffusion[MARSMD_MAX_TYPES][3];
+    float trans_damping[MARSMD_MAX_TYPES][3];
+    int32_t pmf_grid_offset[MARSMD_MAX_TYPES];
+    int32_t pmf_grid_count[MARSMD_MAX_TYPES];
+    marsmd_grid_term pmf_grid_terms[MARSMD_MAX_PMF_TERMS];
+    int32_t force_grid_id[MARSMD_MAX_TYPES][3];
+    float force_grid_scale[MARSMD_MAX_TYPES][3];
+} marsmd_particle_types;
+
+/* ParticleView (src/Objects/DeviceParticle.h:18-28). Vector3 fields are float4
+ * records; flags and attached_rigid_body_id may be NULL. */
+typedef struct marsmd_particle_view {
+    int32_t* id;
+    int32_t* type_id;
+    float* pos;
+    float* mom;
+    float* force_energy;
+    uint32_t* flags;
+    int32_t* attached_rigid_body_id;
+} marsmd_particle_view;
+
+/* Pairlist::Statistics (src/PatchOperation/Pairlist.h:177-199) plus the cell grid
+ * of ZOrderPairlist::select_cell_grid (ZOrderPairlist.cpp:24-81). */
+typedef struct marsmd_pairlist_stats {
+    uint64_t num_particles;
+    uint64_t num_pairs;
+    double average_neighbors_per_particle;
+    uint32_t max_neighbors_per_particle;
+    double build_time_ms;
+    int32_t cell_grid_bits;     /* the reference's choice (select_cell_grid)        */
+    int32_t cell_radii[3];
+    int32_t neighbors_per_cell;
+    int32_t build_grid_bits;    /* the grid this build actually walked              */
+    int32_t build_radii[3];
+    int32_t reference_grid_complete; /* 1: the reference's stencil covers every pair */
+    uint64_t pair_capacity;
+} marsmd_pairlist_stats;
+
+/* ------------------------------------------------------------------ errors */
+
+const char* marsmd_last_error(void);
+/* Library identification: returns a static string "marsmd-hip <version> gfx950". */
+const char* marsmd_version(void);
+
+/* ------------------------------------------------------------------ Z-order sort
+ * Replaces class ZOrderSort (src/PatchOperation/ZOrderKernels/ZOrderSort.h:43-313):
+ * MortonEncodeKernel (ZOrderKernels.h:27-43) + stable radix sort of (key, index)
+ * (ZOrderSort.h:255-294, CUB SortPairs in CUDASort.cu:13-98) + InverseIndexKernel
+ * (ZOrderKernels.h:68-83). */
+typedef struct marsmd_zorder marsmd_zorder;
+
+marsmd_status marsmd_zorder_create(uint64_t max_particles, marsmd_zorder** out);
+marsmd_status marsmd_zorder_destroy(marsmd_zorder* z);
+/* ZOrderSort::sort_particles(positions, n, box_min, box_max); box_min/max host. */
+marsmd_status marsmd_zorder_sort_particles(marsmd_zorder* z, const float* pos4, uint64_t n,
+                                           const float box_min[3], const float box_max[3],
+                                           marsmd_stream stream);
+/* get_morton_codes / get_sorted_indices / get_inverse_indices (device pointers,
+ * valid until the next sort). */
+marsmd_status marsmd_zorder_get_buffers(marsmd_zorder* z, const uint32_t** morton_codes,
+                                        const uint32_t** sorted_indices,
+                                        const uint32_t** inverse_indices);
+/* ZOrderSort::reorder_data<T>: out[i] = in[sorted_indices[i]]; elem_size 4, 8 or 16. */
+marsmd_status marsmd_zorder_reorder_data(marsmd_zorder* z, const void* in, void* out,
+                                         uint64_t n, uint32_t elem_size, marsmd_stream stream);
+/* ZOrderSort::remap_indices (RemapIndicesKernel ZOrderKernels.h:87-101). */
+marsmd_status marsmd_zorder_remap_indices(marsmd_zorder* z, int32_t* indices, uint64_t n,
+                                          marsmd_stream stream);
+/* Stand-alone stable key/index sort used by the above (device keys in, sorted out). */
+marsmd_status marsmd_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in,
+                                      uint32_t* keys_out, uint32_t* vals_out, uint64_t n,
+                                      int32_t key_bits, void* scratch, uint64_t scratch_bytes,
+                                      marsmd_stream stream);
+/* Scratch bytes marsmd_radix_sort_pairs needs for n keys. */
+uint64_t marsmd_radix_sort_scratch_bytes(uint64_t n);
+
+/* ------------------------------------------------------------------ pairlist
+ * Replaces ZOrderPairlist (src/PatchOperation/PairListKernels/ZOrderPairlist.h:27-190,
+ * ZOrderPairlist.cpp:102-411; ZOrderNeighbor.h:27-225) behind Pairlist
+ * (src/PatchOperation/Pairlist.h:56-252) and create_pairlist (Pairlist.cpp:7-31). */
+typedef struct marsmd_pairlist marsmd_pairlist;
+
+marsmd_status marsmd_pairlist_create(uint64_t max_particles, uint64_t max_pairs,
+                                     marsmd_pairlist** out);
+marsmd_status marsmd_pairlist_destroy(marsmd_pairlist* pl);
+/* Pairlist::set_periodic_box (host struct). */
+marsmd_status marsmd_pairlist_set_periodic_box(marsmd_pairlist* pl, const marsmd_box* box);
+/* Pairlist::set_exclusions(ExclusionView) — pointers borrowed until the next build. */
+marsmd_status marsmd_pairlist_set_exclusions(marsmd_pairlist* pl, marsmd_exclusion_view view);
+/* Pairlist::set_max_pairs_ceiling (Pairlist.h:24-35): the regrow cap in pairs. */
+marsmd_status marsmd_pairlist_set_max_pairs_ceiling(marsmd_pairlist* pl, uint64_t ceiling);
+/* ZOrderPairlist::build_pairlist(positions, n, pairlist_cutoff). type_ids may be NULL
+ * (types read as 0). Synchronises once to size the list (as the reference's
+ * pair_count read-back at ZOrderPairlist.cpp:294). */
+marsmd_status marsmd_pairlist_build(marsmd_pairlist* pl, const float* pos4,
+                                    const int32_t* type_ids, uint64_t n, float pairlist_cutoff,
+                                    marsmd_stream stream);
+/* ZOrderPairlist::needs_update (ZOrderPairlist.cpp:184-190): max displacement since
+ * the last build > skin/2. Host result; synchronises. */
+marsmd_status marsmd_pairlist_needs_update(marsmd_pairlist* pl, const float* pos4, uint64_t n,
+                                           float skin, int32_t* needs, float* max_disp,
+                                           marsmd_stream stream);
+/* get_num_pairs (host). */
+marsmd_status marsmd_pairlist_get_num_pairs(marsmd_pairlist* pl, uint64_t* num_pairs);
+/* get_neighbor_pairs(): int2 pairs in the caller's indices, x < y. The list is
+ * materialised on first request after a build. *pairs is a device pointer. */
+marsmd_status marsmd_pairlist_get_neighbor_pairs(marsmd_pairlist* pl, const int32_t** pairs,
+                                                 marsmd_stream stream);
+/* get_statistics(). */
+marsmd_status marsmd_pairlist_get_statistics(marsmd_pairlist* pl, marsmd_pairlist_stats* out);
+/* The sorter of the last build (get_morton_codes etc. of ZOrderPairlist's sorter_). */
+marsmd_status marsmd_pairlist_get_sorter(marsmd_pairlist* pl, marsmd_zorder** out);
+/* Cell tables of the reference grid for the last build (BuildCellRangesKernel,
+ * BuildCellNeighborsKernel): device pointers + sizes. Built on request. */
+marsmd_status marsmd_pairlist_get_reference_cells(marsmd_pairlist* pl, const uint32_t** begin,
+                                                  const uint32_t** end,
+                                                  const uint32_t** neighbors,
+                                                  uint64_t* num_cells, int32_t* slots,
+                                                  marsmd_stream stream);
+
+/* ------------------------------------------------------------------ launchers */
+
+/* launch_resolve_pair_tables (src/Interactions/NonBondedInteraction.h:205-261;
+ * ResolvePairTableKernel Pairwise.h:104-120). */
+marsmd_status marsmd_launch_resolve_pair_tables(const int32_t* pairs, const int32_t* type_ids,
+                                                const uint32_t* term_matrix,
+                                                uint64_t num_types, uint32_t* pair_tag,
+                                                uint64_t num_pairs, marsmd_stream stream);
+/* launch_pairwise_nonbonded (NonBondedInteraction.h:262-296; PairNonbondedComputer
+ * Pairwise.h:184-305). tables: device array of marsmd_table. pair_tag may be NULL
+ * (every pair takes enabled_terms). Only the TABULATED term is evaluated on this
+ * path; other enabled bits are rejected with MARSMD_ERR_VALUE. */
+marsmd_status marsmd_launch_pairwise_nonbonded(const int32_t* pairs, marsmd_particle_view view,
+                                               const uint32_t* pair_tag,
+                                               const marsmd_table* tables, marsmd_box box,
+                                               int32_t get_energy, uint64_t num_pairs,
+                                               float cutoff_squared, uint32_t enabled_terms,
+                                               marsmd_stream stream);
+/* launch_tabulated_bonds (src/Interactions/DeviceBondedInteraction.h:495-518;
+ * TabulatedBondComputer BondComputer.h:136-192). bonds: int2 array. */
+marsmd_status marsmd_launch_tabulated_bonds(const int32_t* bonds, const float* pos4,
+                                            float* force_energy, const marsmd_table* tables,
+                                            const int32_t* table_indices, const int32_t* forms,
+                                            marsmd_box box, int32_t get_energy,
+                                            uint64_t num_bonds, marsmd_stream stream);
+/* launch_tabulated_angles (DeviceBondedInteraction.h:520-546; TabulatedAngleComputer
+ * BondComputer.h:204-269). angles: 16-byte int records {i, j, k, pad}. */
+marsmd_status marsmd_launch_tabulated_angles(const int32_t* angles, const float* pos4,
+                                             float* force_energy, const marsmd_table* tables,
+                                             const int32_t* table_indices, const int32_t* forms,
+                                             marsmd_box box, int32_t get_energy,
+                                             uint64_t num_angles, marsmd_stream stream);
+/* launch_BD (src/PatchOperation/Integrator.h:25-54; BDIntegrate BD.h:44-92). grids:
+ * device array of marsmd_grid_view (NULL = none). */
+marsmd_status marsmd_launch_bd(marsmd_particle_view view, const marsmd_particle_types* types,
+                               float timestep, uint64_t current_step, float kT,
+                               uint64_t num_particles, marsmd_box box, uint64_t base_seed,
+                               uint32_t base_ctr, const marsmd_grid_view* grids,
+                               const float electric_field[3], int32_t interpolation,
+                               marsmd_stream stream);
+/* launch_BAOAB (Integrator.h:56-85; BAOABIntegrate BAOAB.h:46-120). */
+marsmd_status marsmd_launch_baoab(marsmd_particle_view view, const marsmd_particle_types* types,
+                                  marsmd_box box, float timestep, uint64_t current_step, float kT,
+                                  uint64_t num_particles, uint64_t base_seed, uint32_t base_ctr,
+                                  marsmd_stream stream);
+/* launch_BAOAB_LastUpdate (Integrator.h:97-123; BAOAB.h:151-172). */
+marsmd_status marsmd_launch_baoab_last_update(marsmd_particle_view view,
+                                              const marsmd_particle_types* types,
+                                              float timestep, uint64_t num_particles,
+                                              marsmd_stream stream);
+/* launch_PMF (Integrator.h:127-144; ComputePMFKernel Pmf.h:126-152). */
+marsmd_status marsmd_launch_pmf(marsmd_particle_view view, const marsmd_particle_types* types,
+                                uint64_t num_particles, const marsmd_grid_view* grids,
+                                const float electric_field[3], int32_t interpolation,
+                                marsmd_stream stream);
+/* Raw Philox4x32-10 draws (openrand::Philox, src/PatchOperation/Random/philox.h:60-139)
+ * for counters ctr0 = base_ctr + i, i in [0, n): out is n uint4 records. Used to pin
+ * the device RNG bit-for-bit. */
+marsmd_status marsmd_philox_draw(uint64_t seed, uint32_t base_ctr, uint32_t global_seed,
+                                 uint32_t ctr1, uint32_t draws, uint64_t n, uint32_t* out,
+                                 marsmd_stream stream);
+
+/* ------------------------------------------------------------------ Patch
+ * Replaces class Patch (src/PatchOperation/Patch.h:48-645, Patch.cpp:43-480) for one
+ * patch on one GPU. The patch owns its particle state in HBM in its own (Morton)
+ * layout; copy_particles_{from,to}_host translate to the caller's canonical order. */
+typedef struct marsmd_patch marsmd_patch;
+
+typedef struct marsmd_patch_desc {
+    uint64_t capacity;           /* Patch(patch_t, capacity, ...)                     */
+    uint64_t patch_id;           /* patch_t, enters the seed mix (Patch.cpp:252-254)   */
+    marsmd_box box;
+    marsmd_particle_types types;
+    /* nonbonded: tag matrix [num_types^2] (host) and tables (host knots) */
+    const uint32_t* term_matrix;
+    uint32_t enabled_terms;
+    int32_t num_nb_tables;
+    const marsmd_table* nb_tables; /* host records; .pot host pointers to knots */
+    /* bonded topology, host arrays in canonical indices (BondedInteractions) */
+    uint64_t num_bonds;
+    const int32_t* bonds;          /* 2 per bond */
+    const int32_t* bond_table;     /* table index per bond */
+    uint64_t num_angles;
+    const int32_t* angles;         /* 3 per angle */
+    const int32_t* angle_table;
+    int32_t num_bond_tables;
+    const marsmd_table* bond_tables;  /* host */
+    int32_t num_angle_tables;
+    const marsmd_table* angle_tables; /* host, is_periodic set, radians */
+    uint64_t num_exclusions;
+    const int32_t* exclusions;     /* 2 per exclusion (host) */
+    /* grids (host data, copied to HBM) */
+    int32_t num_grids;
+    const marsmd_grid_view* grids; /* host records; .data host pointers */
+} marsmd_patch_desc;
+
+marsmd_status marsmd_patch_create(const marsmd_patch_desc* desc, marsmd_patch** out);
+marsmd_status marsmd_patch_destroy(marsmd_patch* p);
+/* Patch::set_base_seed. */
+marsmd_status marsmd_patch_set_base_seed(marsmd_patch* p, uint64_t seed);
+/* Patch::copy_particles_from_host (host arrays, canonical order; mom may be NULL). */
+marsmd_status marsmd_patch_copy_particles_from_host(marsmd_patch* p, const float* pos4,
+                                                    const float* mom4, const int32_t* type_ids,
+                                                    uint64_t n);
+/* Patch::copy_particles_to_host (host arrays, canonical order; any may be NULL). The
+ * canonical order changes at each reorder_particles, as the reference's SoA does;
+ * ids returns each slot's global id (HostParticleData::global_id). */
+marsmd_status marsmd_patch_copy_particles_to_host(marsmd_patch* p, float* pos4, float* mom4,
+                                                  float* force_energy4, int32_t* type_ids,
+                                                  int32_t* ids, uint64_t n);
+/* Patch::calculate_nonbonded_forces (Patch.cpp:43-154): clear forces, rebuild policy,
+ * pairwise force. interpolation/electric_field are stashed for integrate_motion. */
+marsmd_status marsmd_patch_calculate_nonbonded_forces(marsmd_patch* p, float pairlist_cutoff,
+                                                      float interaction_cutoff, uint64_t step,
+                                                      uint64_t rebuild_period,
+                                                      const float electric_field[3],
+                                                      int32_t interpolation,
+                                                      int32_t compute_energy,
+                                                      marsmd_stream stream);
+/* Patch::calculate_bonded_forces (Patch.cpp:156-236). */
+marsmd_status marsmd_patch_calculate_bonded_forces(marsmd_patch* p, int32_t compute_energy,
+                                                   marsmd_stream stream);
+/* Patch::integrate_motion (Patch.cpp:238-352). */
+marsmd_status marsmd_patch_integrate_motion(marsmd_patch* p, float dt, float kT,
+                                            int32_t integrator, uint64_t step,
+                                            marsmd_stream stream);
+/* Patch::finish_deferred_kick (Patch.cpp:354-373). */
+marsmd_status marsmd_patch_finish_deferred_kick(marsmd_patch* p, float dt, marsmd_stream stream);
+/* Patch::reorder_particles (Patch.cpp:456-480) via ParticleReorderManager. */
+marsmd_status marsmd_patch_reorder_particles(marsmd_patch* p, marsmd_stream stream);
+/* Patch::get_pairlist(). */
+marsmd_status marsmd_patch_get_pairlist(marsmd_patch* p, marsmd_pairlist** out);
+
+/* The per-step loop of SimManager::run (SimManager.cpp:271-296, 337-504) for steps
+ * first_step+1 .. first_step+num_steps, without output: reorder cadence
+ * (ReorderManager.cpp:8-25), rebuild policy (Patch.cpp:85-121), forces and
+ * integration, with force + PMF + integrator fused into one kernel per step. */
+typedef struct marsmd_run_params {
+    float pairlist_cutoff;
+    float cutoff;
+    uint64_t rebuild_period;   /* decompPeriod */
+    int64_t reorder_period;    /* reorderPeriod (<= 0: falls back to rebuild_period) */
+    float timestep;
+    float kT;
+    int32_t integrator;
+    int32_t interpolation;
+    float electric_field[3];
+    uint64_t energy_period;    /* outputEnergyPeriod (0 = never) */
+} marsmd_run_params;
+
+marsmd_status marsmd_patch_run(marsmd_patch* p, const marsmd_run_params* params,
+                               uint64_t first_step, uint64_t num_steps, marsmd_stream stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* MARSMD_H_ */

@@ -1,0 +1,395 @@
+// capi.cpp — the extern "C" boundary declared in include/marsmd.h. Every entry
+// point converts exceptions into a marsmd_status and a thread-local message.
+#include "launchers.hpp"
+#include "pairlist.hpp"
+#include "patch.hpp"
+
+#include <cstring>
+#include <string>
+
+struct marsmd_zorder {
+	mmd::ZSort impl;
+	explicit marsmd_zorder(uint64_t n) : impl(n) {}
+};
+struct marsmd_pairlist {
+	mmd::PairlistImpl* impl;
+	bool owned;
+};
+struct marsmd_patch {
+	mmd::PatchImpl impl;
+	marsmd_pairlist pl_view{};
+	explicit marsmd_patch(const marsmd_patch_desc& d) : impl(d) {}
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template<class F>
+marsmd_status guard(F&& f) {
+	try {
+		f();
+		return MARSMD_OK;
+	} catch (const mmd::Error& e) {
+		g_last_error = e.what();
+		return e.code;
+	} catch (const std::exception& e) {
+		g_last_error = e.what();
+		return MARSMD_ERR_RUNTIME;
+	} catch (...) {
+		g_last_error = "unknown error";
+		return MARSMD_ERR_RUNTIME;
+	}
+}
+
+hipStream_t st(marsmd_stream s) { return static_cast<hipStream_t>(s); }
+
+void need(const void* p, const char* what) {
+	if (!p)
+		mmd::raise(MARSMD_ERR_VALUE, std::string(what) + " is NULL");
+}
+
+} // namespace
+
+extern "C" {
+
+const char* marsmd_last_error(void) { return g_last_error.c_str(); }
+const char* marsmd_version(void) { return "marsmd-hip 0.1.0 gfx950"; }
+
+// ---------------------------------------------------------------- Z-order sort
+marsmd_status marsmd_zorder_create(uint64_t max_particles, marsmd_zorder** out) {
+	return guard([&] {
+		need(out, "out");
+		*out = new marsmd_zorder(max_particles);
+	});
+}
+marsmd_status marsmd_zorder_destroy(marsmd_zorder* z) {
+	return guard([&] { delete z; });
+}
+marsmd_status marsmd_zorder_sort_particles(marsmd_zorder* z, const float* pos4, uint64_t n, const float box_min[3],
+										   const float box_max[3], marsmd_stream stream) {
+	return guard([&] {
+		need(z, "sorter");
+		need(box_min, "box_min");
+		need(box_max, "box_max");
+		if (n)
+			need(pos4, "positions");
+		z->impl.sort(reinterpret_cast<const float4*>(pos4), n, box_min, box_max, nullptr, false, st(stream));
+	});
+}
+marsmd_status marsmd_zorder_get_buffers(marsmd_zorder* z, const uint32_t** codes, const uint32_t** sorted,
+										const uint32_t** inverse) {
+	return guard([&] {
+		need(z, "sorter");
+		if (codes)
+			*codes = z->impl.codes.data();
+		if (sorted)
+			*sorted = z->impl.sorted_idx.data();
+		if (inverse)
+			*inverse = z->impl.inverse.data();
+	});
+}
+marsmd_status marsmd_zorder_reorder_data(marsmd_zorder* z, const void* in, void* out, uint64_t n, uint32_t elem,
+										 marsmd_stream stream) {
+	return guard([&] {
+		need(z, "sorter");
+		z->impl.reorder_data(in, out, n, elem, st(stream));
+	});
+}
+marsmd_status marsmd_zorder_remap_indices(marsmd_zorder* z, int32_t* idx, uint64_t n, marsmd_stream stream) {
+	return guard([&] {
+		need(z, "sorter");
+		z->impl.remap_indices(idx, n, st(stream));
+	});
+}
+marsmd_status marsmd_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+									  uint32_t* vals_out, uint64_t n, int32_t key_bits, void* scratch,
+									  uint64_t scratch_bytes, marsmd_stream stream) {
+	return guard([&] {
+		mmd::radix_sort_pairs(keys_in, vals_in, keys_out, vals_out, n, key_bits, scratch, scratch_bytes, st(stream));
+	});
+}
+uint64_t marsmd_radix_sort_scratch_bytes(uint64_t n) { return mmd::radix_sort_scratch_bytes(n); }
+
+// ---------------------------------------------------------------- pairlist
+marsmd_status marsmd_pairlist_create(uint64_t max_particles, uint64_t max_pairs, marsmd_pairlist** out) {
+	return guard([&] {
+		need(out, "out");
+		auto* p = new marsmd_pairlist;
+		p->impl = new mmd::PairlistImpl(max_particles, max_pairs);
+		p->owned = true;
+		*out = p;
+	});
+}
+marsmd_status marsmd_pairlist_destroy(marsmd_pairlist* pl) {
+	return guard([&] {
+		if (!pl)
+			return;
+		if (pl->owned) {
+			delete pl->impl;
+			delete pl;
+		}
+	});
+}
+marsmd_status marsmd_pairlist_set_periodic_box(marsmd_pairlist* pl, const marsmd_box* box) {
+	return guard([&] {
+		need(pl, "pairlist");
+		need(box, "box");
+		pl->impl->set_box(*box);
+	});
+}
+marsmd_status marsmd_pairlist_set_exclusions(marsmd_pairlist* pl, marsmd_exclusion_view view) {
+	return guard([&] {
+		need(pl, "pairlist");
+		pl->impl->excl = view;
+	});
+}
+marsmd_status marsmd_pairlist_set_max_pairs_ceiling(marsmd_pairlist* pl, uint64_t ceiling) {
+	return guard([&] {
+		need(pl, "pairlist");
+		pl->impl->max_pairs_ceiling = ceiling;
+		pl->impl->ref_max_pairs = std::min<uint64_t>(pl->impl->ref_max_pairs, ceiling);
+	});
+}
+marsmd_status marsmd_pairlist_build(marsmd_pairlist* pl, const float* pos4, const int32_t* type_ids, uint64_t n,
+									float pairlist_cutoff, marsmd_stream stream) {
+	return guard([&] {
+		need(pl, "pairlist");
+		if (n)
+			need(pos4, "positions");
+		pl->impl->build(reinterpret_cast<const float4*>(pos4), nullptr, false, type_ids, n, pairlist_cutoff,
+						st(stream));
+	});
+}
+marsmd_status marsmd_pairlist_needs_update(marsmd_pairlist* pl, const float* pos4, uint64_t n, float skin,
+										   int32_t* needs, float* max_disp, marsmd_stream stream) {
+	return guard([&] {
+		need(pl, "pairlist");
+		if (!pl->impl->built || n != pl->impl->n)
+			mmd::raise(MARSMD_ERR_RUNTIME, "needs_update: no pairlist built for this particle count");
+		const float d = pl->impl->max_displacement(reinterpret_cast<const float4*>(pos4),
+												   pl->impl->sorter.inverse.data(), n, st(stream));
+		if (needs)
+			*needs = d > skin * 0.5f ? 1 : 0;
+		if (max_disp)
+			*max_disp = d;
+	});
+}
+marsmd_status marsmd_pairlist_get_num_pairs(marsmd_pairlist* pl, uint64_t* num_pairs) {
+	return guard([&] {
+		need(pl, "pairlist");
+		need(num_pairs, "num_pairs");
+		*num_pairs = pl->impl->num_pairs();
+	});
+}
+marsmd_status marsmd_pairlist_get_neighbor_pairs(marsmd_pairlist* pl, const int32_t** pairs, marsmd_stream stream) {
+	return guard([&] {
+		need(pl, "pairlist");
+		need(pairs, "pairs");
+		*pairs = pl->impl->neighbor_pairs(st(stream));
+	});
+}
+marsmd_status marsmd_pairlist_get_statistics(marsmd_pairlist* pl, marsmd_pairlist_stats* out) {
+	return guard([&] {
+		need(pl, "pairlist");
+		need(out, "out");
+		*out = pl->impl->stats();
+	});
+}
+marsmd_status marsmd_pairlist_get_sorter(marsmd_pairlist* pl, marsmd_zorder** out) {
+	return guard([&] {
+		need(pl, "pairlist");
+		need(out, "out");
+		// A non-owning view: the sorter object lives inside the pairlist.
+		*out = reinterpret_cast<marsmd_zorder*>(&pl->impl->sorter);
+	});
+}
+marsmd_status marsmd_pairlist_get_reference_cells(marsmd_pairlist* pl, const uint32_t** begin, const uint32_t** end,
+												  const uint32_t** neighbors, uint64_t* num_cells, int32_t* slots,
+												  marsmd_stream stream) {
+	return guard([&] {
+		need(pl, "pairlist");
+		uint64_t nc = 0;
+		pl->impl->reference_cells(st(stream), &nc);
+		if (begin)
+			*begin = pl->impl->ref_begin.data();
+		if (end)
+			*end = pl->impl->ref_end.data();
+		if (neighbors)
+			*neighbors = pl->impl->ref_neighbors.data();
+		if (num_cells)
+			*num_cells = nc;
+		if (slots)
+			*slots = int32_t(pl->impl->ref_grid.slots);
+	});
+}
+
+// ---------------------------------------------------------------- launchers
+marsmd_status marsmd_launch_resolve_pair_tables(const int32_t* pairs, const int32_t* type_ids,
+												const uint32_t* term_matrix, uint64_t num_types, uint32_t* pair_tag,
+												uint64_t num_pairs, marsmd_stream stream) {
+	return guard([&] {
+		mmd::launch_resolve_pair_tables(pairs, type_ids, term_matrix, num_types, pair_tag, num_pairs, st(stream));
+	});
+}
+marsmd_status marsmd_launch_pairwise_nonbonded(const int32_t* pairs, marsmd_particle_view view,
+											   const uint32_t* pair_tag, const marsmd_table* tables, marsmd_box box,
+											   int32_t get_energy, uint64_t num_pairs, float cutoff_squared,
+											   uint32_t enabled_terms, marsmd_stream stream) {
+	return guard([&] {
+		mmd::launch_pairwise_nonbonded(pairs, view, pair_tag, tables, box, get_energy != 0, num_pairs, cutoff_squared,
+									   enabled_terms, st(stream));
+	});
+}
+marsmd_status marsmd_launch_tabulated_bonds(const int32_t* bonds, const float* pos4, float* fe,
+											const marsmd_table* tables, const int32_t* table_indices,
+											const int32_t* forms, marsmd_box box, int32_t get_energy,
+											uint64_t num_bonds, marsmd_stream stream) {
+	return guard([&] {
+		mmd::launch_tabulated_bonds(bonds, pos4, fe, tables, table_indices, forms, box, get_energy != 0, num_bonds,
+									st(stream));
+	});
+}
+marsmd_status marsmd_launch_tabulated_angles(const int32_t* angles, const float* pos4, float* fe,
+											 const marsmd_table* tables, const int32_t* table_indices,
+											 const int32_t* forms, marsmd_box box, int32_t get_energy,
+											 uint64_t num_angles, marsmd_stream stream) {
+	return guard([&] {
+		mmd::launch_tabulated_angles(angles, pos4, fe, tables, table_indices, forms, box, get_energy != 0, num_angles,
+									 st(stream));
+	});
+}
+marsmd_status marsmd_launch_bd(marsmd_particle_view view, const marsmd_particle_types* types, float timestep,
+							   uint64_t current_step, float kT, uint64_t num_particles, marsmd_box box,
+							   uint64_t base_seed, uint32_t base_ctr, const marsmd_grid_view* grids,
+							   const float electric_field[3], int32_t interpolation, marsmd_stream stream) {
+	return guard([&] {
+		need(types, "types");
+		const float zero[3] = {0, 0, 0};
+		mmd::launch_bd(view, *types, timestep, current_step, kT, num_particles, box, base_seed, base_ctr, grids,
+					   electric_field ? electric_field : zero, interpolation, st(stream));
+	});
+}
+marsmd_status marsmd_launch_baoab(marsmd_particle_view view, const marsmd_particle_types* types, marsmd_box box,
+								  float timestep, uint64_t current_step, float kT, uint64_t num_particles,
+								  uint64_t base_seed, uint32_t base_ctr, marsmd_stream stream) {
+	return guard([&] {
+		need(types, "types");
+		mmd::launch_baoab(view, *types, box, timestep, current_step, kT, num_particles, base_seed, base_ctr,
+						  st(stream));
+	});
+}
+marsmd_status marsmd_launch_baoab_last_update(marsmd_particle_view view, const marsmd_particle_types* types,
+											  float timestep, uint64_t num_particles, marsmd_stream stream) {
+	return guard([&] {
+		(void)types;
+		mmd::launch_baoab_last_update(view, timestep, num_particles, st(stream));
+	});
+}
+marsmd_status marsmd_launch_pmf(marsmd_particle_view view, const marsmd_particle_types* types, uint64_t num_particles,
+								const marsmd_grid_view* grids, const float electric_field[3], int32_t interpolation,
+								marsmd_stream stream) {
+	return guard([&] {
+		need(types, "types");
+		const float zero[3] = {0, 0, 0};
+		mmd::launch_pmf(view, *types, num_particles, grids, electric_field ? electric_field : zero, interpolation,
+						st(stream));
+	});
+}
+marsmd_status marsmd_philox_draw(uint64_t seed, uint32_t base_ctr, uint32_t global_seed, uint32_t ctr1, uint32_t draws,
+								 uint64_t n, uint32_t* out, marsmd_stream stream) {
+	return guard([&] { mmd::philox_draw(seed, base_ctr, global_seed, ctr1, draws, n, out, st(stream)); });
+}
+
+// ---------------------------------------------------------------- Patch
+marsmd_status marsmd_patch_create(const marsmd_patch_desc* desc, marsmd_patch** out) {
+	return guard([&] {
+		need(desc, "desc");
+		need(out, "out");
+		auto* p = new marsmd_patch(*desc);
+		p->pl_view.impl = p->impl.pl.get();
+		p->pl_view.owned = false;
+		*out = p;
+	});
+}
+marsmd_status marsmd_patch_destroy(marsmd_patch* p) {
+	return guard([&] { delete p; });
+}
+marsmd_status marsmd_patch_set_base_seed(marsmd_patch* p, uint64_t seed) {
+	return guard([&] {
+		need(p, "patch");
+		p->impl.base_seed = seed;
+	});
+}
+marsmd_status marsmd_patch_copy_particles_from_host(marsmd_patch* p, const float* pos4, const float* mom4,
+													const int32_t* type_ids, uint64_t n) {
+	return guard([&] {
+		need(p, "patch");
+		if (n) {
+			need(pos4, "positions");
+			need(type_ids, "type_ids");
+		}
+		p->impl.copy_from_host(pos4, mom4, type_ids, n);
+	});
+}
+marsmd_status marsmd_patch_copy_particles_to_host(marsmd_patch* p, float* pos4, float* mom4, float* force_energy4,
+												  int32_t* type_ids, int32_t* ids, uint64_t n) {
+	return guard([&] {
+		need(p, "patch");
+		p->impl.copy_to_host(pos4, mom4, force_energy4, type_ids, ids, n);
+	});
+}
+marsmd_status marsmd_patch_calculate_nonbonded_forces(marsmd_patch* p, float pairlist_cutoff, float interaction_cutoff,
+													  uint64_t step, uint64_t rebuild_period,
+													  const float electric_field[3], int32_t interpolation,
+													  int32_t compute_energy, marsmd_stream stream) {
+	return guard([&] {
+		need(p, "patch");
+		const float zero[3] = {0, 0, 0};
+		p->impl.calculate_nonbonded(pairlist_cutoff, interaction_cutoff, step, rebuild_period,
+									electric_field ? electric_field : zero, interpolation, compute_energy != 0,
+									st(stream));
+	});
+}
+marsmd_status marsmd_patch_calculate_bonded_forces(marsmd_patch* p, int32_t compute_energy, marsmd_stream stream) {
+	return guard([&] {
+		need(p, "patch");
+		p->impl.calculate_bonded(compute_energy != 0, st(stream));
+	});
+}
+marsmd_status marsmd_patch_integrate_motion(marsmd_patch* p, float dt, float kT, int32_t integrator, uint64_t step,
+											marsmd_stream stream) {
+	return guard([&] {
+		need(p, "patch");
+		p->impl.integrate(dt, kT, integrator, step, st(stream));
+	});
+}
+marsmd_status marsmd_patch_finish_deferred_kick(marsmd_patch* p, float dt, marsmd_stream stream) {
+	return guard([&] {
+		need(p, "patch");
+		p->impl.finish_deferred_kick(dt, st(stream));
+	});
+}
+marsmd_status marsmd_patch_reorder_particles(marsmd_patch* p, marsmd_stream stream) {
+	return guard([&] {
+		need(p, "patch");
+		p->impl.reorder(st(stream));
+	});
+}
+marsmd_status marsmd_patch_get_pairlist(marsmd_patch* p, marsmd_pairlist** out) {
+	return guard([&] {
+		need(p, "patch");
+		need(out, "out");
+		*out = &p->pl_view;
+	});
+}
+marsmd_status marsmd_patch_run(marsmd_patch* p, const marsmd_run_params* params, uint64_t first_step,
+							   uint64_t num_steps, marsmd_stream stream) {
+	return guard([&] {
+		need(p, "patch");
+		need(params, "params");
+		p->impl.run(*params, first_step, num_steps, st(stream));
+	});
+}
+
+} // extern "C"

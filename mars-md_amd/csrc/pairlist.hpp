@@ -1,0 +1,94 @@
+// pairlist.hpp — host classes behind marsmd_zorder / marsmd_pairlist.
+#pragma once
+
+#include "internal.hpp"
+
+namespace mmd {
+
+// Reference cell grid of ZOrderPairlist::select_cell_grid (ZOrderPairlist.cpp:24-81).
+struct CellGrid {
+	int m = 0;
+	int r[3] = {1, 1, 1};
+	long long slots = 1;
+};
+CellGrid select_cell_grid(float cutoff, uint64_t n, const float extent[3], uint64_t max_pairs);
+
+// ZOrderSort (src/PatchOperation/ZOrderKernels/ZOrderSort.h:43-313).
+struct ZSort {
+	uint64_t max_particles = 0;
+	uint64_t n = 0;
+	DevArray<uint32_t> keys_unsorted, vals_unsorted; // canonical-order keys and iota
+	DevArray<uint32_t> codes, sorted_idx, inverse;    // results
+	DevArray<uint8_t> scratch;
+	DevArray<uint32_t> outside_flag; // set when a coordinate was clamped by the encode
+	explicit ZSort(uint64_t max_n);
+	void ensure(uint64_t n);
+	// Encode positions (slot order) into canonical-order keys and sort them.
+	// canon_from_meta: the canonical index of slot s is meta_canon(pos[s].w);
+	// otherwise it is s. domain_dev (6 floats: min xyz, max xyz) overrides bmin/bmax
+	// when non-null.
+	void sort(const float4* pos, uint64_t n, const float bmin[3], const float bmax[3],
+			  const float* domain_dev, bool canon_from_meta, hipStream_t s);
+	// ReorderDataKernel<T> (ZOrderKernels.h:48-63).
+	void reorder_data(const void* in, void* out, uint64_t n, uint32_t elem, hipStream_t s) const;
+	// RemapIndicesKernel (ZOrderKernels.h:87-101).
+	void remap_indices(int32_t* idx, uint64_t count, hipStream_t s) const;
+};
+
+struct PairlistImpl {
+	uint64_t max_particles = 0;
+	uint64_t ref_max_pairs = 0;     // Pairlist::max_pairs_ bookkeeping (grows x1.25)
+	uint64_t max_pairs_ceiling = ~uint64_t(0);
+	marsmd_box box{};
+	marsmd_exclusion_view excl{};
+	ZSort sorter;
+	uint64_t n = 0;
+	float cutoff = 0.0f, cutoff_squared = 0.0f;
+	bool built = false;
+	// sorted positions (meta in .w) of the last build = the displacement snapshot
+	DevArray<float4> spos;
+	// build-grid cell ranges
+	DevArray<uint32_t> cell_begin, cell_end;
+	// CSR full neighbour list over sorted indices
+	DevArray<uint32_t> counts, offsets, nbr;
+	uint64_t nbr_entries = 0; // = 2 * num_pairs
+	DevArray<uint8_t> scan_scratch;
+	// lazily materialised int2 pairs (canonical, x < y)
+	DevArray<int32_t> pairs;
+	DevArray<uint32_t> half_counts, half_offsets;
+	bool pairs_valid = false;
+	// bookkeeping
+	CellGrid ref_grid, build_grid;
+	bool ref_complete = false;
+	int build_m_override = -1;
+	float domain_min[3]{}, domain_max[3]{};
+	double last_build_ms = 0.0;
+	uint32_t max_neighbors = 0;
+	// misc device scalars: [0] outside flag, [1] max displacement^2 bits, [2] max count
+	DevArray<uint32_t> dev_scalars;
+	DevArray<float> dev_domain;
+	DevArray<uint32_t> ref_begin, ref_end, ref_neighbors;
+
+	PairlistImpl(uint64_t max_particles_, uint64_t max_pairs_ceiling_);
+	void set_box(const marsmd_box& b) { box = b; }
+	// Build from positions in slot order. slot_of_canon maps canonical index -> slot
+	// (NULL = identity); canon_from_meta takes canonical indices (and types) from
+	// pos[s].w, else from the slot index and `types`.
+	void build(const float4* pos, const uint32_t* slot_of_canon, bool canon_from_meta,
+			   const int32_t* types, uint64_t n, float pairlist_cutoff, hipStream_t s);
+	// max |min_image(pos - snapshot)|^2 over particles; pos in slot order, mapped to
+	// sorted slots by `sorted_of_slot` (NULL = pos already in sorted order).
+	float max_displacement(const float4* pos, const uint32_t* sorted_of_slot, uint64_t n,
+						   hipStream_t s);
+	uint64_t num_pairs() const { return nbr_entries / 2; }
+	const int32_t* neighbor_pairs(hipStream_t s);
+	void reference_cells(hipStream_t s, uint64_t* num_cells);
+	marsmd_pairlist_stats stats() const;
+
+  private:
+	void choose_grids(uint64_t n);
+	void walk(bool fill, bool use_ref_grid, hipStream_t s);
+	void ensure(uint64_t n);
+};
+
+} // namespace mmd

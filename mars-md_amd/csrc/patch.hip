@@ -1,0 +1,899 @@
+// patch.hip — the Patch of the reference (src/PatchOperation/Patch.cpp:43-480) on one
+// MI355X: particle state resident in HBM in Morton (sorted) slot order, the
+// rebuild / reorder policy on the host, and per-particle kernels.
+//
+// Layout in HBM (slot order = sorted order of the last pairlist build):
+//   pos[2]  float4 {x, y, z, meta}; meta = type << 24 | canonical index. Two
+//           buffers so the fused step reads step-n positions while writing n+1.
+//   mom     float4 {px, py, pz, 0}            (Langevin)
+//   force   float4 {fx, fy, fz, energy}       (ForceEnergy; bonded scatter target)
+//   ids     int32 global id per slot; slot_of_canon: canonical index -> slot.
+// The canonical index is the reference's slot index: it is the Philox counter of a
+// particle and the index space of bonds, exclusions and exported arrays, and it is
+// relabelled at each reorder exactly as Patch::reorder_particles permutes the SoA.
+//
+// Kernels:
+//   k_nb_forces      nonbonded force from the CSR full list, written (not added)
+//                    to `force` — calculate_nonbonded_forces.
+//   k_integrate_*    BD / PMF / BAOAB last-update / BAOAB on slot order, Philox
+//                    counter = canonical index — integrate_motion.
+//   k_step<...>      nonbonded force + bonded force (read from `force`) + PMF +
+//                    integrator in one pass; positions double-buffered. Used by run().
+// The per-pair term is evaluated with the reference's operation order (canonical
+// orientation: r = wrap_diff(pos[max canon] - pos[min canon])), so it is bitwise
+// the term the reference's PairNonbondedComputer adds; only the summation order of
+// a particle's terms differs (fixed here: its neighbour-list order).
+#include "launchers.hpp"
+#include "pairlist.hpp"
+#include "patch.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace mmd {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxTables = 32;
+constexpr int kLdsKnots = 15360; // 60 KB of table knots staged in LDS
+
+struct NbTables {
+	int num_tables;
+	int num_types;
+	uint32_t enabled;
+	int lds; // knots staged in LDS
+	int total_knots;
+	float start[kMaxTables];
+	float step_inv[kMaxTables];
+	int size[kMaxTables];
+	int offset[kMaxTables];
+	uint32_t tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+};
+
+struct StepParams {
+	DevBox box;
+	float cutoff2;
+	float dt;
+	float kT;
+	uint64_t seed;
+	uint32_t ctr;
+	uint32_t step;
+	int scheme;
+	int has_grids;
+	int deferred;
+	float ef[3];
+};
+
+__device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
+
+// Sum of the nonbonded pair terms on slot t from its neighbour list.
+__device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const float4* __restrict__ pos,
+										   const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ nbr,
+										   const NbTables& T, const float* __restrict__ knots, const uint32_t* tagm,
+										   const DevBox& box, float cutoff2, bool energy) {
+	const uint32_t mt = fb(p.w);
+	const uint32_t ct = meta_canon(mt);
+	const uint32_t tt = meta_type(mt);
+	float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
+	const uint32_t k0 = offsets[t], k1 = offsets[t + 1];
+	for (uint32_t k = k0; k < k1; ++k) {
+		const uint32_t j = nbr[k];
+		const float4 q = pos[j];
+		const uint32_t mj = fb(q.w);
+		const uint32_t cj = meta_canon(mj);
+		const uint32_t tj = meta_type(mj);
+		const bool t_is_x = ct < cj;
+		const float4 px = t_is_x ? p : q;
+		const float4 py = t_is_x ? q : p;
+		const uint32_t tx = t_is_x ? tt : tj;
+		const uint32_t ty = t_is_x ? tj : tt;
+		const uint32_t tag = tagm[tx * uint32_t(T.num_types) + ty];
+		if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
+			continue;
+		const float3 r = wrap_diff(box, sub3(py, px));
+		const float d = sqrtf(len2(r));
+		float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+		if (d > 1e-6f) {
+			const float inv = 1.0f / d;
+			ux = inv * r.x;
+			uy = inv * r.y;
+			uz = inv * r.z;
+		}
+		if (d < 1e-6f)
+			continue;
+		if (cutoff2 > 0.0f && d * d > cutoff2)
+			continue;
+		const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
+		const FE v = table_compute(d, knots + T.offset[ti], T.size[ti], T.start[ti], T.step_inv[ti], false);
+		const float fm = 0.0f + v.f;
+		const float gx = fm * ux, gy = fm * uy, gz = fm * uz;
+		if (t_is_x) {
+			fx += -gx;
+			fy += -gy;
+			fz += -gz;
+		} else {
+			fx += gx;
+			fy += gy;
+			fz += gz;
+		}
+		if (energy)
+			e += (0.0f + v.e) * 0.5f;
+	}
+	return make_float4(fx, fy, fz, e);
+}
+
+// Stage table knots and the tag matrix in LDS.
+__device__ __forceinline__ const float* stage_tables(const NbTables& T, const float* __restrict__ knots_g,
+													 float* lds_knots, uint32_t* lds_tag) {
+	const int nt2 = T.num_types * T.num_types;
+	for (int i = threadIdx.x; i < nt2; i += blockDim.x)
+		lds_tag[i] = T.tag[i];
+	if (T.lds) {
+		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
+			lds_knots[i] = knots_g[i];
+	}
+	__syncthreads();
+	return T.lds ? lds_knots : knots_g;
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_forces(const float4* __restrict__ pos, uint64_t n,
+													  const uint32_t* __restrict__ offsets,
+													  const uint32_t* __restrict__ nbr, NbTables T,
+													  const float* __restrict__ knots_g, DevBox box, float cutoff2,
+													  int energy, float4* __restrict__ force) {
+	extern __shared__ float lds_knots[];
+	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+	const float* knots = stage_tables(T, knots_g, lds_knots, lds_tag);
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const float4 p = pos[t];
+	force[t] = nb_force(uint32_t(t), p, pos, offsets, nbr, T, knots, lds_tag, box, cutoff2, energy != 0);
+}
+
+__global__ void k_zero_force(float4* __restrict__ f, uint64_t n) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t < n)
+		f[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ float4 wrap_pos(const DevBox& box, float4 p) {
+	if (box.periodic[0])
+		p.x = wrap_scalar(p.x, box.origin[0], box.L[0], box.inv_L[0]);
+	if (box.periodic[1])
+		p.y = wrap_scalar(p.y, box.origin[1], box.L[1], box.inv_L[1]);
+	if (box.periodic[2])
+		p.z = wrap_scalar(p.z, box.origin[2], box.L[2], box.inv_L[2]);
+	return p;
+}
+
+// BDIntegrate with the PMF fused (BD.h:44-92); f is ForceEnergy (xyz force, w energy).
+__device__ __forceinline__ float4 bd_update(float4 p, float4& f, const marsmd_particle_types& types,
+										   const marsmd_grid_view* grids, const StepParams& S) {
+	const uint32_t m = fb(p.w);
+	const int ty = int(meta_type(m));
+	const float4 pmf = position_dependent_force(p.x, p.y, p.z, ty, types, grids, S.ef, S.scheme);
+	const float Fx = f.x + pmf.x, Fy = f.y + pmf.y, Fz = f.z + pmf.z;
+	f.w += pmf.w;
+	const float* D = types.diffusion[ty];
+	const float3 rn = box_muller(philox_first(S.seed, S.ctr + meta_canon(m), S.step, 0x5324120u));
+	p.x += (D[0] / S.kT) * Fx * S.dt + sqrtf(2.0f * D[0] * S.dt) * rn.x;
+	p.y += (D[1] / S.kT) * Fy * S.dt + sqrtf(2.0f * D[1] * S.dt) * rn.y;
+	p.z += (D[2] / S.kT) * Fz * S.dt + sqrtf(2.0f * D[2] * S.dt) * rn.z;
+	return wrap_pos(S.box, p);
+}
+
+// Deferred closing kick (BAOAB_LastUpdate, BAOAB.h:151-172) when pending, then
+// B-A-O-A (BAOABIntegrate, BAOAB.h:46-120). f already includes the PMF.
+__device__ __forceinline__ float4 baoab_update(float4 p, float4& mo, const float4& f,
+											   const marsmd_particle_types& types, const StepParams& S) {
+	const uint32_t m = fb(p.w);
+	const int ty = int(meta_type(m));
+	const float FCF = 4.18679994e4f;
+	const float SQ = 2.046167337f;
+	const float hdt = 0.5f * S.dt;
+	if (S.deferred) {
+		mo.x += hdt * f.x * FCF;
+		mo.y += hdt * f.y * FCF;
+		mo.z += hdt * f.z * FCF;
+	}
+	const float mass = types.mass[ty];
+	const float* g = types.trans_damping[ty];
+	mo.x += hdt * f.x * FCF;
+	mo.y += hdt * f.y * FCF;
+	mo.z += hdt * f.z * FCF;
+	const float inv_mass = 1.0f / mass;
+	p.x += hdt * mo.x * inv_mass * 10000.0f;
+	p.y += hdt * mo.y * inv_mass * 10000.0f;
+	p.z += hdt * mo.z * inv_mass * 10000.0f;
+	const float c0 = expf(-g[0] * S.dt), c1 = expf(-g[1] * S.dt), c2 = expf(-g[2] * S.dt);
+	const float n0 = sqrtf(S.kT * mass * (1.0f - c0 * c0)) * SQ;
+	const float n1 = sqrtf(S.kT * mass * (1.0f - c1 * c1)) * SQ;
+	const float n2 = sqrtf(S.kT * mass * (1.0f - c2 * c2)) * SQ;
+	const float3 rn = box_muller(philox_first(S.seed, S.ctr + meta_canon(m), S.step, 0x1356914u));
+	mo.x = c0 * mo.x + n0 * rn.x;
+	mo.y = c1 * mo.y + n1 * rn.y;
+	mo.z = c2 * mo.z + n2 * rn.z;
+	p.x += hdt * mo.x * inv_mass * 10000.0f;
+	p.y += hdt * mo.y * inv_mass * 10000.0f;
+	p.z += hdt * mo.z * inv_mass * 10000.0f;
+	return wrap_pos(S.box, p);
+}
+
+__global__ __launch_bounds__(kBlock) void k_integrate_bd(float4* __restrict__ pos, float4* __restrict__ force,
+														 uint64_t n, marsmd_particle_types types,
+														 const marsmd_grid_view* __restrict__ grids, StepParams S) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	float4 f = force[t];
+	const float4 p = bd_update(pos[t], f, types, S.has_grids ? grids : nullptr, S);
+	pos[t] = p;
+	force[t].w = f.w;
+}
+
+__global__ __launch_bounds__(kBlock) void k_integrate_pmf(const float4* __restrict__ pos, float4* __restrict__ force,
+														  uint64_t n, marsmd_particle_types types,
+														  const marsmd_grid_view* __restrict__ grids, StepParams S) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const float4 p = pos[t];
+	const float4 g = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, grids, S.ef, S.scheme);
+	float4 f = force[t];
+	f.x += g.x;
+	f.y += g.y;
+	f.z += g.z;
+	f.w += g.w;
+	force[t] = f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_integrate_last(float4* __restrict__ mom, const float4* __restrict__ force,
+														   uint64_t n, float dt) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const float4 f = force[t];
+	float4 m = mom[t];
+	const float hdt = 0.5f * dt;
+	const float FCF = 4.18679994e4f;
+	m.x += hdt * f.x * FCF;
+	m.y += hdt * f.y * FCF;
+	m.z += hdt * f.z * FCF;
+	mom[t] = m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_integrate_baoab(float4* __restrict__ pos, float4* __restrict__ mom,
+															const float4* __restrict__ force, uint64_t n,
+															marsmd_particle_types types, StepParams S) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	float4 mo = mom[t];
+	const float4 p = baoab_update(pos[t], mo, force[t], types, S);
+	pos[t] = p;
+	mom[t] = mo;
+}
+
+// The fused step: nonbonded + bonded (from `force` when has_bonded) + PMF + integrator.
+template<int INTEG, bool ENERGY>
+__global__ __launch_bounds__(kBlock) void k_step(const float4* __restrict__ pos, float4* __restrict__ pos_next,
+												 float4* __restrict__ mom, float4* __restrict__ force, uint64_t n,
+												 const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ nbr,
+												 NbTables T, const float* __restrict__ knots_g,
+												 marsmd_particle_types types,
+												 const marsmd_grid_view* __restrict__ grids, StepParams S,
+												 int has_bonded, int store_force) {
+	extern __shared__ float lds_knots[];
+	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+	const float* knots = stage_tables(T, knots_g, lds_knots, lds_tag);
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const float4 p = pos[t];
+	float4 f = nb_force(uint32_t(t), p, pos, offsets, nbr, T, knots, lds_tag, S.box, S.cutoff2, ENERGY);
+	if (has_bonded) {
+		const float4 b = force[t];
+		f.x += b.x;
+		f.y += b.y;
+		f.z += b.z;
+		f.w += b.w;
+	}
+	float4 pn;
+	if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
+		pn = bd_update(p, f, types, S.has_grids ? grids : nullptr, S);
+	} else {
+		if (S.has_grids) {
+			const float4 g = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, grids, S.ef,
+													  S.scheme);
+			f.x += g.x;
+			f.y += g.y;
+			f.z += g.z;
+			f.w += g.w;
+		}
+		float4 mo = mom[t];
+		pn = baoab_update(p, mo, f, types, S);
+		mom[t] = mo;
+	}
+	pos_next[t] = pn;
+	if (ENERGY || store_force)
+		force[t] = f;
+}
+
+// Permute slot-indexed state into the new sorted order:
+// out[t] = in[old_slot_of_canon[sorted_canon[t]]].
+__global__ void k_gather_state(const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ old_slot, uint64_t n,
+							   const float4* __restrict__ mom_in, float4* __restrict__ mom_out,
+							   const int32_t* __restrict__ ids_in, int32_t* __restrict__ ids_out) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const uint32_t s = old_slot[sidx[t]];
+	if (s >= n)
+		return;
+	mom_out[t] = mom_in[s];
+	ids_out[t] = ids_in[s];
+}
+
+// canonical -> slot remap of bonded index tuples (stride ints per term, first k used).
+__global__ void k_topo_to_slots(const int32_t* __restrict__ canon, int32_t* __restrict__ slots, uint64_t count,
+								const uint32_t* __restrict__ slot_of_canon, uint64_t n) {
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= count)
+		return;
+	const int32_t c = canon[i];
+	slots[i] = (c >= 0 && uint64_t(c) < n) ? int32_t(slot_of_canon[c]) : c;
+}
+
+// Reorder relabel: canonical c -> relabel[c] in every slot's meta.
+__global__ void k_relabel(float4* __restrict__ pos, uint64_t n, const uint32_t* __restrict__ relabel,
+						  uint32_t* __restrict__ slot_of_canon) {
+	const uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (s >= n)
+		return;
+	const uint32_t m = fb(pos[s].w);
+	const uint32_t c = meta_canon(m);
+	if (c >= n)
+		return;
+	const uint32_t cn = relabel[c];
+	pos[s].w = __uint_as_float(make_meta(meta_type(m), cn));
+	slot_of_canon[cn] = uint32_t(s);
+}
+
+__global__ void k_relabel_pairs(int32_t* __restrict__ pairs, uint64_t ne, const uint32_t* __restrict__ relabel,
+								uint64_t n) {
+	const uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (k >= ne)
+		return;
+	int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+	if (a >= 0 && uint64_t(a) < n)
+		a = int32_t(relabel[a]);
+	if (b >= 0 && uint64_t(b) < n)
+		b = int32_t(relabel[b]);
+	pairs[2 * k] = a < b ? a : b;
+	pairs[2 * k + 1] = a < b ? b : a;
+}
+
+// Exclusion CSR on the device (DeviceBondedInteraction.h:111-167 and 386-437):
+// degree count, scan, fill. Rows are sets; their internal order is not used.
+__global__ void k_excl_degree(const int32_t* __restrict__ pairs, uint64_t ne, uint32_t* __restrict__ deg) {
+	const uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (k >= ne)
+		return;
+	atomicAdd(&deg[pairs[2 * k]], 1u);
+	atomicAdd(&deg[pairs[2 * k + 1]], 1u);
+}
+__global__ void k_excl_fill(const int32_t* __restrict__ pairs, uint64_t ne, uint32_t* __restrict__ cursor,
+							int32_t* __restrict__ excluded) {
+	const uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (k >= ne)
+		return;
+	const int32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+	excluded[atomicAdd(&cursor[a], 1u)] = b;
+	excluded[atomicAdd(&cursor[b], 1u)] = a;
+}
+
+__global__ void k_import(const float4* __restrict__ pos_h, const float4* __restrict__ mom_h,
+						 const int32_t* __restrict__ types, uint64_t n, float4* __restrict__ pos,
+						 float4* __restrict__ mom, int32_t* __restrict__ ids, uint32_t* __restrict__ slot_of_canon) {
+	const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (c >= n)
+		return;
+	const float4 p = pos_h[c];
+	pos[c] = make_float4(p.x, p.y, p.z, __uint_as_float(make_meta(uint32_t(types[c]), uint32_t(c))));
+	mom[c] = mom_h ? make_float4(mom_h[c].x, mom_h[c].y, mom_h[c].z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	ids[c] = int32_t(c);
+	slot_of_canon[c] = uint32_t(c);
+}
+
+__global__ void k_export(const float4* __restrict__ pos, const float4* __restrict__ mom,
+						 const float4* __restrict__ force, const int32_t* __restrict__ ids,
+						 const uint32_t* __restrict__ slot_of_canon, uint64_t n, float4* __restrict__ pos_o,
+						 float4* __restrict__ mom_o, float4* __restrict__ force_o, int32_t* __restrict__ type_o,
+						 int32_t* __restrict__ ids_o) {
+	const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (c >= n)
+		return;
+	const uint32_t s = slot_of_canon[c];
+	const float4 p = pos[s];
+	pos_o[c] = make_float4(p.x, p.y, p.z, 0.0f);
+	mom_o[c] = mom[s];
+	force_o[c] = force[s];
+	type_o[c] = int32_t(meta_type(fb(p.w)));
+	ids_o[c] = ids[s];
+}
+
+uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+} // namespace
+
+// ------------------------------------------------------------------ PatchImpl
+
+struct PatchImpl::Tables {
+	NbTables nb{};
+	DevArray<float> nb_knots;
+	std::vector<marsmd_table> bond_h, angle_h;
+	DevArray<float> bond_knots, angle_knots;
+	DevArray<marsmd_table> bond_dev, angle_dev;
+};
+
+PatchImpl::PatchImpl(const marsmd_patch_desc& d) : tables(new Tables) {
+	capacity = d.capacity;
+	patch_id = d.patch_id;
+	box = d.box;
+	types = d.types;
+	if (types.num_types < 1 || types.num_types > MARSMD_MAX_TYPES)
+		raise(MARSMD_ERR_VALUE, "Patch: num_types must be in [1, " + std::to_string(MARSMD_MAX_TYPES) + "]");
+	if (capacity >= kMaxParticles)
+		raise(MARSMD_ERR_VALUE, "Patch: capacity must be below 2^24");
+	NbTables& T = tables->nb;
+	T.num_types = types.num_types;
+	T.enabled = d.enabled_terms;
+	if (d.enabled_terms & ~uint32_t(MARSMD_PAIR_TERM_TABULATED) & MARSMD_PAIR_TERM_MASK)
+		raise(MARSMD_ERR_VALUE, "Patch: only the TABULATED pair term is on this path");
+	const int nt2 = types.num_types * types.num_types;
+	for (int i = 0; i < nt2; ++i)
+		T.tag[i] = d.term_matrix ? d.term_matrix[i] : 0u;
+	if (d.num_nb_tables > kMaxTables)
+		raise(MARSMD_ERR_VALUE, "Patch: too many nonbonded tables");
+	T.num_tables = d.num_nb_tables;
+	std::vector<float> knots;
+	for (int k = 0; k < d.num_nb_tables; ++k) {
+		const marsmd_table& t = d.nb_tables[k];
+		if (t.size < 2)
+			raise(MARSMD_ERR_VALUE, "Patch: tables need at least 2 knots");
+		if (t.is_periodic)
+			raise(MARSMD_ERR_VALUE, "Patch: nonbonded tables are not periodic");
+		T.start[k] = t.start;
+		T.step_inv[k] = t.step_inv;
+		T.size[k] = int(t.size);
+		T.offset[k] = int(knots.size());
+		knots.insert(knots.end(), t.pot, t.pot + t.size);
+	}
+	for (int i = 0; i < nt2; ++i) {
+		const uint32_t tag = T.tag[i];
+		if ((tag & MARSMD_PAIR_TERM_TABULATED) && int(tag >> MARSMD_PAIR_TABLE_SHIFT) >= d.num_nb_tables)
+			raise(MARSMD_ERR_VALUE, "Patch: a pair tag names a missing table");
+	}
+	T.total_knots = int(knots.size());
+	T.lds = T.total_knots <= kLdsKnots ? 1 : 0;
+	tables->nb_knots.resize(std::max<uint64_t>(knots.size(), 1));
+	if (!knots.empty())
+		MMD_HIP(hipMemcpy(tables->nb_knots.data(), knots.data(), knots.size() * 4, hipMemcpyHostToDevice));
+	auto upload_tables = [&](int num, const marsmd_table* src, std::vector<marsmd_table>& host, DevArray<float>& kd,
+							 DevArray<marsmd_table>& td) {
+		std::vector<float> kn;
+		std::vector<size_t> off;
+		for (int k = 0; k < num; ++k) {
+			off.push_back(kn.size());
+			kn.insert(kn.end(), src[k].pot, src[k].pot + src[k].size);
+		}
+		kd.resize(std::max<uint64_t>(kn.size(), 1));
+		if (!kn.empty())
+			MMD_HIP(hipMemcpy(kd.data(), kn.data(), kn.size() * 4, hipMemcpyHostToDevice));
+		host.clear();
+		for (int k = 0; k < num; ++k) {
+			marsmd_table t = src[k];
+			t.pot = kd.data() + off[size_t(k)];
+			host.push_back(t);
+		}
+		td.resize(std::max<uint64_t>(host.size(), 1));
+		if (!host.empty())
+			MMD_HIP(hipMemcpy(td.data(), host.data(), host.size() * sizeof(marsmd_table), hipMemcpyHostToDevice));
+	};
+	upload_tables(d.num_bond_tables, d.bond_tables, tables->bond_h, tables->bond_knots, tables->bond_dev);
+	upload_tables(d.num_angle_tables, d.angle_tables, tables->angle_h, tables->angle_knots, tables->angle_dev);
+	// topology
+	num_bonds = d.num_bonds;
+	num_angles = d.num_angles;
+	for (uint64_t k = 0; k < 2 * num_bonds; ++k)
+		topo_max_index = std::max<int64_t>(topo_max_index, d.bonds[k]);
+	for (uint64_t k = 0; k < 3 * num_angles; ++k)
+		topo_max_index = std::max<int64_t>(topo_max_index, d.angles[k]);
+	for (uint64_t k = 0; k < 2 * d.num_exclusions; ++k)
+		topo_max_index = std::max<int64_t>(topo_max_index, d.exclusions[k]);
+	for (uint64_t k = 0; k < 2 * num_bonds; ++k)
+		if (d.bonds[k] < 0)
+			raise(MARSMD_ERR_VALUE, "Patch: negative bond index");
+	for (uint64_t k = 0; k < 3 * num_angles; ++k)
+		if (d.angles[k] < 0)
+			raise(MARSMD_ERR_VALUE, "Patch: negative angle index");
+	if (num_bonds) {
+		bonds_canon.resize(2 * num_bonds);
+		bonds_slot.resize(2 * num_bonds);
+		bond_table.resize(num_bonds);
+		MMD_HIP(hipMemcpy(bonds_canon.data(), d.bonds, 8 * num_bonds, hipMemcpyHostToDevice));
+		std::vector<int32_t> bt(num_bonds, 0);
+		if (d.bond_table)
+			std::copy(d.bond_table, d.bond_table + num_bonds, bt.begin());
+		for (auto v : bt)
+			if (v < 0 || v >= d.num_bond_tables)
+				raise(MARSMD_ERR_VALUE, "Patch: bond table index out of range");
+		MMD_HIP(hipMemcpy(bond_table.data(), bt.data(), 4 * num_bonds, hipMemcpyHostToDevice));
+	}
+	if (num_angles) {
+		std::vector<int32_t> a4(4 * num_angles, 0);
+		for (uint64_t k = 0; k < num_angles; ++k)
+			for (int q = 0; q < 3; ++q)
+				a4[4 * k + uint64_t(q)] = d.angles[3 * k + uint64_t(q)];
+		angles_canon.resize(4 * num_angles);
+		angles_slot.resize(4 * num_angles);
+		angle_table.resize(num_angles);
+		MMD_HIP(hipMemcpy(angles_canon.data(), a4.data(), 16 * num_angles, hipMemcpyHostToDevice));
+		std::vector<int32_t> at(num_angles, 0);
+		if (d.angle_table)
+			std::copy(d.angle_table, d.angle_table + num_angles, at.begin());
+		for (auto v : at)
+			if (v < 0 || v >= d.num_angle_tables)
+				raise(MARSMD_ERR_VALUE, "Patch: angle table index out of range");
+		MMD_HIP(hipMemcpy(angle_table.data(), at.data(), 4 * num_angles, hipMemcpyHostToDevice));
+	}
+	num_excl = d.num_exclusions;
+	if (num_excl) {
+		std::vector<int32_t> ex(2 * num_excl);
+		for (uint64_t k = 0; k < num_excl; ++k) {
+			const int32_t a = d.exclusions[2 * k], b = d.exclusions[2 * k + 1];
+			if (a < 0 || b < 0 || uint64_t(a) >= capacity || uint64_t(b) >= capacity)
+				raise(MARSMD_ERR_VALUE, "Patch: exclusion index out of range");
+			ex[2 * k] = std::min(a, b);
+			ex[2 * k + 1] = std::max(a, b);
+		}
+		excl_pairs.resize(2 * num_excl);
+		MMD_HIP(hipMemcpy(excl_pairs.data(), ex.data(), 8 * num_excl, hipMemcpyHostToDevice));
+	}
+	// grids
+	num_grids = d.num_grids;
+	std::vector<marsmd_grid_view> gv;
+	for (int k = 0; k < d.num_grids; ++k) {
+		marsmd_grid_view g = d.grids[k];
+		const uint64_t cnt = g.dims[0] * g.dims[1] * g.dims[2];
+		grid_data.emplace_back(cnt);
+		MMD_HIP(hipMemcpy(grid_data.back().data(), g.data, cnt * 4, hipMemcpyHostToDevice));
+		g.data = grid_data.back().data();
+		gv.push_back(g);
+	}
+	if (!gv.empty()) {
+		grids_dev.resize(gv.size());
+		MMD_HIP(hipMemcpy(grids_dev.data(), gv.data(), gv.size() * sizeof(marsmd_grid_view), hipMemcpyHostToDevice));
+	}
+	pl.reset(new PairlistImpl(capacity, ~uint64_t(0)));
+	pl->set_box(box);
+}
+
+PatchImpl::~PatchImpl() { delete tables; }
+
+void PatchImpl::rebuild_exclusions(hipStream_t s) {
+	if (num_excl == 0 || n == 0)
+		return;
+	excl_offsets.resize(n + 1);
+	excl_deg.resize(n + 1);
+	excl_nbrs.resize(2 * num_excl);
+	excl_scan.resize(scan_scratch_bytes(n + 1));
+	fill_u32(excl_deg.data(), 0u, n + 1, s);
+	k_excl_degree<<<grid_for(num_excl, 256), 256, 0, s>>>(excl_pairs.data(), num_excl, excl_deg.data());
+	MMD_LAUNCH_CHECK();
+	exclusive_scan_u32(excl_deg.data(), excl_offsets.data(), n, excl_scan.data(), s);
+	MMD_HIP(hipMemcpyAsync(excl_deg.data(), excl_offsets.data(), 4 * n, hipMemcpyDeviceToDevice, s));
+	k_excl_fill<<<grid_for(num_excl, 256), 256, 0, s>>>(excl_pairs.data(), num_excl, excl_deg.data(),
+														 reinterpret_cast<int32_t*>(excl_nbrs.data()));
+	MMD_LAUNCH_CHECK();
+}
+
+void PatchImpl::copy_from_host(const float* pos4, const float* mom4, const int32_t* type_ids, uint64_t nn) {
+	if (nn > capacity)
+		raise(MARSMD_ERR_RUNTIME, "Cannot copy " + std::to_string(nn) + " particles to patch with capacity " +
+									  std::to_string(capacity));
+	for (uint64_t i = 0; i < nn; ++i)
+		if (type_ids[i] < 0 || type_ids[i] >= types.num_types)
+			raise(MARSMD_ERR_VALUE, "Patch: particle type out of range");
+	if (topo_max_index >= int64_t(nn))
+		raise(MARSMD_ERR_VALUE, "Patch: bonded topology references particle " + std::to_string(topo_max_index) +
+									" but only " + std::to_string(nn) + " were copied");
+	n = nn;
+	for (int b = 0; b < 2; ++b)
+		pos[b].resize(std::max<uint64_t>(n, 1));
+	mom.resize(std::max<uint64_t>(n, 1));
+	mom_tmp.resize(std::max<uint64_t>(n, 1));
+	force.resize(std::max<uint64_t>(n, 1));
+	ids.resize(std::max<uint64_t>(n, 1));
+	ids_tmp.resize(std::max<uint64_t>(n, 1));
+	slot_of_canon.resize(std::max<uint64_t>(n, 1));
+	cur = 0;
+	if (n) {
+		DevArray<float4> hp(n), hm(mom4 ? n : 0);
+		DevArray<int32_t> ht(n);
+		MMD_HIP(hipMemcpy(hp.data(), pos4, 16 * n, hipMemcpyHostToDevice));
+		if (mom4)
+			MMD_HIP(hipMemcpy(hm.data(), mom4, 16 * n, hipMemcpyHostToDevice));
+		MMD_HIP(hipMemcpy(ht.data(), type_ids, 4 * n, hipMemcpyHostToDevice));
+		k_import<<<grid_for(n, 256), 256>>>(hp.data(), mom4 ? hm.data() : nullptr, ht.data(), n, pos[0].data(),
+											mom.data(), ids.data(), slot_of_canon.data());
+		MMD_LAUNCH_CHECK();
+		k_zero_force<<<grid_for(n, 256), 256>>>(force.data(), n);
+		MMD_LAUNCH_CHECK();
+		MMD_HIP(hipDeviceSynchronize());
+	}
+	built = false;
+	force_rebuild = false;
+	deferred_kick = false;
+	rebuild_exclusions(nullptr);
+	MMD_HIP(hipDeviceSynchronize());
+}
+
+void PatchImpl::copy_to_host(float* pos4, float* mom4, float* force4, int32_t* type_ids, int32_t* id_out,
+							 uint64_t nn) {
+	if (nn > n)
+		raise(MARSMD_ERR_RUNTIME, "Cannot copy " + std::to_string(nn) + " particles from patch containing only " +
+									  std::to_string(n));
+	if (nn == 0)
+		return;
+	DevArray<float4> op(n), om(n), of(n);
+	DevArray<int32_t> ot(n), oi(n);
+	MMD_HIP(hipDeviceSynchronize());
+	k_export<<<grid_for(n, 256), 256>>>(pos[cur].data(), mom.data(), force.data(), ids.data(), slot_of_canon.data(), n,
+										op.data(), om.data(), of.data(), ot.data(), oi.data());
+	MMD_LAUNCH_CHECK();
+	MMD_HIP(hipDeviceSynchronize());
+	if (pos4)
+		MMD_HIP(hipMemcpy(pos4, op.data(), 16 * nn, hipMemcpyDeviceToHost));
+	if (mom4)
+		MMD_HIP(hipMemcpy(mom4, om.data(), 16 * nn, hipMemcpyDeviceToHost));
+	if (force4)
+		MMD_HIP(hipMemcpy(force4, of.data(), 16 * nn, hipMemcpyDeviceToHost));
+	if (type_ids)
+		MMD_HIP(hipMemcpy(type_ids, ot.data(), 4 * nn, hipMemcpyDeviceToHost));
+	if (id_out)
+		MMD_HIP(hipMemcpy(id_out, oi.data(), 4 * nn, hipMemcpyDeviceToHost));
+}
+
+void PatchImpl::build(float pairlist_cutoff, hipStream_t s) {
+	marsmd_exclusion_view ev{};
+	if (num_excl) {
+		ev.offsets = reinterpret_cast<const int32_t*>(excl_offsets.data());
+		ev.excluded = reinterpret_cast<const int32_t*>(excl_nbrs.data());
+		ev.num_particles = n;
+	}
+	pl->excl = ev;
+	pl->build(pos[cur].data(), slot_of_canon.data(), true, nullptr, n, pairlist_cutoff, s);
+	// The sorted positions become the slot-order state; gather the rest.
+	MMD_HIP(hipMemcpyAsync(pos[cur].data(), pl->spos.data(), 16 * n, hipMemcpyDeviceToDevice, s));
+	k_gather_state<<<grid_for(n, 256), 256, 0, s>>>(pl->sorter.sorted_idx.data(), slot_of_canon.data(), n, mom.data(),
+													mom_tmp.data(), ids.data(), ids_tmp.data());
+	MMD_LAUNCH_CHECK();
+	mom.swap(mom_tmp);
+	ids.swap(ids_tmp);
+	MMD_HIP(hipMemcpyAsync(slot_of_canon.data(), pl->sorter.inverse.data(), 4 * n, hipMemcpyDeviceToDevice, s));
+	if (num_bonds) {
+		k_topo_to_slots<<<grid_for(2 * num_bonds, 256), 256, 0, s>>>(bonds_canon.data(), bonds_slot.data(),
+																	  2 * num_bonds, slot_of_canon.data(), n);
+		MMD_LAUNCH_CHECK();
+	}
+	if (num_angles) {
+		k_topo_to_slots<<<grid_for(4 * num_angles, 256), 256, 0, s>>>(angles_canon.data(), angles_slot.data(),
+																	   4 * num_angles, slot_of_canon.data(), n);
+		MMD_LAUNCH_CHECK();
+	}
+	built = true;
+}
+
+bool PatchImpl::rebuild_policy(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period, hipStream_t s) {
+	// Patch::calculate_nonbonded_forces rebuild decision (Patch.cpp:85-121).
+	const bool at_period = (period == 0 || (step - 1) % period == 0);
+	bool rebuild = at_period || !built;
+	const bool forced = force_rebuild;
+	force_rebuild = false;
+	rebuild = rebuild || forced;
+	const float skin = pairlist_cutoff - cutoff;
+	if (rebuild && !forced && period != 0 && built && skin > 0.0f && n > 0)
+		rebuild = pl->max_displacement(pos[cur].data(), nullptr, n, s) > skin * 0.5f;
+	if (rebuild)
+		build(pairlist_cutoff, s);
+	return rebuild;
+}
+
+unsigned PatchImpl::lds_bytes() const { return tables->nb.lds ? unsigned(tables->nb.total_knots) * 4u : 0u; }
+
+void PatchImpl::calculate_nonbonded(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period,
+									const float ef[3], int interp, bool energy, hipStream_t s) {
+	if (n == 0)
+		return;
+	bonded_prepared = true;
+	for (int a = 0; a < 3; ++a)
+		efield[a] = ef[a];
+	interpolation = interp;
+	rebuild_policy(pairlist_cutoff, cutoff, step, period, s);
+	const float cutoff2 = cutoff > 0.0f ? cutoff * cutoff : 0.0f;
+	k_nb_forces<<<grid_for(n, kBlock), kBlock, lds_bytes(), s>>>(pos[cur].data(), n, pl->offsets.data(),
+																 pl->nbr.data(), tables->nb, tables->nb_knots.data(),
+																 make_devbox(box), cutoff2, energy ? 1 : 0,
+																 force.data());
+	MMD_LAUNCH_CHECK();
+}
+
+void PatchImpl::calculate_bonded(bool energy, hipStream_t s) {
+	if (n == 0 || (num_bonds == 0 && num_angles == 0))
+		return;
+	if (!built)
+		raise(MARSMD_ERR_RUNTIME, "Patch: calculate_bonded_forces before the first nonbonded pass");
+	if (num_bonds)
+		launch_tabulated_bonds(bonds_slot.data(), reinterpret_cast<const float*>(pos[cur].data()),
+							   reinterpret_cast<float*>(force.data()), tables->bond_dev.data(), bond_table.data(),
+							   nullptr, box, energy, num_bonds, s);
+	if (num_angles)
+		launch_tabulated_angles(angles_slot.data(), reinterpret_cast<const float*>(pos[cur].data()),
+								reinterpret_cast<float*>(force.data()), tables->angle_dev.data(), angle_table.data(),
+								nullptr, box, energy, num_angles, s);
+}
+
+void PatchImpl::integrate(float dt, float kT, int integrator, uint64_t step, hipStream_t s) {
+	if (n == 0)
+		return;
+	StepParams S{};
+	S.box = make_devbox(box);
+	S.dt = dt;
+	S.kT = kT;
+	S.seed = mix_seed(base_seed, patch_id, step);
+	S.ctr = 0;
+	S.step = uint32_t(step);
+	S.scheme = interpolation;
+	S.has_grids = num_grids > 0 ? 1 : 0;
+	for (int a = 0; a < 3; ++a)
+		S.ef[a] = efield[a];
+	const unsigned g = grid_for(n, kBlock);
+	if (integrator == MARSMD_INTEGRATOR_BROWNIAN) {
+		k_integrate_bd<<<g, kBlock, 0, s>>>(pos[cur].data(), force.data(), n, types, grids_dev.data(), S);
+		MMD_LAUNCH_CHECK();
+		return;
+	}
+	if (integrator != MARSMD_INTEGRATOR_LANGEVIN)
+		raise(MARSMD_ERR_VALUE, "Unsupported integrator type: " + std::to_string(integrator));
+	if (num_grids > 0) {
+		k_integrate_pmf<<<g, kBlock, 0, s>>>(pos[cur].data(), force.data(), n, types, grids_dev.data(), S);
+		MMD_LAUNCH_CHECK();
+	}
+	if (deferred_kick) {
+		k_integrate_last<<<g, kBlock, 0, s>>>(mom.data(), force.data(), n, dt);
+		MMD_LAUNCH_CHECK();
+		deferred_kick = false;
+	}
+	deferred_kick = true;
+	k_integrate_baoab<<<g, kBlock, 0, s>>>(pos[cur].data(), mom.data(), force.data(), n, types, S);
+	MMD_LAUNCH_CHECK();
+}
+
+void PatchImpl::finish_deferred_kick(float dt, hipStream_t s) {
+	if (!deferred_kick)
+		return;
+	deferred_kick = false;
+	if (n == 0)
+		return;
+	k_integrate_last<<<grid_for(n, kBlock), kBlock, 0, s>>>(mom.data(), force.data(), n, dt);
+	MMD_LAUNCH_CHECK();
+}
+
+void PatchImpl::reorder(hipStream_t s) {
+	// Patch::reorder_particles (Patch.cpp:456-480): System-mode Morton sort over the
+	// periodic box, relabel canonical indices, remap topology, force a rebuild.
+	if (n == 0)
+		return;
+	float bmin[3], bmax[3];
+	for (int a = 0; a < 3; ++a) {
+		bmin[a] = box.origin[a];
+		bmax[a] = box.origin[a] + box.size[a];
+		if (!(bmax[a] > bmin[a]))
+			return;
+	}
+	ZSort& z = pl->sorter;
+	z.sort(pos[cur].data(), n, bmin, bmax, nullptr, true, s);
+	// relabel: canonical c -> rank of c in the System sort = z.inverse[c]
+	relabel.resize(n);
+	MMD_HIP(hipMemcpyAsync(relabel.data(), z.inverse.data(), 4 * n, hipMemcpyDeviceToDevice, s));
+	k_relabel<<<grid_for(n, 256), 256, 0, s>>>(pos[cur].data(), n, relabel.data(), slot_of_canon.data());
+	MMD_LAUNCH_CHECK();
+	if (num_bonds) {
+		k_remap_canon(bonds_canon.data(), 2 * num_bonds, s);
+	}
+	if (num_angles) {
+		k_remap_canon(angles_canon.data(), 4 * num_angles, s);
+	}
+	if (num_excl) {
+		k_relabel_pairs<<<grid_for(num_excl, 256), 256, 0, s>>>(excl_pairs.data(), num_excl, relabel.data(), n);
+		MMD_LAUNCH_CHECK();
+		rebuild_exclusions(s);
+	}
+	force_rebuild = true;
+}
+
+void PatchImpl::k_remap_canon(int32_t* idx, uint64_t count, hipStream_t s) {
+	remap_tmp.resize(count);
+	MMD_HIP(hipMemcpyAsync(remap_tmp.data(), idx, 4 * count, hipMemcpyDeviceToDevice, s));
+	k_topo_to_slots<<<grid_for(count, 256), 256, 0, s>>>(remap_tmp.data(), idx, count, relabel.data(), n);
+	MMD_LAUNCH_CHECK();
+}
+
+void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t num_steps, hipStream_t s) {
+	if (P.integrator != MARSMD_INTEGRATOR_BROWNIAN && P.integrator != MARSMD_INTEGRATOR_LANGEVIN)
+		raise(MARSMD_ERR_VALUE, "Unsupported integrator type: " + std::to_string(P.integrator));
+	if (n == 0)
+		return;
+	const int64_t rp = P.reorder_period > 0 ? P.reorder_period : int64_t(P.rebuild_period);
+	const float cutoff2 = P.cutoff > 0.0f ? P.cutoff * P.cutoff : 0.0f;
+	const bool has_bonded = (num_bonds + num_angles) > 0;
+	const unsigned g = grid_for(n, kBlock);
+	for (uint64_t step = first_step + 1; step <= first_step + num_steps; ++step) {
+		// ParticleReorderManager::maybe_reorder (ReorderManager.cpp:8-25)
+		if (rp > 0 && step != 0 && (step - 1) % uint64_t(rp) == 0 && bonded_prepared)
+			reorder(s);
+		const bool energy = P.energy_period > 0 && step % P.energy_period == 0;
+		bonded_prepared = true;
+		for (int a = 0; a < 3; ++a)
+			efield[a] = P.electric_field[a];
+		interpolation = P.interpolation;
+		rebuild_policy(P.pairlist_cutoff, P.cutoff, step, P.rebuild_period, s);
+		if (has_bonded) {
+			k_zero_force<<<g, kBlock, 0, s>>>(force.data(), n);
+			MMD_LAUNCH_CHECK();
+			calculate_bonded(energy, s);
+		}
+		StepParams S{};
+		S.box = make_devbox(box);
+		S.cutoff2 = cutoff2;
+		S.dt = P.timestep;
+		S.kT = P.kT;
+		S.seed = mix_seed(base_seed, patch_id, step);
+		S.ctr = 0;
+		S.step = uint32_t(step);
+		S.scheme = P.interpolation;
+		S.has_grids = num_grids > 0 ? 1 : 0;
+		S.deferred = deferred_kick ? 1 : 0;
+		for (int a = 0; a < 3; ++a)
+			S.ef[a] = P.electric_field[a];
+		const int store = (step == first_step + num_steps) ? 1 : 0;
+		float4* pnext = pos[1 - cur].data();
+		if (P.integrator == MARSMD_INTEGRATOR_BROWNIAN) {
+			if (energy)
+				k_step<MARSMD_INTEGRATOR_BROWNIAN, true><<<g, kBlock, lds_bytes(), s>>>(
+					pos[cur].data(), pnext, mom.data(), force.data(), n, pl->offsets.data(), pl->nbr.data(), tables->nb,
+					tables->nb_knots.data(), types, grids_dev.data(), S, has_bonded, store);
+			else
+				k_step<MARSMD_INTEGRATOR_BROWNIAN, false><<<g, kBlock, lds_bytes(), s>>>(
+					pos[cur].data(), pnext, mom.data(), force.data(), n, pl->offsets.data(), pl->nbr.data(), tables->nb,
+					tables->nb_knots.data(), types, grids_dev.data(), S, has_bonded, store);
+		} else {
+			if (energy)
+				k_step<MARSMD_INTEGRATOR_LANGEVIN, true><<<g, kBlock, lds_bytes(), s>>>(
+					pos[cur].data(), pnext, mom.data(), force.data(), n, pl->offsets.data(), pl->nbr.data(), tables->nb,
+					tables->nb_knots.data(), types, grids_dev.data(), S, has_bonded, store);
+			else
+				k_step<MARSMD_INTEGRATOR_LANGEVIN, false><<<g, kBlock, lds_bytes(), s>>>(
+					pos[cur].data(), pnext, mom.data(), force.data(), n, pl->offsets.data(), pl->nbr.data(), tables->nb,
+					tables->nb_knots.data(), types, grids_dev.data(), S, has_bonded, store);
+			deferred_kick = true;
+		}
+		MMD_LAUNCH_CHECK();
+		cur = 1 - cur;
+	}
+}
+
+} // namespace mmd

@@ -1,0 +1,64 @@
+// patch.hpp — host class behind marsmd_patch (see patch.hip).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "pairlist.hpp"
+
+namespace mmd {
+
+struct PatchImpl {
+	struct Tables;
+	uint64_t capacity = 0, patch_id = 0, base_seed = 0;
+	marsmd_box box{};
+	marsmd_particle_types types{};
+	Tables* tables = nullptr;
+	// topology (canonical) and its slot-order copy
+	uint64_t num_bonds = 0, num_angles = 0, num_excl = 0;
+	int64_t topo_max_index = -1;
+	DevArray<int32_t> bonds_canon, bonds_slot, bond_table;
+	DevArray<int32_t> angles_canon, angles_slot, angle_table; // 4 ints per angle
+	DevArray<int32_t> excl_pairs;
+	DevArray<uint32_t> excl_offsets, excl_deg, excl_nbrs;
+	DevArray<uint8_t> excl_scan;
+	// grids
+	int num_grids = 0;
+	std::vector<DevArray<float>> grid_data;
+	DevArray<marsmd_grid_view> grids_dev;
+	// particle state, slot order
+	uint64_t n = 0;
+	DevArray<float4> pos[2];
+	int cur = 0;
+	DevArray<float4> mom, mom_tmp, force;
+	DevArray<int32_t> ids, ids_tmp, remap_tmp;
+	DevArray<uint32_t> slot_of_canon, relabel;
+	std::unique_ptr<PairlistImpl> pl;
+	bool built = false, force_rebuild = false, deferred_kick = false, bonded_prepared = false;
+	float efield[3] = {0, 0, 0};
+	int interpolation = MARSMD_INTERP_LINEAR;
+
+	explicit PatchImpl(const marsmd_patch_desc& d);
+	~PatchImpl();
+	PatchImpl(const PatchImpl&) = delete;
+	PatchImpl& operator=(const PatchImpl&) = delete;
+
+	void copy_from_host(const float* pos4, const float* mom4, const int32_t* type_ids, uint64_t n);
+	void copy_to_host(float* pos4, float* mom4, float* force4, int32_t* type_ids, int32_t* ids, uint64_t n);
+	void calculate_nonbonded(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period, const float ef[3],
+							 int interp, bool energy, hipStream_t s);
+	void calculate_bonded(bool energy, hipStream_t s);
+	void integrate(float dt, float kT, int integrator, uint64_t step, hipStream_t s);
+	void finish_deferred_kick(float dt, hipStream_t s);
+	void reorder(hipStream_t s);
+	void run(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s);
+
+  private:
+	void build(float pairlist_cutoff, hipStream_t s);
+	bool rebuild_policy(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period, hipStream_t s);
+	void rebuild_exclusions(hipStream_t s);
+	void k_remap_canon(int32_t* idx, uint64_t count, hipStream_t s);
+	unsigned lds_bytes() const;
+};
+
+} // namespace mmd

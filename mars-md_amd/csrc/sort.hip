@@ -1,0 +1,295 @@
+// sort.hip — stable LSD radix sort of (uint32 key, uint32 value) pairs and an
+// exclusive scan, written for gfx950 wave64.
+//
+// Replaces the CUB DeviceRadixSort::SortPairs call of the reference
+// (src/PatchOperation/ZOrderKernels/CUDASort.cu:13-98; stability contract pinned by
+// Unit_test/System/zorder_tests.cpp:139-151). One pass per 8-bit digit:
+//   1. k_rs_hist     per-tile digit histogram (wave-aggregated LDS counts),
+//   2. k_rs_scan     one-block exclusive scan over [digit][tile] (digit-major),
+//   3. k_rs_scatter  per-tile stable ranks from wave64 ballot peer masks, then
+//                    scatter to offset[digit][tile] + rank.
+// A tile is 4 waves x 16 rows x 64 lanes = 4096 keys; wave w owns the contiguous
+// quarter [w*1024, (w+1)*1024) of its tile, so ranks follow index order and the
+// sort is stable.
+#include "internal.hpp"
+
+namespace mmd {
+
+namespace {
+
+constexpr int kRsBlock = 256;
+constexpr int kRsRows = 16;
+constexpr int kRsTile = kRsBlock * kRsRows;
+constexpr int kRsBuckets = 256;
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+	const int lane = threadIdx.x & 63;
+	return (uint64_t(1) << lane) - 1;
+}
+
+// Lanes of this wave holding the same 8-bit digit (and valid).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
+	uint64_t peers = __ballot(valid);
+#pragma unroll
+	for (int b = 0; b < 8; ++b) {
+		const bool bit = (d >> b) & 1u;
+		const uint64_t m = __ballot(bit);
+		peers &= bit ? m : ~m;
+	}
+	return valid ? peers : 0;
+}
+
+__global__ __launch_bounds__(kRsBlock) void k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n,
+													  int shift, uint32_t* __restrict__ tile_hist,
+													  uint32_t num_tiles) {
+	__shared__ uint32_t h[kRsBuckets];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	const int lane = threadIdx.x & 63;
+	const int w = threadIdx.x >> 6;
+	const uint64_t wbase = uint64_t(blockIdx.x) * kRsTile + uint64_t(w) * (kRsTile / 4);
+	for (int r = 0; r < kRsRows; ++r) {
+		const uint64_t i = wbase + uint64_t(r) * 64 + lane;
+		const bool valid = i < n;
+		const uint32_t d = valid ? (keys[i] >> shift) & 255u : 0u;
+		const uint64_t peers = digit_peers(d, valid);
+		if (valid && (peers & lanemask_lt()) == 0)
+			atomicAdd(&h[d], uint32_t(__popcll(peers)));
+	}
+	__syncthreads();
+	tile_hist[uint64_t(threadIdx.x) * num_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Block-wide exclusive scan for blockDim.x = nthreads (multiple of 64, <= 1024).
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+	const int lane = threadIdx.x & 63;
+	const int w = threadIdx.x >> 6;
+	const int nw = blockDim.x >> 6;
+	uint32_t incl = v;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t t = __shfl_up(incl, o, 64);
+		if (lane >= o)
+			incl += t;
+	}
+	if (lane == 63)
+		wsum[w] = incl;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t acc = 0;
+		for (int k = 0; k < nw; ++k) {
+			const uint32_t s = wsum[k];
+			wsum[k] = acc;
+			acc += s;
+		}
+		wsum[nw] = acc;
+	}
+	__syncthreads();
+	const uint32_t out = wsum[w] + incl - v;
+	if (total)
+		*total = wsum[nw];
+	__syncthreads();
+	return out;
+}
+
+// In-place exclusive scan of `count` values with one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_one_block(uint32_t* __restrict__ data, uint64_t count,
+														 uint32_t* __restrict__ total_out) {
+	__shared__ uint32_t wsum[17];
+	const uint64_t per = (count + blockDim.x - 1) / blockDim.x;
+	const uint64_t b = uint64_t(threadIdx.x) * per;
+	const uint64_t e = b + per < count ? b + per : count;
+	uint32_t s = 0;
+	for (uint64_t i = b; i < e; ++i)
+		s += data[i];
+	uint32_t total;
+	uint32_t run = block_exclusive_scan(s, wsum, &total);
+	for (uint64_t i = b; i < e; ++i) {
+		const uint32_t v = data[i];
+		data[i] = run;
+		run += v;
+	}
+	if (threadIdx.x == 0 && total_out)
+		*total_out = total;
+}
+
+__global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const uint32_t* __restrict__ keys,
+														 const uint32_t* __restrict__ vals,
+														 uint32_t* __restrict__ keys_out,
+														 uint32_t* __restrict__ vals_out, uint64_t n,
+														 int shift,
+														 const uint32_t* __restrict__ tile_offsets,
+														 uint32_t num_tiles) {
+	__shared__ uint32_t base[kRsBuckets];
+	__shared__ uint32_t wcnt[4][kRsBuckets];
+	const int lane = threadIdx.x & 63;
+	const int w = threadIdx.x >> 6;
+	base[threadIdx.x] = tile_offsets[uint64_t(threadIdx.x) * num_tiles + blockIdx.x];
+#pragma unroll
+	for (int k = 0; k < 4; ++k)
+		wcnt[w][lane + 64 * k] = 0;
+	__syncthreads();
+	uint32_t mk[kRsRows], mv[kRsRows], mr[kRsRows];
+	const uint64_t wbase = uint64_t(blockIdx.x) * kRsTile + uint64_t(w) * (kRsTile / 4);
+#pragma unroll
+	for (int r = 0; r < kRsRows; ++r) {
+		const uint64_t i = wbase + uint64_t(r) * 64 + lane;
+		const bool valid = i < n;
+		const uint32_t k = valid ? keys[i] : 0u;
+		const uint32_t v = valid ? vals[i] : 0u;
+		const uint32_t d = (k >> shift) & 255u;
+		const uint64_t peers = digit_peers(d, valid);
+		const uint64_t below = peers & lanemask_lt();
+		const uint32_t rank = wcnt[w][d] + uint32_t(__popcll(below));
+		__builtin_amdgcn_wave_barrier();
+		if (valid && below == 0)
+			wcnt[w][d] += uint32_t(__popcll(peers));
+		__builtin_amdgcn_wave_barrier();
+		mk[r] = k;
+		mv[r] = v;
+		mr[r] = valid ? rank : 0xFFFFFFFFu;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < kRsRows; ++r) {
+		if (mr[r] == 0xFFFFFFFFu)
+			continue;
+		const uint32_t d = (mk[r] >> shift) & 255u;
+		uint32_t dst = base[d] + mr[r];
+		for (int q = 0; q < w; ++q)
+			dst += wcnt[q][d];
+		keys_out[dst] = mk[r];
+		vals_out[dst] = mv[r];
+	}
+}
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+															uint32_t* __restrict__ block_sums) {
+	__shared__ uint32_t wsum[5];
+	const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+	uint32_t s = 0;
+#pragma unroll
+	for (int k = 0; k < kScanItems; ++k)
+		if (base + k < n)
+			s += in[base + k];
+	uint32_t total;
+	(void)block_exclusive_scan(s, wsum, &total);
+	if (threadIdx.x == 0)
+		block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t* __restrict__ in, uint64_t n,
+														   const uint32_t* __restrict__ block_offsets,
+														   uint32_t* __restrict__ out) {
+	__shared__ uint32_t wsum[5];
+	const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+	uint32_t v[kScanItems];
+	uint32_t s = 0;
+#pragma unroll
+	for (int k = 0; k < kScanItems; ++k) {
+		v[k] = base + k < n ? in[base + k] : 0u;
+		s += v[k];
+	}
+	uint32_t run = block_exclusive_scan(s, wsum, nullptr) + block_offsets[blockIdx.x];
+#pragma unroll
+	for (int k = 0; k < kScanItems; ++k) {
+		if (base + k < n)
+			out[base + k] = run;
+		run += v[k];
+	}
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+__global__ void k_fill_f32(float* p, float v, uint64_t n) {
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+
+inline uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
+
+} // namespace
+
+uint64_t radix_sort_scratch_bytes(uint64_t n) {
+	const uint64_t tiles = (n + kRsTile - 1) / kRsTile;
+	return align256(n * 4) * 2 + align256(tiles * kRsBuckets * 4 + 4);
+}
+
+void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+					  uint32_t* vals_out, uint64_t n, int key_bits, void* scratch,
+					  uint64_t scratch_bytes, hipStream_t s) {
+	if (n == 0)
+		return;
+	if (n > 0xFFFFFFFFull)
+		raise(MARSMD_ERR_VALUE, "radix sort: more than 2^32 keys");
+	if (scratch_bytes < radix_sort_scratch_bytes(n))
+		raise(MARSMD_ERR_VALUE, "radix sort: scratch too small");
+	if (key_bits < 1 || key_bits > 32)
+		raise(MARSMD_ERR_VALUE, "radix sort: key_bits must be in [1, 32]");
+	const uint32_t tiles = uint32_t((n + kRsTile - 1) / kRsTile);
+	char* sp = static_cast<char*>(scratch);
+	uint32_t* tmp_k = reinterpret_cast<uint32_t*>(sp);
+	uint32_t* tmp_v = reinterpret_cast<uint32_t*>(sp + align256(n * 4));
+	uint32_t* hist = reinterpret_cast<uint32_t*>(sp + 2 * align256(n * 4));
+	const int passes = (key_bits + 7) / 8;
+	const uint32_t* src_k = keys_in;
+	const uint32_t* src_v = vals_in;
+	for (int p = 0; p < passes; ++p) {
+		const bool to_out = ((passes - 1 - p) % 2) == 0;
+		uint32_t* dk = to_out ? keys_out : tmp_k;
+		uint32_t* dv = to_out ? vals_out : tmp_v;
+		const int shift = 8 * p;
+		k_rs_hist<<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, hist, tiles);
+		MMD_LAUNCH_CHECK();
+		k_scan_one_block<<<1, 1024, 0, s>>>(hist, uint64_t(tiles) * kRsBuckets, nullptr);
+		MMD_LAUNCH_CHECK();
+		k_rs_scatter<<<tiles, kRsBlock, 0, s>>>(src_k, src_v, dk, dv, n, shift, hist, tiles);
+		MMD_LAUNCH_CHECK();
+		src_k = dk;
+		src_v = dv;
+	}
+}
+
+uint64_t scan_scratch_bytes(uint64_t n) {
+	const uint64_t blocks = (n + kScanTile - 1) / kScanTile;
+	return align256((blocks + 1) * 4);
+}
+
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* scratch, hipStream_t s) {
+	uint32_t* sums = static_cast<uint32_t*>(scratch);
+	if (n == 0) {
+		k_fill_u32<<<1, 64, 0, s>>>(out, 0u, 1);
+		MMD_LAUNCH_CHECK();
+		return;
+	}
+	const uint64_t blocks = (n + kScanTile - 1) / kScanTile;
+	k_scan_reduce<<<unsigned(blocks), kScanBlock, 0, s>>>(in, n, sums);
+	MMD_LAUNCH_CHECK();
+	k_scan_one_block<<<1, 1024, 0, s>>>(sums, blocks, out + n);
+	MMD_LAUNCH_CHECK();
+	k_scan_apply<<<unsigned(blocks), kScanBlock, 0, s>>>(in, n, sums, out);
+	MMD_LAUNCH_CHECK();
+}
+
+void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+	if (n == 0)
+		return;
+	k_fill_u32<<<grid_for(n, 256), 256, 0, s>>>(p, v, n);
+	MMD_LAUNCH_CHECK();
+}
+void fill_f32(float* p, float v, uint64_t n, hipStream_t s) {
+	if (n == 0)
+		return;
+	k_fill_f32<<<grid_for(n, 256), 256, 0, s>>>(p, v, n);
+	MMD_LAUNCH_CHECK();
+}
+
+} // namespace mmd

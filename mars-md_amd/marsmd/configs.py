@@ -1,0 +1,253 @@
+"""Seeded synthetic systems for the five workloads of the hot path.
+
+Every constant the workload descriptions leave open is pinned here once:
+  * pair tables: WCA(sigma, eps) + Debye-Hueckel 332.0636 q_i q_j / eps_r e^{-r/lambda} / r
+    with eps_r = 80 (the DebyeHuckelPotential default, Columb.h:30-43), lambda = 10 A;
+    below r_cap = 0.8 sigma the potential continues linearly with its slope at r_cap
+    (finite knots near r = 0); shifted so U(r_max) = 0; knots on [0, r_max] with step
+    r_max / (K - 1), so the last knot sits on the cutoff;
+  * BD: D = 150 A^2/ns, kT = 0.593 kcal/mol, dt = 1e-4 ns; Langevin: gamma = 1/ps
+    (trans_damping 1000 ns^-1), mass 100 amu, Maxwell-Boltzmann momenta
+    p = sqrt(kT m) * SQRT_CAL_TO_JOULE * N(0,1) (SimManager.cpp:969-991);
+  * positions: Philox stream STREAM_POSITIONS, uniform in the box, candidates closer
+    than 3 A (minimum image) to an earlier accepted bead are redrawn;
+  * rebuild period 20 and skin 5 A (pairlist cutoff = cutoff + 5).
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional
+
+import numpy as np
+
+from . import _capi as api
+from . import rng
+from .core import Grid, ParticleType, PeriodicBox, TabulatedPotential, make_pair_tag
+from .system import RunParams, SystemSpec
+
+COULOMB = 332.0636
+KT = 0.593
+SQRT_CAL_TO_JOULE = 2.046167337
+DEG_TO_RAD = np.float32(math.pi / 180.0)
+
+
+def wca_dh(r: np.ndarray, sigma: float, eps: float, qq: float, eps_r: float = 80.0, debye: float = 10.0):
+    r = np.asarray(r, np.float64)
+
+    def raw(x):
+        sr6 = (sigma / x) ** 6
+        wca = np.where(x < 2.0 ** (1.0 / 6.0) * sigma, 4.0 * eps * (sr6 * sr6 - sr6) + eps, 0.0)
+        dh = COULOMB * qq / eps_r * np.exp(-x / debye) / x
+        return wca + dh
+
+    rc = 0.8 * sigma
+    h = 1e-6
+    slope = (raw(np.array(rc + h)) - raw(np.array(rc - h))) / (2 * h)
+    return np.where(r < rc, raw(np.array(rc)) + slope * (r - rc), raw(np.maximum(r, rc)))
+
+
+def pair_table(sigma=5.0, eps=0.5, qq=1.0, rmax=30.0, knots=1024, eps_r=80.0, debye=10.0) -> TabulatedPotential:
+    step = rmax / (knots - 1)
+    r = np.arange(knots) * step
+    u = wca_dh(r, sigma, eps, qq, eps_r, debye) - wca_dh(np.array(rmax), sigma, eps, qq, eps_r, debye)
+    return TabulatedPotential(u.astype(np.float32), 0.0, step)
+
+
+def harmonic_bond_table(k=10.0, r0=5.0, rmax=20.0, knots=512) -> TabulatedPotential:
+    step = rmax / (knots - 1)
+    r = np.arange(knots) * step
+    return TabulatedPotential((0.5 * k * (r - r0) ** 2).astype(np.float32), 0.0, step)
+
+
+def cosine_angle_table(k=2.0, knots=256) -> TabulatedPotential:
+    """U(theta) = k (1 + cos theta) on [0, 180] degrees, abscissa converted to radians
+    as Table::convert_angular_abscissa_to_radians (Tables.h:109-119); periodic."""
+    step_deg = np.float32(180.0 / (knots - 1))
+    th = np.arange(knots) * float(step_deg)
+    u = k * (1.0 + np.cos(np.deg2rad(th)))
+    start = np.float32(0.0) * DEG_TO_RAD
+    step = np.float32(step_deg * DEG_TO_RAD)
+    return TabulatedPotential(u.astype(np.float32), float(start), float(step), is_periodic=True)
+
+
+def uniform_positions(n: int, L, seed: int, min_sep: float = 3.0, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Uniform positions with rejection of candidates within min_sep of an earlier
+    accepted bead (minimum image). Deterministic for a seed."""
+    from scipy.spatial import cKDTree
+
+    L = np.asarray(L, np.float64) * np.ones(3)
+    pos = np.zeros((0, 3))
+    next_ctr = 0
+    need = n
+    accepted = np.zeros((0, 3))
+    rounds = 0
+    while need > 0:
+        rounds += 1
+        if rounds > 64:
+            raise RuntimeError("could not place particles at this density")
+        u = rng.uniform(seed, rng.STREAM_POSITIONS, need, offset=next_ctr)[:, :3].astype(np.float64)
+        next_ctr += need
+        cand = np.minimum(u * L, np.nextafter(L, 0))
+        allp = np.vstack([accepted, cand])
+        tree = cKDTree(allp, boxsize=L)
+        pairs = tree.query_pairs(min_sep, output_type="ndarray")
+        ok = np.ones(len(allp), bool)
+        na = len(accepted)
+        if len(pairs):
+            # a candidate is rejected when an accepted bead, or an earlier surviving
+            # candidate, lies within min_sep: process candidates in order
+            pairs = np.sort(pairs, axis=1)
+            nbrs = {}
+            for a, b in pairs:
+                nbrs.setdefault(int(b), []).append(int(a))
+            for b in sorted(nbrs):
+                if b < na:
+                    continue
+                if any(ok[a] for a in nbrs[b]):
+                    ok[b] = False
+        accepted = allp[ok]
+        need = n - len(accepted)
+    return (accepted[:n] + np.asarray(origin)).astype(np.float32)
+
+
+def maxwell_momenta(n: int, masses: np.ndarray, kT: float, seed: int) -> np.ndarray:
+    z = rng.normal(seed, rng.STREAM_MOMENTA, n)
+    return (np.sqrt(kT * masses)[:, None] * SQRT_CAL_TO_JOULE * z).astype(np.float32)
+
+
+# ------------------------------------------------------------------------ configs
+def config0(n: int = 10_000, L: float = 250.0, seed: int = 42, steps: int = 1000) -> SystemSpec:
+    """Plain BD: point beads of one type in a periodic cube, WCA+DH table (1024 knots),
+    cutoff 30, skin 5, D = 150, kT = 0.593, dt = 100 fs, Philox seed 42."""
+    pos = uniform_positions(n, L, seed)
+    t = ParticleType(mass=1.0, diffusion=(150.0, 150.0, 150.0))
+    spec = SystemSpec(
+        box=PeriodicBox((L, L, L)),
+        types=[t],
+        positions=pos,
+        type_ids=np.zeros(n, np.int32),
+        params=RunParams(cutoff=30.0, pairlist_cutoff=35.0, rebuild_period=20, timestep=1e-4, kT=KT,
+                         integrator=api.INTEGRATOR_BROWNIAN),
+        nb_tables=[pair_table(qq=1.0)],
+        seed=seed,
+        name=f"bd-{n}-beads-L{L:g}",
+    )
+    spec.steps = steps
+    return spec
+
+
+def config1(chains: int = 200, beads: int = 500, L: float = 800.0, seed: int = 42, steps: int = 1000) -> SystemSpec:
+    """Bead-spring polymers: random-walk chains (step r0 = 5 A, a bead is redrawn when it
+    lands within 3 A of the bead two back), two types in blocks of 50 beads, harmonic
+    bond table U = k/2 (r - r0)^2 (512 knots on [0, 20]), cosine angle table k(1 + cos)
+    (256 knots, degrees), WCA+DH for 3 type pairs with charges (+0.5, -0.5), 1-2 and 1-3
+    exclusions, BD, reorder period 20."""
+    n = chains * beads
+    r0 = 5.0
+    starts = rng.uniform(seed, rng.STREAM_CHAINS, chains, draw=0)[:, :3].astype(np.float64) * L
+    pos = np.zeros((chains, beads, 3))
+    pos[:, 0] = starts
+    for b in range(1, beads):
+        d = rng.normal(seed, rng.STREAM_CHAINS, chains, draw=b, offset=0)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        cand = pos[:, b - 1] + r0 * d
+        if b >= 2:
+            bad = np.linalg.norm(cand - pos[:, b - 2], axis=1) < 3.0
+            tries = 0
+            while bad.any():
+                tries += 1
+                d2 = rng.normal(seed, rng.STREAM_CHAINS, chains, draw=b, offset=(tries * chains))
+                d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+                cand[bad] = pos[bad, b - 1] + r0 * d2[bad]
+                bad = np.linalg.norm(cand - pos[:, b - 2], axis=1) < 3.0
+        pos[:, b] = cand
+    pos = np.mod(pos.reshape(n, 3), L)
+    pos = np.minimum(pos, np.nextafter(L, 0)).astype(np.float32)
+    type_ids = np.tile((np.arange(beads) // 50) % 2, chains).astype(np.int32)
+    idx = np.arange(n).reshape(chains, beads)
+    bonds = np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], axis=1).astype(np.int32)
+    angles = np.stack([idx[:, :-2].ravel(), idx[:, 1:-1].ravel(), idx[:, 2:].ravel()], axis=1).astype(np.int32)
+    excl = np.vstack([bonds, angles[:, [0, 2]]]).astype(np.int32)
+    q = [0.5, -0.5]
+    nb = [pair_table(qq=q[0] * q[0]), pair_table(qq=q[0] * q[1]), pair_table(qq=q[1] * q[1])]
+    types = [ParticleType(mass=1.0, charge=qi, diffusion=(150.0, 150.0, 150.0)) for qi in q]
+    spec = SystemSpec(
+        box=PeriodicBox((L, L, L)),
+        types=types,
+        positions=pos,
+        type_ids=type_ids,
+        params=RunParams(cutoff=30.0, pairlist_cutoff=35.0, rebuild_period=20, reorder_period=20, timestep=1e-4,
+                         kT=KT, integrator=api.INTEGRATOR_BROWNIAN),
+        nb_tables=nb,
+        bonds=bonds,
+        bond_table=np.zeros(len(bonds), np.int32),
+        bond_tables=[harmonic_bond_table()],
+        angles=angles,
+        angle_table=np.zeros(len(angles), np.int32),
+        angle_tables=[cosine_angle_table()],
+        exclusions=excl,
+        seed=seed,
+        name=f"bd-polymers-{chains}x{beads}-L{L:g}",
+    )
+    spec.steps = steps
+    return spec
+
+
+def membrane_grid(L: float = 2000.0, n: int = 256, height: float = 2.0, thickness: float = 40.0,
+                  pore_radius: float = 20.0, width: float = 5.0) -> Grid:
+    """Slab membrane centred at z = L/2 (thickness 40 A, tanh edges of width 5 A) with a
+    cylindrical pore of radius 20 A (40 A diameter) on the box axis; V = height inside
+    the slab away from the pore."""
+    h = L / n
+    c = np.arange(n) * h
+    x = c[:, None, None]
+    y = c[None, :, None]
+    z = c[None, None, :]
+    z0 = L / 2
+    slab = 0.5 * (np.tanh((z - z0 + thickness / 2) / width) - np.tanh((z - z0 - thickness / 2) / width))
+    r = np.sqrt((x - L / 2) ** 2 + (y - L / 2) ** 2)
+    outside = 0.5 * (1.0 + np.tanh((r - pore_radius) / width))
+    v = (height * slab * outside).astype(np.float32)
+    return Grid(v, origin=(0.0, 0.0, 0.0), spacing=(h, h, h), boundary_condition=api.BC_PERIODIC)
+
+
+def config2(n: int = 1_000_000, L: float = 2000.0, seed: int = 42, steps: int = 500,
+            grid_n: int = 256) -> SystemSpec:
+    """Large system with an external field: 3 types (charges +1, -1, +0.5), 6 WCA+DH
+    pair tables (1024 knots), 256^3 membrane grid sampled tricubically (type scales
+    +1, -1, +0.5 give the +-2 kcal/mol membrane), Langevin gamma = 1/ps."""
+    pos = uniform_positions(n, L, seed)
+    ids = rng.uniform(seed, rng.STREAM_POSITIONS ^ 0x7, n)[:, 0]
+    type_ids = np.minimum((ids * 3).astype(np.int32), 2)
+    q = [1.0, -1.0, 0.5]
+    scales = [1.0, -1.0, 0.5]
+    mass = 100.0
+    gamma = 1000.0
+    types = [ParticleType(mass=mass, charge=q[i], diffusion=(KT / (mass * gamma),) * 3,
+                          trans_damping=(gamma, gamma, gamma), pmf_terms=[(0, scales[i], -1)]) for i in range(3)]
+    nb = []
+    tags = np.zeros((3, 3), np.uint32)
+    k = 0
+    for i in range(3):
+        for j in range(i, 3):
+            nb.append(pair_table(qq=q[i] * q[j]))
+            tags[i, j] = tags[j, i] = make_pair_tag(api.PAIR_TERM_TABULATED, k)
+            k += 1
+    masses = np.full(n, mass)
+    mom = maxwell_momenta(n, masses, KT, seed)
+    spec = SystemSpec(
+        box=PeriodicBox((L, L, L)),
+        types=types,
+        positions=pos,
+        type_ids=type_ids,
+        momenta=mom,
+        params=RunParams(cutoff=30.0, pairlist_cutoff=35.0, rebuild_period=20, timestep=1e-4, kT=KT,
+                         integrator=api.INTEGRATOR_LANGEVIN, interpolation=api.INTERP_CUBIC),
+        nb_tables=nb,
+        term_matrix=tags,
+        grids=[membrane_grid(L, grid_n)],
+        seed=seed,
+        name=f"langevin-{n}-beads-3types-L{L:g}-grid{grid_n}",
+    )
+    spec.steps = steps
+    return spec

@@ -1,0 +1,105 @@
+"""Patch (src/PatchOperation/Patch.h:48-645) over the C ABI."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _capi as api
+from .core import ZOrderPairlist, _stream
+from .system import RunParams, SystemSpec
+
+
+class Patch:
+    """One patch on one GPU. State lives in HBM inside the library; particle arrays
+    cross the boundary in the reference's canonical order (float4 records)."""
+
+    def __init__(self, spec: SystemSpec, capacity: Optional[int] = None, load_particles: bool = True):
+        desc, keep = spec.to_desc(capacity)
+        h = C.c_void_p()
+        api.call("marsmd_patch_create", C.byref(desc), C.byref(h))
+        self._h = h
+        self.spec = spec
+        self.n = 0
+        api.call("marsmd_patch_set_base_seed", self._h, spec.seed)
+        if load_particles:
+            self.copy_particles_from_host(spec.pos4(), spec.mom4(), spec.type_ids)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            api.lib().marsmd_patch_destroy(self._h)
+            self._h = None
+
+    def set_base_seed(self, seed: int):
+        api.call("marsmd_patch_set_base_seed", self._h, seed)
+
+    def copy_particles_from_host(self, pos4: np.ndarray, mom4: Optional[np.ndarray], type_ids: np.ndarray):
+        pos4 = np.ascontiguousarray(pos4, np.float32)
+        mom = None if mom4 is None else np.ascontiguousarray(mom4, np.float32)
+        types = np.ascontiguousarray(type_ids, np.int32)
+        api.call("marsmd_patch_copy_particles_from_host", self._h, pos4.ctypes.data,
+                 None if mom is None else mom.ctypes.data, types.ctypes.data, len(types))
+        self.n = len(types)
+
+    def copy_particles_to_host(self):
+        n = self.n
+        pos = np.zeros((n, 4), np.float32)
+        mom = np.zeros((n, 4), np.float32)
+        fe = np.zeros((n, 4), np.float32)
+        ty = np.zeros(n, np.int32)
+        ids = np.zeros(n, np.int32)
+        api.call("marsmd_patch_copy_particles_to_host", self._h, pos.ctypes.data, mom.ctypes.data, fe.ctypes.data,
+                 ty.ctypes.data, ids.ctypes.data, n)
+        return {"pos": pos, "mom": mom, "force": fe, "type": ty, "id": ids}
+
+    def calculate_nonbonded_forces(self, pairlist_cutoff: float, interaction_cutoff: float, step: int,
+                                   rebuild_period: int, electric_field=(0.0, 0.0, 0.0),
+                                   interpolation: int = api.INTERP_LINEAR, compute_energy: bool = False,
+                                   stream=None):
+        ef = (C.c_float * 3)(*map(float, electric_field))
+        api.call("marsmd_patch_calculate_nonbonded_forces", self._h, float(pairlist_cutoff),
+                 float(interaction_cutoff), step, rebuild_period, ef, interpolation, 1 if compute_energy else 0,
+                 _stream(stream))
+
+    def calculate_bonded_forces(self, compute_energy: bool = False, stream=None):
+        api.call("marsmd_patch_calculate_bonded_forces", self._h, 1 if compute_energy else 0, _stream(stream))
+
+    def integrate_motion(self, dt: float, kT: float, integrator: int, step: int, stream=None):
+        api.call("marsmd_patch_integrate_motion", self._h, float(dt), float(kT), integrator, step, _stream(stream))
+
+    def finish_deferred_kick(self, dt: float, stream=None):
+        api.call("marsmd_patch_finish_deferred_kick", self._h, float(dt), _stream(stream))
+
+    def reorder_particles(self, stream=None):
+        api.call("marsmd_patch_reorder_particles", self._h, _stream(stream))
+
+    def get_pairlist(self) -> ZOrderPairlist:
+        p = C.c_void_p()
+        api.call("marsmd_patch_get_pairlist", self._h, C.byref(p))
+        return ZOrderPairlist(0, _handle=p.value)
+
+    def run(self, num_steps: int, first_step: int = 0, params: Optional[RunParams] = None, stream=None):
+        p = (params or self.spec.params).to_c()
+        api.call("marsmd_patch_run", self._h, C.byref(p), first_step, num_steps, _stream(stream))
+
+    def maybe_reorder(self, step: int, params: Optional[RunParams] = None, stream=None) -> bool:
+        """ParticleReorderManager::maybe_reorder (ReorderManager.cpp:8-25) with the
+        reorderPeriod fallback of SimManager::execute_force_calculation (:343-349)."""
+        P = params or self.spec.params
+        rp = P.reorder_period if P.reorder_period > 0 else P.rebuild_period
+        if rp > 0 and step != 0 and (step - 1) % rp == 0 and getattr(self, "_prepared", False):
+            self.reorder_particles(stream)
+            return True
+        return False
+
+    def step_unfused(self, step: int, params: Optional[RunParams] = None, stream=None):
+        """One SimManager step through the separate reference-shaped calls."""
+        P = params or self.spec.params
+        self.maybe_reorder(step, P, stream)
+        energy = P.energy_period > 0 and step % P.energy_period == 0
+        self.calculate_nonbonded_forces(P.pairlist_cutoff, P.cutoff, step, P.rebuild_period, P.electric_field,
+                                        P.interpolation, energy, stream)
+        self._prepared = True
+        self.calculate_bonded_forces(energy, stream)
+        self.integrate_motion(P.timestep, P.kT, P.integrator, step, stream)

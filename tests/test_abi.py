@@ -1,0 +1,75 @@
+"""The C ABI: the library loads without a GPU, exports every function declared in
+include/marsmd.h, and the ctypes structure layouts match the C compiler's."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from marsmd import _capi as api
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "marsmd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:marsmd_status|const char\*|uint64_t)\s+(marsmd_\w+)\s*\(", text, re.M)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = api.lib()
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(api.PROTOTYPES), set(names) ^ set(api.PROTOTYPES)
+
+
+def test_version_and_error_strings_without_gpu():
+    lib = api.lib()
+    assert lib.marsmd_version().decode().startswith("marsmd-hip")
+    assert lib.marsmd_last_error() is not None
+
+
+def test_null_arguments_are_rejected_without_touching_the_device():
+    with pytest.raises(api.MarsError):
+        api.call("marsmd_zorder_create", 10, None)
+    with pytest.raises(api.MarsError):
+        api.call("marsmd_pairlist_get_num_pairs", None, None)
+    with pytest.raises(api.MarsError):
+        api.call("marsmd_patch_run", None, None, 0, 1, None)
+
+
+STRUCTS = {
+    "marsmd_box": api.Box,
+    "marsmd_table": api.Table,
+    "marsmd_exclusion_view": api.ExclusionView,
+    "marsmd_grid_term": api.GridTerm,
+    "marsmd_grid_view": api.GridView,
+    "marsmd_particle_types": api.ParticleTypes,
+    "marsmd_particle_view": api.ParticleView,
+    "marsmd_pairlist_stats": api.PairlistStats,
+    "marsmd_patch_desc": api.PatchDesc,
+    "marsmd_run_params": api.RunParams,
+}
+
+
+def test_struct_layouts_match_c(tmp_path):
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in STRUCTS.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(l.rsplit(" ", 1) for l in out if l)
+    for cname, py in STRUCTS.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, f"{cname}.{fname}"

@@ -1,0 +1,215 @@
+"""GPU parity: Morton keys, stable radix sort, inverse map and the pair set through
+the C ABI, bit for bit against the oracle (ZOrderSort / ZOrderPairlist semantics)."""
+import numpy as np
+import pytest
+
+import marsmd
+from marsmd import PeriodicBox, configs
+from oracle import pyoracle as orc
+from tests.conftest import sorted_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(cuda, a, dtype=None):
+    t = cuda.from_numpy(np.ascontiguousarray(a))
+    return t.cuda() if dtype is None else t.to(dtype).cuda()
+
+
+def test_philox_bitexact(cuda):
+    for seed, ctr, gs, c1 in [(0, 0, 0, 0), (42, 1000, 7, 0x5324120), (0xFFFFFFFFFFFFFFFF, 5, 99, 0x1356914)]:
+        g = marsmd.philox_draw(seed, ctr, gs, c1, draws=2, n=257).cpu().numpy().view(np.uint32)
+        for i in (0, 1, 100, 256):
+            o = orc.philox_draw(seed, ctr + i, gs, c1, draws=2)
+            assert np.array_equal(g[i], o)
+
+
+@pytest.mark.parametrize("n", [1, 63, 4096, 4097, 100_000, 1_000_000])
+def test_morton_sort_inverse_bitexact(cuda, n):
+    rs = np.random.default_rng(n)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rs.uniform(-10.0, 510.0, size=(n, 3))  # some outside the domain: clamped
+    lo, hi = [0.0, 0.0, 0.0], [500.0, 500.0, 500.0]
+    z = marsmd.ZOrderSort(n)
+    z.sort_particles(_dev(cuda, pos), n, lo, hi)
+    cuda.cuda.synchronize()
+    keys = orc.morton_encode(pos, np.float32(lo), np.float32(hi))
+    sk, sv = orc.stable_sort_pairs(keys)
+    assert np.array_equal(z.get_morton_codes().cpu().numpy().view(np.uint32), sk)
+    assert np.array_equal(z.get_sorted_indices().cpu().numpy().view(np.uint32), sv)
+    inv = np.empty(n, np.uint32)
+    inv[sv] = np.arange(n, dtype=np.uint32)
+    assert np.array_equal(z.get_inverse_indices().cpu().numpy().view(np.uint32), inv)
+
+
+def test_sort_stability_with_heavy_duplicates(cuda):
+    # every particle in one of 8 positions: the permutation must be iota within a key
+    n = 50_000
+    rs = np.random.default_rng(3)
+    choices = rs.uniform(0, 100, size=(8, 3)).astype(np.float32)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = choices[rs.integers(0, 8, n)]
+    z = marsmd.ZOrderSort(n)
+    z.sort_particles(_dev(cuda, pos), n, [0, 0, 0], [100, 100, 100])
+    cuda.cuda.synchronize()
+    keys = orc.morton_encode(pos, np.zeros(3, np.float32), np.full(3, 100, np.float32))
+    assert np.array_equal(z.get_sorted_indices().cpu().numpy(), np.argsort(keys, kind="stable"))
+
+
+def test_radix_sort_full_32bit_keys(cuda):
+    import ctypes as C
+
+    n = 300_000
+    rs = np.random.default_rng(4)
+    keys = rs.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+    keys[::7] = keys[3]  # duplicates
+    vals = np.arange(n, dtype=np.uint32)
+    kd, vd = _dev(cuda, keys.view(np.int32)), _dev(cuda, vals.view(np.int32))
+    ko, vo = cuda.empty_like(kd), cuda.empty_like(vd)
+    nbytes = marsmd.lib().marsmd_radix_sort_scratch_bytes(n)
+    scratch = cuda.empty(nbytes, dtype=cuda.uint8, device="cuda")
+    marsmd._capi.call("marsmd_radix_sort_pairs", kd.data_ptr(), vd.data_ptr(), ko.data_ptr(), vo.data_ptr(), n, 32,
+                      scratch.data_ptr(), nbytes, cuda.cuda.current_stream().cuda_stream)
+    cuda.cuda.synchronize()
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(vo.cpu().numpy().view(np.uint32), order)
+    assert np.array_equal(ko.cpu().numpy().view(np.uint32), keys[order])
+
+
+def _gpu_pairs(cuda, pos, box, rc, excl=None, types=None, max_pairs=1 << 62):
+    n = len(pos)
+    pl = marsmd.ZOrderPairlist(max(n, 1), max_pairs)
+    pl.set_periodic_box(box)
+    pl.set_exclusions(excl)
+    pl.build_pairlist(_dev(cuda, pos), n, rc)
+    cuda.cuda.synchronize()
+    return pl, sorted_pairs(pl.get_neighbor_pairs().cpu().numpy())
+
+
+@pytest.mark.parametrize("periodic", [(True, True, True), (False, False, False), (True, False, True)])
+@pytest.mark.parametrize("n,L,rc", [(2000, 80.0, 9.0), (5000, 60.0, 12.0), (300, 30.0, 14.0)])
+def test_pair_set_bitexact_random(cuda, periodic, n, L, rc):
+    rs = np.random.default_rng(n + int(rc))
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rs.uniform(0, L, size=(n, 3))
+    box = PeriodicBox((L, L, L), periodic=periodic)
+    pl, gp = _gpu_pairs(cuda, pos, box, rc)
+    ref = orc.build_pairlist(pos, box, rc)
+    assert np.array_equal(gp, ref["pairs"].astype(np.int64))
+    z = pl.get_sorter()
+    assert np.array_equal(z.get_sorted_indices().cpu().numpy().view(np.uint32), ref["sorted_idx"])
+    assert np.array_equal(z.get_morton_codes().cpu().numpy().view(np.uint32), ref["codes"])
+
+
+def test_pair_set_positions_on_box_faces(cuda):
+    # particles exactly on the periodic faces and outside the box (wrap_diff paths)
+    rs = np.random.default_rng(11)
+    n, L = 3000, 50.0
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rs.uniform(0, L, size=(n, 3))
+    pos[:100, 0] = 0.0
+    pos[100:200, 1] = L
+    pos[200:260, 2] = -3.0
+    pos[260:300, 0] = L + 2.0
+    box = PeriodicBox((L, L, L))
+    _, gp = _gpu_pairs(cuda, pos, box, 10.0)
+    ref = orc.build_pairlist(pos, box, 10.0)
+    assert np.array_equal(gp, ref["pairs"].astype(np.int64))
+
+
+def test_known_answers_and_exclusions(cuda):
+    pos = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [2, 0, 0, 0]], np.float32)
+    tb = PeriodicBox((64.0, 64.0, 64.0), periodic=(False, False, False), origin=(-4.0, -4.0, -4.0))
+    pl, gp = _gpu_pairs(cuda, pos, tb, 1.1)
+    assert pl.get_num_pairs() == 2
+    pl, gp = _gpu_pairs(cuda, pos, tb, 2.5)
+    assert pl.get_num_pairs() == 3
+    four = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [2, 0, 0, 0], [3, 0, 0, 0]], np.float32)
+    eb = PeriodicBox((60.0, 60.0, 60.0), periodic=(False, False, False), origin=(-30.0, -30.0, -30.0))
+
+    def csr(off, exc, body=None, n=4):
+        return marsmd.ExclusionCSR(
+            None if off is None else _dev(cuda, np.array(off, np.int32)),
+            None if exc is None else _dev(cuda, np.array(exc, np.int32)),
+            n if off is not None else 0,
+            None if body is None else _dev(cuda, np.array(body, np.int32)))
+
+    cases = [
+        (None, [(0, 1), (0, 2), (1, 2), (1, 3), (2, 3)]),
+        (csr([0, 1, 3, 3, 4], [1, 0, 3, 1]), [(0, 2), (1, 2), (2, 3)]),
+        (csr([0, 1, 2, 2, 3], [1, 0, 1]), [(0, 2), (1, 2), (1, 3), (2, 3)]),
+        (csr([0, 2, 5, 8, 10], [1, 2, 0, 2, 3, 0, 1, 3, 1, 2]), []),
+        (csr(None, None, [7, 7, 9, -1]), [(0, 2), (1, 2), (1, 3), (2, 3)]),
+        (csr(None, None, [-1, -1, -1, -1]), [(0, 1), (0, 2), (1, 2), (1, 3), (2, 3)]),
+        (csr([0, 0, 0, 1, 2], [3, 2], [5, 5, -1, -1]), [(0, 2), (1, 2), (1, 3)]),
+    ]
+    for ex, expect in cases:
+        _, gp = _gpu_pairs(cuda, four, eb, 2.5, ex)
+        assert [tuple(p) for p in gp.tolist()] == expect
+
+
+def test_config0_pairs_and_reference_cells(cuda):
+    spec = configs.config0()
+    pos = spec.pos4()
+    pl, gp = _gpu_pairs(cuda, pos, spec.box, 35.0)
+    ref = orc.build_pairlist(pos, spec.box, 35.0)
+    assert np.array_equal(gp, ref["pairs"].astype(np.int64))
+    st = pl.get_statistics()
+    m, r, slots = ref["grid"]
+    assert (st.cell_grid_bits, tuple(st.cell_radii), st.neighbors_per_cell) == (m, r, slots)
+    b, e, nb = pl.get_reference_cells()
+    ob, oe, onb = orc.cell_tables(ref["codes"], m, r, slots, [1, 1, 1])
+    assert np.array_equal(b.cpu().numpy().view(np.uint32), ob)
+    assert np.array_equal(e.cpu().numpy().view(np.uint32), oe)
+    assert np.array_equal(nb.cpu().numpy().view(np.uint32), onb)
+
+
+def test_config1_pairs_with_exclusions(cuda):
+    spec = configs.config1(chains=40, beads=250, L=400.0)
+    pos = spec.pos4()
+    off, exc, ne = marsmd.exclusion_csr_from_pairs(spec.exclusions)
+    ex = marsmd.ExclusionCSR(_dev(cuda, off), _dev(cuda, exc), ne)
+    _, gp = _gpu_pairs(cuda, pos, spec.box, 35.0, ex)
+    ref = orc.build_pairlist(pos, spec.box, 35.0, off, exc, ne)
+    assert np.array_equal(gp, ref["pairs"].astype(np.int64))
+
+
+def test_million_bead_pair_set(cuda):
+    n, L = 1_000_000, 2000.0
+    rs = np.random.default_rng(2024)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rs.uniform(0, L, size=(n, 3))
+    box = PeriodicBox((L, L, L))
+    pl, gp = _gpu_pairs(cuda, pos, box, 35.0)
+    ref = orc.build_pairlist(pos, box, 35.0)
+    assert len(gp) == len(ref["pairs"])
+    assert np.array_equal(gp, ref["pairs"].astype(np.int64))
+    st = pl.get_statistics()
+    assert st.reference_grid_complete == 1
+
+
+def test_needs_update_and_overflow_bookkeeping(cuda):
+    rs = np.random.default_rng(8)
+    n, L = 4000, 80.0
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rs.uniform(0, L, size=(n, 3))
+    box = PeriodicBox((L, L, L))
+    d = _dev(cuda, pos)
+    pl = marsmd.ZOrderPairlist(n, 1 << 40)
+    pl.set_periodic_box(box)
+    pl.build_pairlist(d, n, 12.0)
+    assert not pl.needs_update(d, n, 5.0)
+    moved = pos.copy()
+    moved[17, 0] = (moved[17, 0] + 2.6) % L
+    assert pl.needs_update(_dev(cuda, moved), n, 5.0)
+    moved[17, 0] = (pos[17, 0] + 2.4) % L
+    assert not pl.needs_update(_dev(cuda, moved), n, 5.0)
+    # capacity ceiling: fatal past the cap (ZOrderPairlist.cpp:292-306)
+    small = marsmd.ZOrderPairlist(n, 10)
+    small.set_periodic_box(box)
+    with pytest.raises(marsmd.MarsError):
+        small.build_pairlist(d, n, 12.0)
+    # a zero-size box is rejected (ZOrderPairlist.cpp:118-126)
+    nobox = marsmd.ZOrderPairlist(n)
+    with pytest.raises(marsmd.MarsError):
+        nobox.build_pairlist(d, n, 12.0)
