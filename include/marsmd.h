@@ -15,6 +15,8 @@
  *   - Launches are ordered on the hipStream_t passed (NULL = legacy default stream)
  *     and never synchronise implicitly, except where a function returns a host value.
  *   - The caller owns every buffer it passes; scratch belongs to opaque handles.
+ *   - Structures cross the boundary by pointer, never by value (some FFIs, CPython
+ *     3.10 ctypes among them, mis-classify by-value structs that contain arrays).
  *
  * Each declaration cites the reference interface it replaces (paths relative to the
  * reference tree).
@@ -211,7 +213,7 @@ marsmd_status marsmd_pairlist_destroy(marsmd_pairlist* pl);
 /* Pairlist::set_periodic_box (host struct). */
 marsmd_status marsmd_pairlist_set_periodic_box(marsmd_pairlist* pl, const marsmd_box* box);
 /* Pairlist::set_exclusions(ExclusionView) — pointers borrowed until the next build. */
-marsmd_status marsmd_pairlist_set_exclusions(marsmd_pairlist* pl, marsmd_exclusion_view view);
+marsmd_status marsmd_pairlist_set_exclusions(marsmd_pairlist* pl, const marsmd_exclusion_view* view);
 /* Pairlist::set_max_pairs_ceiling (Pairlist.h:24-35): the regrow cap in pairs. */
 marsmd_status marsmd_pairlist_set_max_pairs_ceiling(marsmd_pairlist* pl, uint64_t ceiling);
 /* ZOrderPairlist::build_pairlist(positions, n, pairlist_cutoff). type_ids may be NULL
@@ -255,9 +257,9 @@ marsmd_status marsmd_launch_resolve_pair_tables(const int32_t* pairs, const int3
  * Pairwise.h:184-305). tables: device array of marsmd_table. pair_tag may be NULL
  * (every pair takes enabled_terms). Only the TABULATED term is evaluated on this
  * path; other enabled bits are rejected with MARSMD_ERR_VALUE. */
-marsmd_status marsmd_launch_pairwise_nonbonded(const int32_t* pairs, marsmd_particle_view view,
+marsmd_status marsmd_launch_pairwise_nonbonded(const int32_t* pairs, const marsmd_particle_view* view,
                                                const uint32_t* pair_tag,
-                                               const marsmd_table* tables, marsmd_box box,
+                                               const marsmd_table* tables, const marsmd_box* box,
                                                int32_t get_energy, uint64_t num_pairs,
                                                float cutoff_squared, uint32_t enabled_terms,
                                                marsmd_stream stream);
@@ -266,35 +268,35 @@ marsmd_status marsmd_launch_pairwise_nonbonded(const int32_t* pairs, marsmd_part
 marsmd_status marsmd_launch_tabulated_bonds(const int32_t* bonds, const float* pos4,
                                             float* force_energy, const marsmd_table* tables,
                                             const int32_t* table_indices, const int32_t* forms,
-                                            marsmd_box box, int32_t get_energy,
+                                            const marsmd_box* box, int32_t get_energy,
                                             uint64_t num_bonds, marsmd_stream stream);
 /* launch_tabulated_angles (DeviceBondedInteraction.h:520-546; TabulatedAngleComputer
  * BondComputer.h:204-269). angles: 16-byte int records {i, j, k, pad}. */
 marsmd_status marsmd_launch_tabulated_angles(const int32_t* angles, const float* pos4,
                                              float* force_energy, const marsmd_table* tables,
                                              const int32_t* table_indices, const int32_t* forms,
-                                             marsmd_box box, int32_t get_energy,
+                                             const marsmd_box* box, int32_t get_energy,
                                              uint64_t num_angles, marsmd_stream stream);
 /* launch_BD (src/PatchOperation/Integrator.h:25-54; BDIntegrate BD.h:44-92). grids:
  * device array of marsmd_grid_view (NULL = none). */
-marsmd_status marsmd_launch_bd(marsmd_particle_view view, const marsmd_particle_types* types,
+marsmd_status marsmd_launch_bd(const marsmd_particle_view* view, const marsmd_particle_types* types,
                                float timestep, uint64_t current_step, float kT,
-                               uint64_t num_particles, marsmd_box box, uint64_t base_seed,
+                               uint64_t num_particles, const marsmd_box* box, uint64_t base_seed,
                                uint32_t base_ctr, const marsmd_grid_view* grids,
                                const float electric_field[3], int32_t interpolation,
                                marsmd_stream stream);
 /* launch_BAOAB (Integrator.h:56-85; BAOABIntegrate BAOAB.h:46-120). */
-marsmd_status marsmd_launch_baoab(marsmd_particle_view view, const marsmd_particle_types* types,
-                                  marsmd_box box, float timestep, uint64_t current_step, float kT,
+marsmd_status marsmd_launch_baoab(const marsmd_particle_view* view, const marsmd_particle_types* types,
+                                  const marsmd_box* box, float timestep, uint64_t current_step, float kT,
                                   uint64_t num_particles, uint64_t base_seed, uint32_t base_ctr,
                                   marsmd_stream stream);
 /* launch_BAOAB_LastUpdate (Integrator.h:97-123; BAOAB.h:151-172). */
-marsmd_status marsmd_launch_baoab_last_update(marsmd_particle_view view,
+marsmd_status marsmd_launch_baoab_last_update(const marsmd_particle_view* view,
                                               const marsmd_particle_types* types,
                                               float timestep, uint64_t num_particles,
                                               marsmd_stream stream);
 /* launch_PMF (Integrator.h:127-144; ComputePMFKernel Pmf.h:126-152). */
-marsmd_status marsmd_launch_pmf(marsmd_particle_view view, const marsmd_particle_types* types,
+marsmd_status marsmd_launch_pmf(const marsmd_particle_view* view, const marsmd_particle_types* types,
                                 uint64_t num_particles, const marsmd_grid_view* grids,
                                 const float electric_field[3], int32_t interpolation,
                                 marsmd_stream stream);

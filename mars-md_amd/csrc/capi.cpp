@@ -138,10 +138,10 @@ marsmd_status marsmd_pairlist_set_periodic_box(marsmd_pairlist* pl, const marsmd
 		pl->impl->set_box(*box);
 	});
 }
-marsmd_status marsmd_pairlist_set_exclusions(marsmd_pairlist* pl, marsmd_exclusion_view view) {
+marsmd_status marsmd_pairlist_set_exclusions(marsmd_pairlist* pl, const marsmd_exclusion_view* view) {
 	return guard([&] {
 		need(pl, "pairlist");
-		pl->impl->excl = view;
+		pl->impl->excl = view ? *view : marsmd_exclusion_view{};
 	});
 }
 marsmd_status marsmd_pairlist_set_max_pairs_ceiling(marsmd_pairlist* pl, uint64_t ceiling) {
@@ -232,67 +232,77 @@ marsmd_status marsmd_launch_resolve_pair_tables(const int32_t* pairs, const int3
 		mmd::launch_resolve_pair_tables(pairs, type_ids, term_matrix, num_types, pair_tag, num_pairs, st(stream));
 	});
 }
-marsmd_status marsmd_launch_pairwise_nonbonded(const int32_t* pairs, marsmd_particle_view view,
-											   const uint32_t* pair_tag, const marsmd_table* tables, marsmd_box box,
-											   int32_t get_energy, uint64_t num_pairs, float cutoff_squared,
-											   uint32_t enabled_terms, marsmd_stream stream) {
+marsmd_status marsmd_launch_pairwise_nonbonded(const int32_t* pairs, const marsmd_particle_view* view,
+											   const uint32_t* pair_tag, const marsmd_table* tables,
+											   const marsmd_box* box, int32_t get_energy, uint64_t num_pairs,
+											   float cutoff_squared, uint32_t enabled_terms, marsmd_stream stream) {
 	return guard([&] {
-		mmd::launch_pairwise_nonbonded(pairs, view, pair_tag, tables, box, get_energy != 0, num_pairs, cutoff_squared,
-									   enabled_terms, st(stream));
+		need(view, "view");
+		need(box, "box");
+		mmd::launch_pairwise_nonbonded(pairs, *view, pair_tag, tables, *box, get_energy != 0, num_pairs,
+									   cutoff_squared, enabled_terms, st(stream));
 	});
 }
 marsmd_status marsmd_launch_tabulated_bonds(const int32_t* bonds, const float* pos4, float* fe,
 											const marsmd_table* tables, const int32_t* table_indices,
-											const int32_t* forms, marsmd_box box, int32_t get_energy,
+											const int32_t* forms, const marsmd_box* box, int32_t get_energy,
 											uint64_t num_bonds, marsmd_stream stream) {
 	return guard([&] {
-		mmd::launch_tabulated_bonds(bonds, pos4, fe, tables, table_indices, forms, box, get_energy != 0, num_bonds,
+		need(box, "box");
+		mmd::launch_tabulated_bonds(bonds, pos4, fe, tables, table_indices, forms, *box, get_energy != 0, num_bonds,
 									st(stream));
 	});
 }
 marsmd_status marsmd_launch_tabulated_angles(const int32_t* angles, const float* pos4, float* fe,
 											 const marsmd_table* tables, const int32_t* table_indices,
-											 const int32_t* forms, marsmd_box box, int32_t get_energy,
+											 const int32_t* forms, const marsmd_box* box, int32_t get_energy,
 											 uint64_t num_angles, marsmd_stream stream) {
 	return guard([&] {
-		mmd::launch_tabulated_angles(angles, pos4, fe, tables, table_indices, forms, box, get_energy != 0, num_angles,
-									 st(stream));
+		need(box, "box");
+		mmd::launch_tabulated_angles(angles, pos4, fe, tables, table_indices, forms, *box, get_energy != 0,
+									 num_angles, st(stream));
 	});
 }
-marsmd_status marsmd_launch_bd(marsmd_particle_view view, const marsmd_particle_types* types, float timestep,
-							   uint64_t current_step, float kT, uint64_t num_particles, marsmd_box box,
+marsmd_status marsmd_launch_bd(const marsmd_particle_view* view, const marsmd_particle_types* types, float timestep,
+							   uint64_t current_step, float kT, uint64_t num_particles, const marsmd_box* box,
 							   uint64_t base_seed, uint32_t base_ctr, const marsmd_grid_view* grids,
 							   const float electric_field[3], int32_t interpolation, marsmd_stream stream) {
 	return guard([&] {
+		need(view, "view");
 		need(types, "types");
+		need(box, "box");
 		const float zero[3] = {0, 0, 0};
-		mmd::launch_bd(view, *types, timestep, current_step, kT, num_particles, box, base_seed, base_ctr, grids,
+		mmd::launch_bd(*view, *types, timestep, current_step, kT, num_particles, *box, base_seed, base_ctr, grids,
 					   electric_field ? electric_field : zero, interpolation, st(stream));
 	});
 }
-marsmd_status marsmd_launch_baoab(marsmd_particle_view view, const marsmd_particle_types* types, marsmd_box box,
-								  float timestep, uint64_t current_step, float kT, uint64_t num_particles,
-								  uint64_t base_seed, uint32_t base_ctr, marsmd_stream stream) {
+marsmd_status marsmd_launch_baoab(const marsmd_particle_view* view, const marsmd_particle_types* types,
+								  const marsmd_box* box, float timestep, uint64_t current_step, float kT,
+								  uint64_t num_particles, uint64_t base_seed, uint32_t base_ctr, marsmd_stream stream) {
 	return guard([&] {
+		need(view, "view");
 		need(types, "types");
-		mmd::launch_baoab(view, *types, box, timestep, current_step, kT, num_particles, base_seed, base_ctr,
+		need(box, "box");
+		mmd::launch_baoab(*view, *types, *box, timestep, current_step, kT, num_particles, base_seed, base_ctr,
 						  st(stream));
 	});
 }
-marsmd_status marsmd_launch_baoab_last_update(marsmd_particle_view view, const marsmd_particle_types* types,
+marsmd_status marsmd_launch_baoab_last_update(const marsmd_particle_view* view, const marsmd_particle_types* types,
 											  float timestep, uint64_t num_particles, marsmd_stream stream) {
 	return guard([&] {
 		(void)types;
-		mmd::launch_baoab_last_update(view, timestep, num_particles, st(stream));
+		need(view, "view");
+		mmd::launch_baoab_last_update(*view, timestep, num_particles, st(stream));
 	});
 }
-marsmd_status marsmd_launch_pmf(marsmd_particle_view view, const marsmd_particle_types* types, uint64_t num_particles,
-								const marsmd_grid_view* grids, const float electric_field[3], int32_t interpolation,
-								marsmd_stream stream) {
+marsmd_status marsmd_launch_pmf(const marsmd_particle_view* view, const marsmd_particle_types* types,
+								uint64_t num_particles, const marsmd_grid_view* grids, const float electric_field[3],
+								int32_t interpolation, marsmd_stream stream) {
 	return guard([&] {
+		need(view, "view");
 		need(types, "types");
 		const float zero[3] = {0, 0, 0};
-		mmd::launch_pmf(view, *types, num_particles, grids, electric_field ? electric_field : zero, interpolation,
+		mmd::launch_pmf(*view, *types, num_particles, grids, electric_field ? electric_field : zero, interpolation,
 						st(stream));
 	});
 }

@@ -177,7 +177,7 @@ PROTOTYPES = {
     "marsmd_pairlist_create": (_I32, [_U64, _U64, _PP]),
     "marsmd_pairlist_destroy": (_I32, [_P]),
     "marsmd_pairlist_set_periodic_box": (_I32, [_P, C.POINTER(Box)]),
-    "marsmd_pairlist_set_exclusions": (_I32, [_P, ExclusionView]),
+    "marsmd_pairlist_set_exclusions": (_I32, [_P, C.POINTER(ExclusionView)]),
     "marsmd_pairlist_set_max_pairs_ceiling": (_I32, [_P, _U64]),
     "marsmd_pairlist_build": (_I32, [_P, _P, _P, _U64, _F, _P]),
     "marsmd_pairlist_needs_update": (_I32, [_P, _P, _U64, _F, C.POINTER(_I32), C.POINTER(_F), _P]),
@@ -187,14 +187,17 @@ PROTOTYPES = {
     "marsmd_pairlist_get_sorter": (_I32, [_P, _PP]),
     "marsmd_pairlist_get_reference_cells": (_I32, [_P, _PP, _PP, _PP, C.POINTER(_U64), C.POINTER(_I32), _P]),
     "marsmd_launch_resolve_pair_tables": (_I32, [_P, _P, _P, _U64, _P, _U64, _P]),
-    "marsmd_launch_pairwise_nonbonded": (_I32, [_P, ParticleView, _P, _P, Box, _I32, _U64, _F, _U32, _P]),
-    "marsmd_launch_tabulated_bonds": (_I32, [_P, _P, _P, _P, _P, _P, Box, _I32, _U64, _P]),
-    "marsmd_launch_tabulated_angles": (_I32, [_P, _P, _P, _P, _P, _P, Box, _I32, _U64, _P]),
-    "marsmd_launch_bd": (_I32, [ParticleView, C.POINTER(ParticleTypes), _F, _U64, _F, _U64, Box, _U64, _U32, _P,
+    "marsmd_launch_pairwise_nonbonded": (_I32, [_P, C.POINTER(ParticleView), _P, _P, C.POINTER(Box), _I32, _U64, _F,
+                                                _U32, _P]),
+    "marsmd_launch_tabulated_bonds": (_I32, [_P, _P, _P, _P, _P, _P, C.POINTER(Box), _I32, _U64, _P]),
+    "marsmd_launch_tabulated_angles": (_I32, [_P, _P, _P, _P, _P, _P, C.POINTER(Box), _I32, _U64, _P]),
+    "marsmd_launch_bd": (_I32, [C.POINTER(ParticleView), C.POINTER(ParticleTypes), _F, _U64, _F, _U64, C.POINTER(Box),
+                                _U64, _U32, _P,
                                 _FP3, _I32, _P]),
-    "marsmd_launch_baoab": (_I32, [ParticleView, C.POINTER(ParticleTypes), Box, _F, _U64, _F, _U64, _U64, _U32, _P]),
-    "marsmd_launch_baoab_last_update": (_I32, [ParticleView, C.POINTER(ParticleTypes), _F, _U64, _P]),
-    "marsmd_launch_pmf": (_I32, [ParticleView, C.POINTER(ParticleTypes), _U64, _P, _FP3, _I32, _P]),
+    "marsmd_launch_baoab": (_I32, [C.POINTER(ParticleView), C.POINTER(ParticleTypes), C.POINTER(Box), _F, _U64, _F, _U64,
+                                   _U64, _U32, _P]),
+    "marsmd_launch_baoab_last_update": (_I32, [C.POINTER(ParticleView), C.POINTER(ParticleTypes), _F, _U64, _P]),
+    "marsmd_launch_pmf": (_I32, [C.POINTER(ParticleView), C.POINTER(ParticleTypes), _U64, _P, _FP3, _I32, _P]),
     "marsmd_philox_draw": (_I32, [_U64, _U32, _U32, _U32, _U32, _U64, _P, _P]),
     "marsmd_patch_create": (_I32, [C.POINTER(PatchDesc), _PP]),
     "marsmd_patch_destroy": (_I32, [_P]),

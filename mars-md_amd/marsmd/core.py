@@ -317,7 +317,7 @@ class ZOrderPairlist:
 
     def set_exclusions(self, view: Optional[ExclusionCSR]):
         self._excl = view  # keep the device tensors alive
-        api.call("marsmd_pairlist_set_exclusions", self._h, (view or ExclusionCSR()).to_c())
+        api.call("marsmd_pairlist_set_exclusions", self._h, C.byref((view or ExclusionCSR()).to_c()))
 
     def set_max_pairs_ceiling(self, ceiling: int):
         api.call("marsmd_pairlist_set_max_pairs_ceiling", self._h, ceiling)
@@ -400,20 +400,21 @@ def launch_resolve_pair_tables(pairs, type_ids, term_matrix, num_types: int, pai
 def launch_pairwise_nonbonded(pairs, view: api.ParticleView, pair_tag, tables: DeviceTables, box: PeriodicBox,
                               get_energy: bool, num_pairs: int, cutoff_squared: float,
                               enabled_terms: int = api.PAIR_TERM_TABULATED, stream=None):
-    api.call("marsmd_launch_pairwise_nonbonded", _ptr(pairs), view, _ptr(pair_tag), tables.ptr, box.to_c(),
+    api.call("marsmd_launch_pairwise_nonbonded", _ptr(pairs), C.byref(view), _ptr(pair_tag), tables.ptr,
+             C.byref(box.to_c()),
              1 if get_energy else 0, num_pairs, float(cutoff_squared), enabled_terms, _stream(stream))
 
 
 def launch_tabulated_bonds(bonds, pos, force_energy, tables: DeviceTables, table_indices, forms, box: PeriodicBox,
                            get_energy: bool, num_bonds: int, stream=None):
     api.call("marsmd_launch_tabulated_bonds", _ptr(bonds), _ptr(pos), _ptr(force_energy), tables.ptr,
-             _ptr(table_indices), _ptr(forms), box.to_c(), 1 if get_energy else 0, num_bonds, _stream(stream))
+             _ptr(table_indices), _ptr(forms), C.byref(box.to_c()), 1 if get_energy else 0, num_bonds, _stream(stream))
 
 
 def launch_tabulated_angles(angles4, pos, force_energy, tables: DeviceTables, table_indices, forms,
                             box: PeriodicBox, get_energy: bool, num_angles: int, stream=None):
     api.call("marsmd_launch_tabulated_angles", _ptr(angles4), _ptr(pos), _ptr(force_energy), tables.ptr,
-             _ptr(table_indices), _ptr(forms), box.to_c(), 1 if get_energy else 0, num_angles, _stream(stream))
+             _ptr(table_indices), _ptr(forms), C.byref(box.to_c()), 1 if get_energy else 0, num_angles, _stream(stream))
 
 
 def _ef(e):
@@ -423,25 +424,26 @@ def _ef(e):
 def launch_BD(view: api.ParticleView, types: api.ParticleTypes, timestep: float, current_step: int, kT: float,
               num_particles: int, box: PeriodicBox, base_seed: int, base_ctr: int = 0, grids=None,
               electric_field=None, interpolation: int = api.INTERP_LINEAR, stream=None):
-    api.call("marsmd_launch_bd", view, C.byref(types), float(timestep), current_step, float(kT), num_particles,
-             box.to_c(), base_seed, base_ctr, _ptr(grids), _ef(electric_field), interpolation, _stream(stream))
+    api.call("marsmd_launch_bd", C.byref(view), C.byref(types), float(timestep), current_step, float(kT), num_particles,
+             C.byref(box.to_c()), base_seed, base_ctr, _ptr(grids), _ef(electric_field), interpolation, _stream(stream))
 
 
 def launch_BAOAB(view: api.ParticleView, types: api.ParticleTypes, box: PeriodicBox, timestep: float,
                  current_step: int, kT: float, num_particles: int, base_seed: int, base_ctr: int = 0, stream=None):
-    api.call("marsmd_launch_baoab", view, C.byref(types), box.to_c(), float(timestep), current_step, float(kT),
+    api.call("marsmd_launch_baoab", C.byref(view), C.byref(types), C.byref(box.to_c()), float(timestep), current_step,
+             float(kT),
              num_particles, base_seed, base_ctr, _stream(stream))
 
 
 def launch_BAOAB_LastUpdate(view: api.ParticleView, types: api.ParticleTypes, timestep: float, num_particles: int,
                             stream=None):
-    api.call("marsmd_launch_baoab_last_update", view, C.byref(types), float(timestep), num_particles,
+    api.call("marsmd_launch_baoab_last_update", C.byref(view), C.byref(types), float(timestep), num_particles,
              _stream(stream))
 
 
 def launch_PMF(view: api.ParticleView, types: api.ParticleTypes, num_particles: int, grids, electric_field=None,
                interpolation: int = api.INTERP_LINEAR, stream=None):
-    api.call("marsmd_launch_pmf", view, C.byref(types), num_particles, _ptr(grids), _ef(electric_field),
+    api.call("marsmd_launch_pmf", C.byref(view), C.byref(types), num_particles, _ptr(grids), _ef(electric_field),
              interpolation, _stream(stream))
 
 

@@ -814,12 +814,13 @@ void orc_cell_tables(const uint32_t* sorted_codes, uint64_t n, int32_t m, const 
 // Builds the pair set. Returns the number of pairs (or -1 on error); writes at most
 // `cap` (x, y) pairs, sorted, canonical indices. sorted_idx/inverse/codes/info may be NULL.
 int64_t orc_build_pairlist(const float* pos4, uint64_t n, const marsmd_box* box,
-						   float pairlist_cutoff, marsmd_exclusion_view ex, uint64_t max_pairs,
+						   float pairlist_cutoff, const marsmd_exclusion_view* exp, uint64_t max_pairs,
 						   int32_t brute_force, int32_t* pairs_out, uint64_t cap,
 						   uint32_t* codes, uint32_t* sorted_idx, uint32_t* inverse,
 						   int32_t* grid_info /* m, rx, ry, rz, slots */) {
 	try {
 		PairlistResult r;
+		const marsmd_exclusion_view ex = exp ? *exp : marsmd_exclusion_view{};
 		build_pairlist(as_v4(pos4), n, make_box(*box), pairlist_cutoff, ex, max_pairs, r,
 					   brute_force != 0);
 		const uint64_t np = r.pairs.size();

@@ -42,7 +42,7 @@ def lib():
             "orc_pairlist_domain": (I32, [P, U64, C.POINTER(api.Box), P, P]),
             "orc_select_cell_grid": (None, [F, U64, P, U64, P, P, P]),
             "orc_cell_tables": (None, [P, U64, I32, P, I32, P, P, P, P]),
-            "orc_build_pairlist": (C.c_int64, [P, U64, C.POINTER(api.Box), F, api.ExclusionView, U64, I32, P, U64,
+            "orc_build_pairlist": (C.c_int64, [P, U64, C.POINTER(api.Box), F, C.POINTER(api.ExclusionView), U64, I32, P, U64,
                                                P, P, P, P]),
             "orc_exclusion_csr": (C.c_int64, [P, U64, P, P]),
             "orc_pair_forces": (I32, [P, U64, P, P, P, I32, P, I32, C.POINTER(api.Box), F, U32, I32, U64, P]),
@@ -191,12 +191,12 @@ def build_pairlist(pos4, box, cutoff, offsets=None, excluded=None, num_excl=0, r
     gi = np.zeros(5, np.int32)
     cap = 0
     mp = max_pairs if max_pairs is not None else max(n, 1)
-    cnt = lib().orc_build_pairlist(pos4.ctypes.data, n, C.byref(b), cutoff, ex, mp, 1 if brute_force else 0, None, 0,
+    cnt = lib().orc_build_pairlist(pos4.ctypes.data, n, C.byref(b), cutoff, C.byref(ex), mp, 1 if brute_force else 0, None, 0,
                                    codes.ctypes.data, sidx.ctypes.data, inv.ctypes.data, gi.ctypes.data)
     if cnt < 0:
         raise RuntimeError(lib().orc_last_error().decode())
     pairs = np.zeros((max(cnt, 1), 2), np.int32)
-    cnt2 = lib().orc_build_pairlist(pos4.ctypes.data, n, C.byref(b), cutoff, ex, mp, 1 if brute_force else 0,
+    cnt2 = lib().orc_build_pairlist(pos4.ctypes.data, n, C.byref(b), cutoff, C.byref(ex), mp, 1 if brute_force else 0,
                                     pairs.ctypes.data, cnt, None, None, None, None)
     assert cnt2 == cnt
     return {"pairs": pairs[:cnt], "codes": codes, "sorted_idx": sidx, "inverse": inv,
