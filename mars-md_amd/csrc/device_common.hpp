@@ -130,6 +130,39 @@ __device__ __forceinline__ FE table_compute(float dx, const float* __restrict__ 
 	return FE{-dU * step_inv, dU * w + U0};
 }
 
+// FP32 arccosine of the angle term, from the arcsine Maclaurin series
+// asin(t) = t + t z sum_k c_k z^k (z = t^2, c_k = (2k+2)! / (4^(k+1) ((k+1)!)^2 (2k+3))):
+// acos(x) = pi/2 - asin(x) for |x| <= 1/2, acos(|x|) = 2 asin(sqrt((1-|x|)/2)) and
+// acos(-|x|) = pi - acos(|x|) otherwise. The same expression is the CPU checker's, so
+// the angle (and its table bin) is the same bits on both sides.
+__host__ __device__ __forceinline__ float acos_series(float x) {
+	const float ax = fabsf(x);
+	float t, z;
+	if (ax <= 0.5f) {
+		t = x;
+		z = x * x;
+	} else {
+		z = (1.0f - ax) * 0.5f;
+		t = sqrtf(z);
+	}
+	float p = 7.312525874e-03f;
+	p = p * z + 8.390335810e-03f;
+	p = p * z + 9.761609529e-03f;
+	p = p * z + 1.155180090e-02f;
+	p = p * z + 1.396484375e-02f;
+	p = p * z + 1.735276442e-02f;
+	p = p * z + 2.237215909e-02f;
+	p = p * z + 3.038194444e-02f;
+	p = p * z + 4.464285714e-02f;
+	p = p * z + 7.500000000e-02f;
+	p = p * z + 1.666666667e-01f;
+	const float as = t + t * z * p;
+	if (ax <= 0.5f)
+		return 1.57079637f - as;
+	const float r = 2.0f * as;
+	return x < 0.0f ? 3.14159274f - r : r;
+}
+
 // ------------------------------------------------------------------ Philox
 // openrand::Philox (src/PatchOperation/Random/philox.h:60-139) and u01
 // (base_state.h:299-305).

@@ -110,7 +110,7 @@ __device__ __forceinline__ void angle_forces(const DevBox& box, const float4& pa
 		c = -1.0f;
 	if (c > 1.0f)
 		c = 1.0f;
-	const float angle = acosf(c);
+	const float angle = acos_series(c);
 	const float sn = sqrtf(1.0f - c * c);
 	const FE v = table_compute(angle, t.pot, int(t.size), t.start, t.step_inv, t.is_periodic != 0);
 	const float s = sn > 1e-3f ? sn : 1e-3f;

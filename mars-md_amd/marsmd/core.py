@@ -380,8 +380,15 @@ def create_pairlist(max_particles: int, max_pairs: int = 1 << 62) -> ZOrderPairl
 
 
 # ---------------------------------------------------------------------------- launchers
+class ParticleViewRef(api.ParticleView):
+    """ParticleView (DeviceParticle.h:18-28) that also holds the tensors it points
+    into, so a view built from temporaries keeps them alive until it is dropped."""
+
+    _keep: tuple = ()
+
+
 def particle_view(pos=None, force_energy=None, type_id=None, mom=None, flags=None, ids=None) -> api.ParticleView:
-    v = api.ParticleView()
+    v = ParticleViewRef()
     v.id = _ptr(ids)
     v.type_id = _ptr(type_id)
     v.pos = _ptr(pos)
@@ -389,6 +396,7 @@ def particle_view(pos=None, force_energy=None, type_id=None, mom=None, flags=Non
     v.force_energy = _ptr(force_energy)
     v.flags = _ptr(flags)
     v.attached_rigid_body_id = None
+    v._keep = (pos, force_energy, type_id, mom, flags, ids)
     return v
 
 

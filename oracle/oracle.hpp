@@ -274,8 +274,43 @@ inline PairTerm<T> bond_term(const Box& box, const V4<T>& px, const V4<T>& py,
 	return out;
 }
 
+// FP32 arccosine shared with the device kernels (device_common.hpp acos_series), so
+// the angle term is the same bits on both sides: the arcsine Maclaurin series
+// asin(t) = t + t z sum_k c_k z^k, z = t^2, c_k = (2k+2)! / (4^(k+1) ((k+1)!)^2 (2k+3)),
+// with acos(x) = pi/2 - asin(x) for |x| <= 1/2 and acos(|x|) = 2 asin(sqrt((1-|x|)/2)),
+// acos(-|x|) = pi - acos(|x|) otherwise (z <= 1/4: the series converges to < 1e-9).
+inline float acos_series(float x) {
+	const float ax = std::fabs(x);
+	float t, z;
+	if (ax <= 0.5f) {
+		t = x;
+		z = x * x;
+	} else {
+		z = (1.0f - ax) * 0.5f;
+		t = std::sqrt(z);
+	}
+	float p = 7.312525874e-03f;
+	p = p * z + 8.390335810e-03f;
+	p = p * z + 9.761609529e-03f;
+	p = p * z + 1.155180090e-02f;
+	p = p * z + 1.396484375e-02f;
+	p = p * z + 1.735276442e-02f;
+	p = p * z + 2.237215909e-02f;
+	p = p * z + 3.038194444e-02f;
+	p = p * z + 4.464285714e-02f;
+	p = p * z + 7.500000000e-02f;
+	p = p * z + 1.666666667e-01f;
+	const float as = t + t * z * p;
+	if (ax <= 0.5f)
+		return 1.57079637f - as;
+	const float r = 2.0f * as;
+	return x < 0.0f ? 3.14159274f - r : r;
+}
+inline float acos_t(float x) { return acos_series(x); }
+inline double acos_t(double x) { return std::acos(x); }
+
 // AngleGeometry (src/Interactions/Bonded/BondGeometry.h:13-45) and
-// TabulatedAngleComputer (BondComputer.h:231-268).
+// TabulatedAngleComputer (BondComputer.h:231-268). The arccosine is acos_t.
 template<class T>
 struct AngleTerm {
 	V4<T> f1, f2, f3;
@@ -293,7 +328,7 @@ inline AngleTerm<T> angle_term(const Box& box, const V4<T>& pa, const V4<T>& pb,
 		cosv = T(-1);
 	if (cosv > T(1))
 		cosv = T(1);
-	const T angle = std::acos(cosv);
+	const T angle = acos_t(cosv);
 	const T sinv = std::sqrt(T(1) - cosv * cosv);
 	const FE<T> fe = table_compute<T>(angle, tab);
 	const T sin_floor = T(1e-3);

@@ -1228,3 +1228,17 @@ int orc_sim_get_topology(void* h, int32_t* bonds, int32_t* angles, int32_t* excl
 }
 
 } // extern "C"
+
+extern "C" {
+// The shared FP32 arccosine of the angle term.
+float orc_acos(float x) { return acos_series(x); }
+// Stash the field and interpolation order the integrator applies (the state
+// calculate_nonbonded_forces records, Patch.cpp:71-73).
+int orc_sim_set_field(void* h, const float ef[3], int32_t interp) {
+	Sim& s = *static_cast<Sim*>(h);
+	for (int a = 0; a < 3; ++a)
+		s.efield[a] = ef[a];
+	s.interp = interp;
+	return 0;
+}
+} // extern "C"

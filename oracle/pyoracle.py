@@ -56,6 +56,8 @@ def lib():
             "orc_sim_set_particles": (I32, [P, P, P, P, U64]),
             "orc_sim_get_particles": (I32, [P, P, P, P, P, P, P, U64]),
             "orc_sim_set_state": (I32, [P, P, P, P, P, P, U64, I32]),
+            "orc_sim_set_field": (I32, [P, P, I32]),
+            "orc_acos": (F, [F]),
             "orc_sim_calculate_nonbonded_forces": (I32, [P, F, F, U64, U64, P, I32, I32]),
             "orc_sim_calculate_bonded_forces": (I32, [P, I32]),
             "orc_sim_integrate_motion": (I32, [P, F, F, I32, U64]),
@@ -101,6 +103,11 @@ def box_muller(u) -> np.ndarray:
 
 def mix_seed(seed, patch_id, step) -> int:
     return lib().orc_mix_seed(seed, patch_id, step)
+
+
+def acos32(x: float) -> float:
+    """The FP32 arccosine shared by the angle kernels and the oracle."""
+    return lib().orc_acos(x)
 
 
 # --------------------------------------------------------------------------- Morton / sort
@@ -319,6 +326,10 @@ class Sim:
                               (ids, np.int32))]
         dk = -1 if deferred_kick is None else int(bool(deferred_kick))
         _check(lib().orc_sim_set_state(self._h, *[_p(a) for a in arrs], self.n, dk))
+
+    def set_field(self, electric_field=(0.0, 0.0, 0.0), interpolation=api.INTERP_LINEAR):
+        ef = np.ascontiguousarray(electric_field, np.float32)
+        _check(lib().orc_sim_set_field(self._h, ef.ctypes.data, interpolation))
 
     def calculate_nonbonded_forces(self, pairlist_cutoff, interaction_cutoff, step, rebuild_period,
                                    electric_field=(0.0, 0.0, 0.0), interpolation=api.INTERP_LINEAR,

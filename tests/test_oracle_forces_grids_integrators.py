@@ -115,6 +115,16 @@ def test_bond_chain_fixture():
     assert fref == pytest.approx(19.19 * (3.8 - 1.885), rel=0.02) or abs(fref) > 10
 
 
+def test_shared_acos_accuracy():
+    # the series arccosine used by kernels and oracle: within 2 ulp of acos over [-1, 1]
+    xs = np.concatenate([np.linspace(-1, 1, 20001), [-1.0, -0.5, 0.0, 0.5, 1.0, 0.49999997, 0.50000006]])
+    for x in np.float32(xs):
+        got = np.float32(orc.acos32(float(x)))
+        ref = np.arccos(np.float64(x))
+        ulp = np.spacing(np.float32(max(ref, 1e-30)))
+        assert abs(float(got) - ref) <= 2.5 * ulp, (x, got, ref)
+
+
 def test_angle_forces_finite_difference_and_invariants():
     # ramp table U(theta) = 2 theta (degrees abscissa, periodic), TabulatedBonded.cpp:189-260
     knots = 361
