@@ -277,6 +277,22 @@ marsmd_status marsmd_launch_tabulated_angles(const int32_t* angles, const float*
                                              const int32_t* table_indices, const int32_t* forms,
                                              const marsmd_box* box, int32_t get_energy,
                                              uint64_t num_angles, marsmd_stream stream);
+/* launch_tabulated_dihedrals (DeviceBondedInteraction.h:548-573; TabulatedDihedralComputer
+ * BondComputer.h:281-346 with DihedralGeometry BondGeometry.h:56-87). dihedrals: 16-byte
+ * int records {i, j, k, l}; tables periodic, abscissa in radians. */
+marsmd_status marsmd_launch_tabulated_dihedrals(const int32_t* dihedrals, const float* pos4,
+                                                float* force_energy, const marsmd_table* tables,
+                                                const int32_t* table_indices, const int32_t* forms,
+                                                const marsmd_box* box, int32_t get_energy,
+                                                uint64_t num_dihedrals, marsmd_stream stream);
+/* launch_harmonic_restraints (DeviceBondedInteraction.h:595-620; HarmonicRestraintComputer
+ * BondComputer.h:84-131): particle_ids (< 0 skipped), anchors as float4 records,
+ * one spring constant per restraint. */
+marsmd_status marsmd_launch_harmonic_restraints(const int32_t* particle_ids, const float* pos4,
+                                                float* force_energy, const float* anchors4,
+                                                const float* spring_constants, const marsmd_box* box,
+                                                int32_t get_energy, uint64_t num_restraints,
+                                                marsmd_stream stream);
 /* launch_BD (src/PatchOperation/Integrator.h:25-54; BDIntegrate BD.h:44-92). grids:
  * device array of marsmd_grid_view (NULL = none). */
 marsmd_status marsmd_launch_bd(const marsmd_particle_view* view, const marsmd_particle_types* types,
@@ -339,6 +355,16 @@ typedef struct marsmd_patch_desc {
     /* grids (host data, copied to HBM) */
     int32_t num_grids;
     const marsmd_grid_view* grids; /* host records; .data host pointers */
+    /* dihedrals and harmonic restraints (BondedInteractions), canonical indices */
+    uint64_t num_dihedrals;
+    const int32_t* dihedrals;      /* 4 per dihedral */
+    const int32_t* dihedral_table;
+    int32_t num_dihedral_tables;
+    const marsmd_table* dihedral_tables; /* host, is_periodic set, radians */
+    uint64_t num_restraints;
+    const int32_t* restraint_ids;     /* particle per restraint */
+    const float* restraint_anchors;   /* 4 floats per restraint (x, y, z, unused) */
+    const float* restraint_k;         /* spring constant per restraint */
 } marsmd_patch_desc;
 
 marsmd_status marsmd_patch_create(const marsmd_patch_desc* desc, marsmd_patch** out);
@@ -397,6 +423,14 @@ typedef struct marsmd_run_params {
 
 marsmd_status marsmd_patch_run(marsmd_patch* p, const marsmd_run_params* params,
                                uint64_t first_step, uint64_t num_steps, marsmd_stream stream);
+
+/* Launch timing of the fused step kernel inside marsmd_patch_run: while enabled, each
+ * launch is bracketed by HIP events on the run's stream. Enabling (or disabling)
+ * clears the record; the getter waits for the recorded launches and returns their
+ * summed duration in milliseconds and their number. A measurement hook with no
+ * counterpart in the reference. */
+marsmd_status marsmd_patch_set_kernel_timing(marsmd_patch* p, int32_t enable);
+marsmd_status marsmd_patch_get_kernel_timing(marsmd_patch* p, double* total_ms, uint64_t* launches);
 
 #ifdef __cplusplus
 } /* extern "C" */

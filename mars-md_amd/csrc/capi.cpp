@@ -263,6 +263,33 @@ marsmd_status marsmd_launch_tabulated_angles(const int32_t* angles, const float*
 									 num_angles, st(stream));
 	});
 }
+
+marsmd_status marsmd_launch_tabulated_dihedrals(const int32_t* dihedrals, const float* pos4, float* fe,
+												const marsmd_table* tables, const int32_t* table_indices,
+												const int32_t* forms, const marsmd_box* box, int32_t get_energy,
+												uint64_t num_dihedrals, marsmd_stream stream) {
+	return guard([&] {
+		need(box, "box");
+		mmd::launch_tabulated_dihedrals(dihedrals, pos4, fe, tables, table_indices, forms, *box, get_energy != 0,
+										num_dihedrals, st(stream));
+	});
+}
+
+marsmd_status marsmd_launch_harmonic_restraints(const int32_t* particle_ids, const float* pos4, float* fe,
+												const float* anchors4, const float* spring_constants,
+												const marsmd_box* box, int32_t get_energy, uint64_t num_restraints,
+												marsmd_stream stream) {
+	return guard([&] {
+		need(box, "box");
+		if (num_restraints) {
+			need(particle_ids, "particle_ids");
+			need(anchors4, "anchors");
+			need(spring_constants, "spring_constants");
+		}
+		mmd::launch_harmonic_restraints(particle_ids, pos4, fe, anchors4, spring_constants, *box, get_energy != 0,
+										num_restraints, st(stream));
+	});
+}
 marsmd_status marsmd_launch_bd(const marsmd_particle_view* view, const marsmd_particle_types* types, float timestep,
 							   uint64_t current_step, float kT, uint64_t num_particles, const marsmd_box* box,
 							   uint64_t base_seed, uint32_t base_ctr, const marsmd_grid_view* grids,
@@ -399,6 +426,22 @@ marsmd_status marsmd_patch_run(marsmd_patch* p, const marsmd_run_params* params,
 		need(p, "patch");
 		need(params, "params");
 		p->impl.run(*params, first_step, num_steps, st(stream));
+	});
+}
+
+marsmd_status marsmd_patch_set_kernel_timing(marsmd_patch* p, int32_t enable) {
+	return guard([&] {
+		need(p, "patch");
+		p->impl.set_kernel_timing(enable != 0);
+	});
+}
+
+marsmd_status marsmd_patch_get_kernel_timing(marsmd_patch* p, double* total_ms, uint64_t* launches) {
+	return guard([&] {
+		need(p, "patch");
+		need(total_ms, "total_ms");
+		need(launches, "launches");
+		p->impl.kernel_timing(total_ms, launches);
 	});
 }
 

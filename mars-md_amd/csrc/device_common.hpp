@@ -75,6 +75,22 @@ __device__ __forceinline__ float3 sub3(const float4& a, const float4& b) {
 }
 __device__ __forceinline__ float len2(const float3& v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
 
+// Persistent schedule over tiles (of slots, or of home cells). Workgroups are
+// dispatched round-robin over the 8 XCDs, so workgroup b runs on XCD b % 8; the tiles
+// are cut into 8 contiguous (Morton-ordered, hence spatially compact) chunks and chunk
+// x goes to the workgroups of XCD x, keeping each XCD's L2 on one region of the
+// system. gridDim.x is a multiple of 8.
+struct TileRange {
+	uint32_t tile, end, stride;
+};
+__device__ __forceinline__ TileRange xcd_tiles(uint32_t tiles) {
+	const uint32_t x = blockIdx.x & 7u;
+	const uint32_t r = blockIdx.x >> 3;
+	const uint32_t cs = uint32_t(uint64_t(tiles) * x / 8u);
+	const uint32_t ce = uint32_t(uint64_t(tiles) * (x + 1u) / 8u);
+	return TileRange{cs + r, ce, gridDim.x >> 3};
+}
+
 // ------------------------------------------------------------------ Morton
 // MortonCode (src/PatchOperation/ZOrderKernels/MortonCode.h:95-180).
 __host__ __device__ __forceinline__ uint32_t split_by3(uint32_t a) {
@@ -163,6 +179,40 @@ __host__ __device__ __forceinline__ float acos_series(float x) {
 	return x < 0.0f ? 3.14159274f - r : r;
 }
 
+// FP32 two-argument arctangent, shared with the CPU checker for the same reason as
+// acos_series (the dihedral angle and its table bin are the same bits on both sides).
+// With t = min(|x|,|y|) / max(|x|,|y|) in [0, 1]: for t > tan(pi/12) use
+// atan(t) = pi/6 + atan((sqrt(3) t - 1) / (t + sqrt(3))), which maps t into
+// |t'| <= tan(pi/12); then the Maclaurin series atan(t') = t' - t'^3/3 + ... + t'^13/13
+// (next term below 3e-10). Octant and quadrant follow from |y| > |x| and the signs
+// (signed zeros as IEEE atan2: atan2(+-0, -0) = +-pi). NaN propagates.
+__host__ __device__ __forceinline__ float atan2_series(float y, float x) {
+	const float ax = fabsf(x), ay = fabsf(y);
+	const float mx = ax > ay ? ax : ay;
+	const float mn = ax > ay ? ay : ax;
+	float t = mx == 0.0f ? 0.0f : mn / mx;
+	float base = 0.0f;
+	if (t > 0.267949194f) {
+		t = (t * 1.73205081f - 1.0f) / (t + 1.73205081f);
+		base = 0.523598776f;
+	}
+	const float z = t * t;
+	float p = 7.69230769e-02f;
+	p = p * z - 9.09090909e-02f;
+	p = p * z + 1.11111111e-01f;
+	p = p * z - 1.42857143e-01f;
+	p = p * z + 2.00000000e-01f;
+	p = p * z - 3.33333333e-01f;
+	float r = base + (t + t * z * p);
+	if (ay > ax)
+		r = 1.57079637f - r;
+	const bool xneg = x < 0.0f || (x == 0.0f && 1.0f / x < 0.0f);
+	if (xneg)
+		r = 3.14159274f - r;
+	const bool yneg = y < 0.0f || (y == 0.0f && 1.0f / y < 0.0f);
+	return yneg ? -r : r;
+}
+
 // ------------------------------------------------------------------ Philox
 // openrand::Philox (src/PatchOperation/Random/philox.h:60-139) and u01
 // (base_state.h:299-305).
@@ -218,13 +268,6 @@ __device__ __forceinline__ bool map_grid_index(int& j, int n, int bc) {
 	}
 	return false;
 }
-__device__ __forceinline__ float grid_fetch(const marsmd_grid_view& g, int jx, int jy, int jz, int bc) {
-	const int nx = int(g.dims[0]), ny = int(g.dims[1]), nz = int(g.dims[2]);
-	if (!map_grid_index(jx, nx, bc) || !map_grid_index(jy, ny, bc) || !map_grid_index(jz, nz, bc))
-		return 0.0f;
-	const size_t idx = size_t(jz) + size_t(jy) * size_t(nz) + size_t(jx) * size_t(ny) * size_t(nz);
-	return g.data[idx];
-}
 __device__ __forceinline__ void mat_transform(const float* m, const float v[3], float out[3]) {
 #pragma unroll
 	for (int k = 0; k < 3; ++k)
@@ -242,16 +285,73 @@ struct GridSample {
 	float grad[3];
 };
 
+// Offset of a tap plane/row along one axis (mapped index x stride), or -1 when the
+// tap lies outside a Dirichlet grid (its value is then 0).
+__device__ __forceinline__ int64_t tap_offset(int j, int n, int bc, int64_t stride) {
+	return map_grid_index(j, n, bc) ? int64_t(j) * stride : int64_t(-1);
+}
+
+// 4-byte aligned runs of grid values along z: one 8- or 16-byte load each.
+struct __attribute__((aligned(4))) GridRun2 {
+	float v[2];
+};
+struct __attribute__((aligned(4))) GridRun4 {
+	float v[4];
+};
+
+// K consecutive z taps starting at jz0 of the row at `row`: a single load when the run
+// lies inside the grid, else tap by tap through the boundary mapping.
+template<int K>
+__device__ __forceinline__ void load_run(const float* __restrict__ row, int jz0, int nz, int bc, float out[K]) {
+	if (jz0 >= 0 && jz0 + K <= nz) {
+		if constexpr (K == 4) {
+			const GridRun4 r = *reinterpret_cast<const GridRun4*>(row + jz0);
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				out[q] = r.v[q];
+		} else {
+			const GridRun2 r = *reinterpret_cast<const GridRun2*>(row + jz0);
+#pragma unroll
+			for (int q = 0; q < 2; ++q)
+				out[q] = r.v[q];
+		}
+	} else {
+#pragma unroll
+		for (int q = 0; q < K; ++q) {
+			int jz = jz0 + q;
+			out[q] = map_grid_index(jz, nz, bc) ? row[jz] : 0.0f;
+		}
+	}
+}
+
+// sample_grid_linear (BaseGridDevice.h:256-365).
 __device__ inline GridSample sample_linear(const marsmd_grid_view& g, float px, float py, float pz, int bc) {
 	const float v[3] = {px - g.origin[0], py - g.origin[1], pz - g.origin[2]};
 	float gp[3];
 	mat_transform(g.basis_inv, v, gp);
 	const int i0 = int(floorf(gp[0])), j0 = int(floorf(gp[1])), k0 = int(floorf(gp[2]));
 	const float fx = gp[0] - float(i0), fy = gp[1] - float(j0), fz = gp[2] - float(k0);
-	const float v000 = grid_fetch(g, i0, j0, k0, bc), v001 = grid_fetch(g, i0, j0, k0 + 1, bc);
-	const float v010 = grid_fetch(g, i0, j0 + 1, k0, bc), v011 = grid_fetch(g, i0, j0 + 1, k0 + 1, bc);
-	const float v100 = grid_fetch(g, i0 + 1, j0, k0, bc), v101 = grid_fetch(g, i0 + 1, j0, k0 + 1, bc);
-	const float v110 = grid_fetch(g, i0 + 1, j0 + 1, k0, bc), v111 = grid_fetch(g, i0 + 1, j0 + 1, k0 + 1, bc);
+	const int nx = int(g.dims[0]), ny = int(g.dims[1]), nz = int(g.dims[2]);
+	const int64_t sx = int64_t(ny) * int64_t(nz);
+	float r[2][2][2]; // [x][y][z]
+#pragma unroll
+	for (int ix = 0; ix < 2; ++ix) {
+		const int64_t ox = tap_offset(i0 + ix, nx, bc, sx);
+#pragma unroll
+		for (int iy = 0; iy < 2; ++iy) {
+			const int64_t oy = tap_offset(j0 + iy, ny, bc, int64_t(nz));
+			if (ox < 0 || oy < 0) {
+				r[ix][iy][0] = 0.0f;
+				r[ix][iy][1] = 0.0f;
+			} else {
+				load_run<2>(g.data + ox + oy, k0, nz, bc, r[ix][iy]);
+			}
+		}
+	}
+	const float v000 = r[0][0][0], v001 = r[0][0][1];
+	const float v010 = r[0][1][0], v011 = r[0][1][1];
+	const float v100 = r[1][0][0], v101 = r[1][0][1];
+	const float v110 = r[1][1][0], v111 = r[1][1][1];
 	const float v00 = v000 * (1.0f - fx) + v100 * fx;
 	const float v01 = v001 * (1.0f - fx) + v101 * fx;
 	const float v10 = v010 * (1.0f - fx) + v110 * fx;
@@ -284,27 +384,41 @@ __device__ __forceinline__ float cr_deriv(float v0, float v1, float v2, float v3
 	return (3.0f * a3 * t + 2.0f * a2) * t + a1;
 }
 
-// sample_grid_cubic (BaseGridDevice.h:405-475). For a periodic grid whose 4-tap
-// z-run does not wrap, the run is read as one 16-byte row.
+// sample_grid_cubic (BaseGridDevice.h:405-475): the x pass runs over the 4 x 4 (y, z)
+// tap lines, then y, then z, as the reference does. Each tap axis is mapped once and
+// the four z taps of a (x, y) line are one 16-byte load when they do not cross a
+// grid face.
 __device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, float py, float pz, int bc) {
 	const float v[3] = {px - g.origin[0], py - g.origin[1], pz - g.origin[2]};
 	float gp[3];
 	mat_transform(g.basis_inv, v, gp);
 	const int hx = int(floorf(gp[0])), hy = int(floorf(gp[1])), hz = int(floorf(gp[2]));
 	const float wx = gp[0] - float(hx), wy = gp[1] - float(hy), wz = gp[2] - float(hz);
+	const int nx = int(g.dims[0]), ny = int(g.dims[1]), nz = int(g.dims[2]);
+	const int64_t sx = int64_t(ny) * int64_t(nz);
+	int64_t ox[4];
+#pragma unroll
+	for (int i = 0; i < 4; ++i)
+		ox[i] = tap_offset(hx - 1 + i, nx, bc, sx);
 	float d1[4][4], v1[4][4];
 #pragma unroll
-	for (int iz = 0; iz < 4; ++iz) {
-		const int jz = hz - 1 + iz;
+	for (int iy = 0; iy < 4; ++iy) {
+		const int64_t oy = tap_offset(hy - 1 + iy, ny, bc, int64_t(nz));
+		float r[4][4]; // [x][z]
 #pragma unroll
-		for (int iy = 0; iy < 4; ++iy) {
-			const int jy = hy - 1 + iy;
-			float t[4];
+		for (int ix = 0; ix < 4; ++ix) {
+			if (ox[ix] < 0 || oy < 0) {
 #pragma unroll
-			for (int ix = 0; ix < 4; ++ix)
-				t[ix] = grid_fetch(g, hx - 1 + ix, jy, jz, bc);
-			d1[iy][iz] = cr_deriv(t[0], t[1], t[2], t[3], wx);
-			v1[iy][iz] = cr_value(t[0], t[1], t[2], t[3], wx);
+				for (int q = 0; q < 4; ++q)
+					r[ix][q] = 0.0f;
+			} else {
+				load_run<4>(g.data + ox[ix] + oy, hz - 1, nz, bc, r[ix]);
+			}
+		}
+#pragma unroll
+		for (int iz = 0; iz < 4; ++iz) {
+			d1[iy][iz] = cr_deriv(r[0][iz], r[1][iz], r[2][iz], r[3][iz], wx);
+			v1[iy][iz] = cr_value(r[0][iz], r[1][iz], r[2][iz], r[3][iz], wx);
 		}
 	}
 	float d2x[4], d2y[4], v2[4];
@@ -323,53 +437,81 @@ __device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, f
 	return s;
 }
 
-// compute_position_dependent_force (src/Interactions/Nonbonded/Pmf.h:68-124). The
-// returned .w is the PMF energy (the reference's .t).
-__device__ inline float4 position_dependent_force(float px, float py, float pz, int type,
-												  const marsmd_particle_types& types,
-												  const marsmd_grid_view* __restrict__ grids,
-												  const float ef[3], int scheme) {
+// Runs body(grid view) for the active lanes whose grid index is gid, one distinct
+// index at a time: the index is wave-uniform inside, so the view is read with scalar
+// loads instead of occupying vector registers in every lane. Lanes with gid < 0 skip.
+template<class Body>
+__device__ __forceinline__ void for_each_grid(int gid, const marsmd_grid_view* __restrict__ grids, Body&& body) {
+	uint64_t pending = __ballot(gid >= 0);
+	while (pending) {
+		const int lane = __ffsll((unsigned long long)pending) - 1;
+		const int ug = __builtin_amdgcn_readlane(gid, lane);
+		if (gid == ug) {
+			const marsmd_grid_view g = grids[ug]; // uniform copy: scalar registers
+			body(g);
+		}
+		pending &= ~__ballot(gid == ug);
+	}
+}
+
+// compute_position_dependent_force (src/Interactions/Nonbonded/Pmf.h:68-124) for one
+// interpolation scheme. The returned .w is the PMF energy (the reference's .t). Each
+// lane adds its type's terms in term order, as the reference does.
+template<bool CUBIC>
+__device__ inline float4 position_dependent_force_t(float px, float py, float pz, int type,
+													const marsmd_particle_types& types,
+													const marsmd_grid_view* __restrict__ grids, const float ef[3]) {
 	const float charge = types.charge[type];
 	float4 f = make_float4(charge * ef[0], charge * ef[1], charge * ef[2], 0.0f);
 	if (grids == nullptr)
 		return f;
+	int max_terms = 0;
+	for (int y = 0; y < types.num_types; ++y)
+		max_terms = max(max_terms, types.pmf_grid_count[y]);
 	const int b = types.pmf_grid_offset[type];
-	const int e = b + types.pmf_grid_count[type];
-	for (int k = b; k < e; ++k) {
-		const marsmd_grid_term term = types.pmf_grid_terms[k];
-		if (term.grid_id < 0)
-			continue;
-		const marsmd_grid_view& g = grids[term.grid_id];
-		if (g.data == nullptr)
-			continue;
-		const int bc = term.boundary_condition >= 0 ? term.boundary_condition : g.boundary_condition;
-		const GridSample s = scheme == MARSMD_INTERP_LINEAR ? sample_linear(g, px, py, pz, bc)
-															: sample_cubic(g, px, py, pz, bc);
-		const float ns = -term.scale;
-		f.x += s.grad[0] * ns;
-		f.y += s.grad[1] * ns;
-		f.z += s.grad[2] * ns;
-		f.w += term.scale * s.value;
+	const int cnt = types.pmf_grid_count[type];
+#pragma unroll 1
+	for (int k = 0; k < max_terms; ++k) {
+		marsmd_grid_term term{-1, 0.0f, 0.0f, -1};
+		if (k < cnt)
+			term = types.pmf_grid_terms[b + k];
+		for_each_grid(term.grid_id, grids, [&](const marsmd_grid_view& g) {
+			if (g.data == nullptr)
+				return;
+			const int bc = term.boundary_condition >= 0 ? term.boundary_condition : g.boundary_condition;
+			const GridSample s = CUBIC ? sample_cubic(g, px, py, pz, bc) : sample_linear(g, px, py, pz, bc);
+			const float ns = -term.scale;
+			f.x += s.grad[0] * ns;
+			f.y += s.grad[1] * ns;
+			f.z += s.grad[2] * ns;
+			f.w += term.scale * s.value;
+		});
 	}
-#pragma unroll
+#pragma unroll 1
 	for (int a = 0; a < 3; ++a) {
-		const int gid = types.force_grid_id[type][a];
-		if (gid < 0)
-			continue;
-		const marsmd_grid_view& g = grids[gid];
-		if (g.data == nullptr)
-			continue;
-		const float v = scheme == MARSMD_INTERP_LINEAR ? sample_linear(g, px, py, pz, g.boundary_condition).value
-													   : sample_cubic(g, px, py, pz, g.boundary_condition).value;
-		const float add = types.force_grid_scale[type][a] * v;
-		if (a == 0)
-			f.x += add;
-		else if (a == 1)
-			f.y += add;
-		else
-			f.z += add;
+		for_each_grid(types.force_grid_id[type][a], grids, [&](const marsmd_grid_view& g) {
+			if (g.data == nullptr)
+				return;
+			const float v = CUBIC ? sample_cubic(g, px, py, pz, g.boundary_condition).value
+								  : sample_linear(g, px, py, pz, g.boundary_condition).value;
+			const float add = types.force_grid_scale[type][a] * v;
+			if (a == 0)
+				f.x += add;
+			else if (a == 1)
+				f.y += add;
+			else
+				f.z += add;
+		});
 	}
 	return f;
+}
+
+__device__ inline float4 position_dependent_force(float px, float py, float pz, int type,
+												  const marsmd_particle_types& types,
+												  const marsmd_grid_view* __restrict__ grids, const float ef[3],
+												  int scheme) {
+	return scheme == MARSMD_INTERP_LINEAR ? position_dependent_force_t<false>(px, py, pz, type, types, grids, ef)
+										  : position_dependent_force_t<true>(px, py, pz, type, types, grids, ef);
 }
 
 __device__ __forceinline__ float atomic_add_f32(float* p, float v) {

@@ -150,6 +150,107 @@ __global__ void k_angles(const int4* __restrict__ angles, const float4* __restri
 	}
 }
 
+__device__ __forceinline__ float3 cross3(const float3& v, const float3& w) {
+	return make_float3(v.y * w.z - v.z * w.y, v.z * w.x - v.x * w.z, v.x * w.y - v.y * w.x);
+}
+__device__ __forceinline__ float dot3(const float3& v, const float3& w) { return v.x * w.x + v.y * w.y + v.z * w.z; }
+// Vector3::rLength2 (src/Types/Vector3.h:233-236).
+__device__ __forceinline__ float rlen2(const float3& v) {
+	const float l2 = len2(v);
+	return l2 != 0.0f ? 1.0f / l2 : 0.0f;
+}
+
+// DihedralGeometry (src/Interactions/Bonded/BondGeometry.h:56-87) and
+// TabulatedDihedralComputer (BondComputer.h:296-343): forces on the four particles
+// (f1, f2 - f1, f3 - f2, -f3, each scaled by the clamped table force) and U/4.
+__device__ __forceinline__ void dihedral_forces(const DevBox& box, const float4& p0, const float4& p1,
+												const float4& p2, const float4& p3, const marsmd_table& t,
+												float3 out[4], float& quarter) {
+	const float3 ab = wrap_diff(box, sub3(p1, p0));
+	const float3 bc = wrap_diff(box, sub3(p2, p1));
+	const float3 cd = wrap_diff(box, sub3(p3, p2));
+	const float3 c_abc = cross3(ab, bc);
+	const float3 c_bcd = cross3(bc, cd);
+	const float3 c_x = cross3(bc, c_abc);
+	const float cos_phi = dot3(c_abc, c_bcd) / (sqrtf(len2(c_abc)) * sqrtf(len2(c_bcd)));
+	const float sin_phi = dot3(c_x, c_bcd) / (sqrtf(len2(c_x)) * sqrtf(len2(c_bcd)));
+	const float phi = atan2_series(sin_phi, cos_phi);
+	const float bcl = sqrtf(len2(bc));
+	const float r_abc = rlen2(c_abc), r_bcd = rlen2(c_bcd), r_bc = rlen2(bc);
+	const float s1 = -bcl * r_abc;
+	const float s3 = -bcl * r_bcd;
+	const float3 f1 = make_float3(s1 * c_abc.x, s1 * c_abc.y, s1 * c_abc.z);
+	const float3 f3 = make_float3(s3 * c_bcd.x, s3 * c_bcd.y, s3 * c_bcd.z);
+	const float a2 = -(dot3(ab, bc) * r_bc);
+	const float b2 = dot3(bc, cd) * r_bc;
+	const float3 f2 = make_float3(a2 * f1.x - b2 * f3.x, a2 * f1.y - b2 * f3.y, a2 * f1.z - b2 * f3.z);
+	const bool degenerate = (len2(ab) * len2(bc) * r_abc > 100.0f) || (len2(bc) * len2(cd) * r_bcd > 100.0f);
+	const FE v = table_compute(phi, t.pot, int(t.size), t.start, t.step_inv, t.is_periodic != 0);
+	float mag = degenerate ? 0.0f : v.f;
+	if (mag > 1000.0f)
+		mag = 1000.0f;
+	else if (mag < -1000.0f)
+		mag = -1000.0f;
+	const float3 g1 = make_float3(f1.x * mag, f1.y * mag, f1.z * mag);
+	const float3 g2 = make_float3(f2.x * mag, f2.y * mag, f2.z * mag);
+	const float3 g3 = make_float3(f3.x * mag, f3.y * mag, f3.z * mag);
+	out[0] = g1;
+	out[1] = make_float3(g2.x - g1.x, g2.y - g1.y, g2.z - g1.z);
+	out[2] = make_float3(g3.x - g2.x, g3.y - g2.y, g3.z - g2.z);
+	out[3] = make_float3(-g3.x, -g3.y, -g3.z);
+	quarter = v.e * 0.25f;
+}
+
+__global__ void k_dihedrals(const int4* __restrict__ dih, const float4* __restrict__ pos, float4* fe,
+							const marsmd_table* __restrict__ tables, const int32_t* __restrict__ tidx,
+							const int32_t* __restrict__ forms, DevBox box, int get_energy, uint64_t nd) {
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= nd)
+		return;
+	if (forms && forms[i] != MARSMD_FORM_TABULATED)
+		return;
+	const int4 q = dih[i];
+	float3 f[4];
+	float quarter;
+	dihedral_forces(box, pos[q.x], pos[q.y], pos[q.z], pos[q.w], tables[tidx ? tidx[i] : 0], f, quarter);
+	const int idx[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+	for (int k = 0; k < 4; ++k)
+		atomic_add4(&fe[idx[k]], f[k].x, f[k].y, f[k].z);
+	if (get_energy) {
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			atomic_add_f32(reinterpret_cast<float*>(&fe[idx[k]]) + 3, quarter);
+	}
+}
+
+// HarmonicRestraintComputer (BondComputer.h:109-129) with CalcDistance
+// (Interactions.h:42-53) and the harmonic form AnalyticalForceComputer<0>
+// (Bonded/Analytical.h:18-30) at r0 = 0: F = unit(wrap_diff(x - x0)) * (-k d), full U.
+__global__ void k_restraints(const int32_t* __restrict__ ids, const float4* __restrict__ pos, float4* fe,
+							 const float4* __restrict__ anchors, const float* __restrict__ kspring, DevBox box,
+							 int get_energy, uint64_t nr) {
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= nr)
+		return;
+	const int id = ids[i];
+	if (id < 0)
+		return;
+	const float3 r = wrap_diff(box, sub3(pos[id], anchors[i]));
+	const float d = sqrtf(len2(r));
+	float3 u = make_float3(0.0f, 0.0f, 0.0f);
+	if (d > 1e-6f) {
+		const float inv = 1.0f / d;
+		u = make_float3(r.x * inv, r.y * inv, r.z * inv);
+	}
+	const float k = kspring[i];
+	const float dr = d - 0.0f;
+	const float force = -k * dr;
+	atomic_add4(&fe[id], u.x * force, u.y * force, u.z * force);
+	if (get_energy)
+		atomic_add_f32(reinterpret_cast<float*>(&fe[id]) + 3, 0.5f * k * dr * dr);
+}
+
 // BDIntegrate::operator() (BD.h:44-92).
 __global__ void k_bd(marsmd_particle_view v, marsmd_particle_types types, DevBox box, float dt, uint64_t step,
 					 float kT, uint64_t n, uint64_t seed, uint32_t ctr, const marsmd_grid_view* grids,
@@ -320,6 +421,30 @@ void launch_tabulated_angles(const int32_t* angles, const float* pos, float* fe,
 	k_angles<<<grid_for(na, 256), 256, 0, s>>>(reinterpret_cast<const int4*>(angles),
 												reinterpret_cast<const float4*>(pos), reinterpret_cast<float4*>(fe),
 												tables, tidx, forms, make_devbox(box), energy ? 1 : 0, na);
+	MMD_LAUNCH_CHECK();
+}
+
+void launch_tabulated_dihedrals(const int32_t* dihedrals, const float* pos, float* fe, const marsmd_table* tables,
+								const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
+								uint64_t nd, hipStream_t s) {
+	if (nd == 0)
+		return;
+	k_dihedrals<<<grid_for(nd, 256), 256, 0, s>>>(reinterpret_cast<const int4*>(dihedrals),
+												   reinterpret_cast<const float4*>(pos),
+												   reinterpret_cast<float4*>(fe), tables, tidx, forms,
+												   make_devbox(box), energy ? 1 : 0, nd);
+	MMD_LAUNCH_CHECK();
+}
+
+void launch_harmonic_restraints(const int32_t* ids, const float* pos, float* fe, const float* anchors,
+								const float* kspring, const marsmd_box& box, bool energy, uint64_t nr,
+								hipStream_t s) {
+	if (nr == 0)
+		return;
+	k_restraints<<<grid_for(nr, 256), 256, 0, s>>>(ids, reinterpret_cast<const float4*>(pos),
+													reinterpret_cast<float4*>(fe),
+													reinterpret_cast<const float4*>(anchors), kspring,
+													make_devbox(box), energy ? 1 : 0, nr);
 	MMD_LAUNCH_CHECK();
 }
 

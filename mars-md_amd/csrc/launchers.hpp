@@ -16,6 +16,12 @@ void launch_tabulated_bonds(const int32_t* bonds, const float* pos, float* fe, c
 void launch_tabulated_angles(const int32_t* angles, const float* pos, float* fe, const marsmd_table* tables,
 							 const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
 							 uint64_t na, hipStream_t s);
+void launch_tabulated_dihedrals(const int32_t* dihedrals, const float* pos, float* fe, const marsmd_table* tables,
+								const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
+								uint64_t nd, hipStream_t s);
+void launch_harmonic_restraints(const int32_t* ids, const float* pos, float* fe, const float* anchors,
+								const float* kspring, const marsmd_box& box, bool energy, uint64_t nr,
+								hipStream_t s);
 void launch_bd(const marsmd_particle_view& v, const marsmd_particle_types& t, float dt, uint64_t step, float kT,
 			   uint64_t n, const marsmd_box& box, uint64_t seed, uint32_t ctr, const marsmd_grid_view* grids,
 			   const float ef[3], int scheme, hipStream_t s);

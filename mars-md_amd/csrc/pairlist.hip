@@ -234,24 +234,28 @@ struct WalkParams {
 // Canonical pair predicate of ZOrderCellNeighborKernel for sorted lo < hi
 // (ZOrderNeighbor.h:204-221): wrap_diff(pos_hi - pos_lo)^2 <= rc^2, not excluded by
 // lo's CSR row or a shared rigid body.
-__device__ __forceinline__ bool pair_in_list(const WalkParams& P, const float4& plo, const float4& phi) {
-	const float3 dr = wrap_diff(P.box, sub3(phi, plo));
-	if (!(len2(dr) <= P.rc2))
+__device__ __forceinline__ bool pair_predicate(const DevBox& box, float rc2, const marsmd_exclusion_view& ex,
+											   const float4& plo, const float4& phi) {
+	const float3 dr = wrap_diff(box, sub3(phi, plo));
+	if (!(len2(dr) <= rc2))
 		return false;
 	const int a = int(meta_canon(__float_as_uint(plo.w)));
 	const int b = int(meta_canon(__float_as_uint(phi.w)));
-	if (P.ex.rigid_body_id) {
-		const int body = P.ex.rigid_body_id[a];
-		if (body >= 0 && P.ex.rigid_body_id[b] == body)
+	if (ex.rigid_body_id) {
+		const int body = ex.rigid_body_id[a];
+		if (body >= 0 && ex.rigid_body_id[b] == body)
 			return false;
 	}
-	if (P.ex.offsets && a >= 0 && uint64_t(a) < P.ex.num_particles) {
-		const int e0 = P.ex.offsets[a], e1 = P.ex.offsets[a + 1];
+	if (ex.offsets && a >= 0 && uint64_t(a) < ex.num_particles) {
+		const int e0 = ex.offsets[a], e1 = ex.offsets[a + 1];
 		for (int e = e0; e < e1; ++e)
-			if (P.ex.excluded[e] == b)
+			if (ex.excluded[e] == b)
 				return false;
 	}
 	return true;
+}
+__device__ __forceinline__ bool pair_in_list(const WalkParams& P, const float4& plo, const float4& phi) {
+	return pair_predicate(P.box, P.rc2, P.ex, plo, phi);
 }
 
 __device__ __forceinline__ float axis_gap(float prel, int cell, float cw, float margin) {
@@ -262,7 +266,7 @@ __device__ __forceinline__ float axis_gap(float prel, int cell, float cw, float 
 
 template<bool FILL>
 __global__ __launch_bounds__(256) void k_walk(WalkParams P, uint64_t n, uint32_t* __restrict__ counts,
-											  const uint32_t* __restrict__ offsets,
+											  const uint32_t* __restrict__ tile_base,
 											  uint32_t* __restrict__ nbr) {
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkParams P, uint64_t n, uint32_t
 	const int cells = 1 << P.m;
 	const float prel[3] = {p.x - P.lo[0], p.y - P.lo[1], p.z - P.lo[2]};
 	uint32_t count = 0;
-	uint64_t wpos = FILL ? offsets[t] : 0;
+	const uint64_t wbase = FILL ? tile_base[t >> 6] : 0;
 	int lo[3], hi[3];
 #pragma unroll
 	for (int a = 0; a < 3; ++a) {
@@ -340,38 +344,299 @@ __global__ __launch_bounds__(256) void k_walk(WalkParams P, uint64_t n, uint32_t
 					const bool in = j > uint32_t(t) ? pair_in_list(P, p, q) : pair_in_list(P, q, p);
 					if (in) {
 						if (FILL)
-							nbr[wpos++] = j;
+							nbr[nbr_slot_index(wbase, uint32_t(t), count)] = j;
 						++count;
 					}
 				}
 			}
 		}
 	}
-	if (!FILL)
+	if (FILL) {
+		for (uint32_t k = count; (k & 3u) != 0; ++k)
+			nbr[nbr_slot_index(wbase, uint32_t(t), k)] = uint32_t(t);
+	} else {
 		counts[t] = count;
+	}
 }
 
-__global__ void k_half_counts(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ nbr,
-							  uint64_t n, uint32_t* __restrict__ half) {
+// Row length of each wave tile: 64 x the tile's largest count rounded up to 4.
+__global__ __launch_bounds__(256) void k_tile_rows(const uint32_t* __restrict__ counts, uint64_t n,
+												   uint32_t* __restrict__ rows) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	uint32_t m = t < n ? counts[t] : 0u;
+	for (int o = 32; o > 0; o >>= 1)
+		m = max(m, uint32_t(__shfl_xor(int(m), o, 64)));
+	const uint64_t w = t >> 6;
+	if ((threadIdx.x & 63) == 0 && w * 64 < n)
+		rows[w] = 64u * ((m + 3u) & ~3u);
+}
+
+// Largest count and total entries: grid-stride partials reduced in the workgroup, then
+// one atomic pair per workgroup (launched with a few hundred workgroups).
+__global__ __launch_bounds__(256) void k_count_stats(const uint32_t* __restrict__ counts, uint64_t n,
+													 unsigned long long* __restrict__ total,
+													 uint32_t* __restrict__ max_count) {
+	__shared__ uint32_t wmax[4];
+	__shared__ unsigned long long wsum[4];
+	uint32_t m = 0;
+	unsigned long long sum = 0;
+	for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += uint64_t(gridDim.x) * blockDim.x) {
+		const uint32_t c = counts[t];
+		m = max(m, c);
+		sum += c;
+	}
+	for (int o = 32; o > 0; o >>= 1) {
+		m = max(m, uint32_t(__shfl_xor(int(m), o, 64)));
+		sum += __shfl_xor(sum, o, 64);
+	}
+	const int w = threadIdx.x >> 6;
+	if ((threadIdx.x & 63) == 0) {
+		wmax[w] = m;
+		wsum[w] = sum;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t bm = 0;
+		unsigned long long bs = 0;
+		for (int k = 0; k < int(blockDim.x >> 6); ++k) {
+			bm = max(bm, wmax[k]);
+			bs += wsum[k];
+		}
+		atomicAdd(total, bs);
+		atomicMax(max_count, bm);
+	}
+}
+
+// tile_base[w] = w * row for w in [0, tiles] (rows of `row` entries per tile).
+__global__ void k_uniform_tile_base(uint32_t* __restrict__ tile_base, uint64_t tiles, uint32_t row) {
+	const uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (w <= tiles)
+		tile_base[w] = uint32_t(w * row);
+}
+
+// ------------------------------------------------------------ home-cell walk
+// One wave per home cell: a Morton cell of the home level h, whose particles are
+// contiguous in sorted order. The wave enumerates the fine cells (level m = h + s)
+// within `reach` fine cells of the home cell, keeps those whose box can lie within rc
+// of the home cell's box (both widened by the quantisation margin, so the test is
+// conservative across the periodic seam), stores their particle ranges as segments
+// in LDS, then streams the candidates 64 at a time (one coalesced load per lane) and
+// tests each against the wave's home particles, the candidate broadcast with
+// readlane. Predicate and orientation are k_walk's (the reference's), so the pair set
+// is the same; a particle's list is in candidate order. Entries past `cap` are not
+// written (the host sees the count and walks again with a larger cap).
+constexpr int kHomeWaves = 4;
+constexpr int kSegMax = 448;
+
+struct HomeParams {
+	const float4* spos;
+	const uint32_t* home_begin;
+	const uint32_t* home_end;
+	const uint32_t* fine_begin;
+	const uint32_t* fine_end;
+	uint32_t num_home;
+	int h;         // home level
+	int s;         // fine level m = h + s
+	int reach[3];  // fine cells enumerated beyond the home cell's span
+	int full[3];   // periodic axis enumerated whole at the fine level
+	float cw[3];   // fine cell width (quantised)
+	float margin[3];
+	float rc2;
+	DevBox box;
+	marsmd_exclusion_view ex;
+	uint32_t cap;  // entries per slot the uniform tile rows hold
+};
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+	const int lane = threadIdx.x & 63;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t t = __shfl_up(v, o, 64);
+		if (lane >= o)
+			v += t;
+	}
+	return v;
+}
+__device__ __forceinline__ uint32_t bcast_u32(uint32_t v, uint32_t lane) {
+	return uint32_t(__builtin_amdgcn_readlane(int(v), int(lane)));
+}
+__device__ __forceinline__ float bcast_f32(float v, uint32_t lane) {
+	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), int(lane)));
+}
+// LDS written by some lanes of this wave and read by others: order the accesses.
+__device__ __forceinline__ void wave_lds_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Tests the candidates of segments [0, nseg) (ntot particles) against this lane's home
+// particle t (when valid) and records the hits.
+template<bool FILL>
+__device__ __forceinline__ void home_test_segments(const HomeParams& P, const uint32_t* seg_b, uint32_t* seg_pre,
+												   uint32_t nseg, uint32_t ntot, bool valid, uint32_t t,
+												   const float4& p, uint32_t& cnt, uint32_t* __restrict__ nbr) {
+	const int lane = threadIdx.x & 63;
+	if (lane == 0)
+		seg_pre[nseg] = ntot;
+	wave_lds_sync();
+	const uint64_t row = (uint64_t(t) >> 6) * 64u * P.cap;
+	for (uint32_t q0 = 0; q0 < ntot; q0 += 64) {
+		const uint32_t qi = q0 + uint32_t(lane);
+		uint32_t j = 0xFFFFFFFFu;
+		float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		if (qi < ntot) {
+			uint32_t lo = 0, hi = nseg; // seg_pre[lo] <= qi < seg_pre[hi]
+			while (hi - lo > 1) {
+				const uint32_t mid = (lo + hi) >> 1;
+				if (seg_pre[mid] <= qi)
+					lo = mid;
+				else
+					hi = mid;
+			}
+			j = seg_b[lo] + (qi - seg_pre[lo]);
+			q = P.spos[j];
+		}
+		const uint32_t m = min(64u, ntot - q0);
+		for (uint32_t u = 0; u < m; ++u) {
+			const uint32_t ju = bcast_u32(j, u);
+			const float4 qu = make_float4(bcast_f32(q.x, u), bcast_f32(q.y, u), bcast_f32(q.z, u), bcast_f32(q.w, u));
+			if (valid && ju != t) {
+				const bool in = ju > t ? pair_predicate(P.box, P.rc2, P.ex, p, qu)
+									   : pair_predicate(P.box, P.rc2, P.ex, qu, p);
+				if (in) {
+					if (FILL && cnt < P.cap)
+						nbr[nbr_slot_index(row, t, cnt)] = ju;
+					++cnt;
+				}
+			}
+		}
+	}
+	wave_lds_sync();
+}
+
+template<bool FILL>
+__device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg_b, uint32_t* seg_pre,
+							   uint32_t* __restrict__ counts, uint32_t* __restrict__ nbr) {
+	const int lane = threadIdx.x & 63;
+	const uint32_t hb = P.home_begin[home], he = P.home_end[home];
+	if (hb >= he)
+		return;
+	const int hc[3] = {int(compact_by3(home >> 2)), int(compact_by3(home >> 1)), int(compact_by3(home))};
+	const int fine_cells = 1 << (P.h + P.s);
+	const int span = 1 << P.s;
+	int lo[3], nax[3];
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		if (P.full[a]) {
+			lo[a] = 0;
+			nax[a] = fine_cells;
+		} else {
+			lo[a] = hc[a] * span - P.reach[a];
+			nax[a] = span + 2 * P.reach[a];
+		}
+	}
+	const int total = nax[0] * nax[1] * nax[2];
+	const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+	for (uint32_t c0 = hb; c0 < he; c0 += 64) {
+		const uint32_t t = c0 + uint32_t(lane);
+		const bool valid = t < he;
+		const float4 p = valid ? P.spos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		uint32_t cnt = 0;
+		uint32_t nseg = 0, ntot = 0; // wave-uniform
+		for (int cb = 0; cb < total; cb += 64) {
+			const int k = cb + lane;
+			uint32_t b = 0, e = 0;
+			if (k < total) {
+				const int iv[3] = {k % nax[0], (k / nax[0]) % nax[1], k / (nax[0] * nax[1])};
+				float gap2 = 0.0f;
+				bool keep = true;
+				uint32_t code = 0;
+#pragma unroll
+				for (int a = 0; a < 3; ++a) {
+					int u = lo[a] + iv[a];
+					if (!P.full[a]) {
+						const int h0 = hc[a] * span;
+						const int apart = u < h0 ? h0 - u - 1 : (u >= h0 + span ? u - h0 - span : 0);
+						const float g = fmaxf(0.0f, float(apart) * P.cw[a] - 2.0f * P.margin[a]);
+						gap2 += g * g;
+						if (P.box.periodic[a])
+							u = (u + fine_cells) & (fine_cells - 1);
+						else if (u < 0 || u >= fine_cells)
+							keep = false;
+					}
+					code |= split_by3(uint32_t(u) & 1023u) << (2 - a);
+				}
+				if (keep && gap2 <= P.rc2) {
+					b = P.fine_begin[code];
+					e = P.fine_end[code];
+				}
+			}
+			const uint32_t c = e > b ? e - b : 0u;
+			const uint64_t mask = __ballot(c > 0);
+			const uint32_t nnew = uint32_t(__popcll(mask));
+			if (nseg + nnew > uint32_t(kSegMax)) {
+				home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr);
+				nseg = 0;
+				ntot = 0;
+			}
+			const uint32_t incl = wave_inclusive_scan(c);
+			const uint32_t sum = bcast_u32(incl, 63);
+			if (c > 0) {
+				const uint32_t r = nseg + uint32_t(__popcll(mask & lt_mask));
+				seg_b[r] = b;
+				seg_pre[r] = ntot + incl - c;
+			}
+			nseg += nnew;
+			ntot += sum;
+		}
+		home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr);
+		if (valid) {
+			counts[t] = cnt;
+			if (FILL && cnt <= P.cap) {
+				const uint64_t row = (uint64_t(t) >> 6) * 64u * P.cap;
+				for (uint32_t k = cnt; (k & 3u) != 0; ++k)
+					nbr[nbr_slot_index(row, t, k)] = t;
+			}
+		}
+	}
+}
+
+template<bool FILL>
+__global__ __launch_bounds__(256) void k_walk_home(HomeParams P, uint32_t* __restrict__ counts,
+												   uint32_t* __restrict__ nbr) {
+	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
+	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
+	const int wv = threadIdx.x >> 6;
+	const TileRange R = xcd_tiles((P.num_home + kHomeWaves - 1) / kHomeWaves);
+	for (uint32_t g = R.tile; g < R.end; g += R.stride) {
+		const uint32_t home = g * kHomeWaves + uint32_t(wv);
+		if (home < P.num_home)
+			walk_home_cell<FILL>(P, home, seg_b[wv], seg_pre[wv], counts, nbr);
+	}
+}
+
+__global__ void k_half_counts(NbrList L, uint64_t n, uint32_t* __restrict__ half) {
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
+	const uint64_t base = L.tile_base[t >> 6];
 	uint32_t h = 0;
-	for (uint32_t k = offsets[t]; k < offsets[t + 1]; ++k)
-		h += nbr[k] > uint32_t(t) ? 1u : 0u;
+	for (uint32_t k = 0; k < L.count[t]; ++k)
+		h += L.nbr[nbr_slot_index(base, uint32_t(t), k)] > uint32_t(t) ? 1u : 0u;
 	half[t] = h;
 }
 
-__global__ void k_write_pairs(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ nbr,
-							  const uint32_t* __restrict__ half_off, const float4* __restrict__ spos,
+__global__ void k_write_pairs(NbrList L, const uint32_t* __restrict__ half_off, const float4* __restrict__ spos,
 							  uint64_t n, int32_t* __restrict__ pairs) {
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
 	uint64_t w = half_off[t];
+	const uint64_t base = L.tile_base[t >> 6];
 	const int32_t a = int32_t(meta_canon(__float_as_uint(spos[t].w)));
-	for (uint32_t k = offsets[t]; k < offsets[t + 1]; ++k) {
-		const uint32_t j = nbr[k];
+	for (uint32_t k = 0; k < L.count[t]; ++k) {
+		const uint32_t j = L.nbr[nbr_slot_index(base, uint32_t(t), k)];
 		if (j > uint32_t(t)) {
 			const int32_t b = int32_t(meta_canon(__float_as_uint(spos[j].w)));
 			pairs[2 * w] = a < b ? a : b;
@@ -408,16 +673,6 @@ __global__ __launch_bounds__(256) void k_max_disp(const float4* __restrict__ cur
 		m = fmaxf(m, __shfl_xor(m, o, 64));
 	if ((threadIdx.x & 63) == 0)
 		atomicMax(out_bits, __float_as_uint(m)); // non-negative floats order as uints
-}
-
-__global__ void k_max_count(const uint32_t* __restrict__ offsets, uint64_t n, uint32_t* __restrict__ out) {
-	uint32_t m = 0;
-	for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += uint64_t(gridDim.x) * blockDim.x)
-		m = max(m, offsets[t + 1] - offsets[t]);
-	for (int o = 32; o > 0; o >>= 1)
-		m = max(m, uint32_t(__shfl_xor(int(m), o, 64)));
-	if ((threadIdx.x & 63) == 0)
-		atomicMax(out, m);
 }
 
 } // namespace
@@ -533,7 +788,9 @@ void PairlistImpl::ensure(uint64_t nn) {
 		raise(MARSMD_ERR_VALUE, "pairlist: particle count exceeds 2^24");
 	spos.resize(nn);
 	counts.resize(nn + 1);
-	offsets.resize(nn + 1);
+	const uint64_t tiles = (nn + 63) / 64;
+	tile_rows.resize(tiles + 1);
+	tile_base.resize(tiles + 1);
 	scan_scratch.resize(scan_scratch_bytes(nn + 1));
 }
 
@@ -609,10 +866,103 @@ void PairlistImpl::walk(bool fill, bool use_ref, hipStream_t s) {
 	P.ex = excl;
 	const unsigned grid = grid_for(n, 256);
 	if (fill)
-		k_walk<true><<<grid, 256, 0, s>>>(P, n, counts.data(), offsets.data(), nbr.data());
+		k_walk<true><<<grid, 256, 0, s>>>(P, n, counts.data(), tile_base.data(), nbr.data());
 	else
-		k_walk<false><<<grid, 256, 0, s>>>(P, n, counts.data(), offsets.data(), nbr.data());
+		k_walk<false><<<grid, 256, 0, s>>>(P, n, counts.data(), tile_base.data(), nbr.data());
 	MMD_LAUNCH_CHECK();
+}
+
+// Largest count and total entries of `counts` into dev_scalars [2] and [4..5].
+void PairlistImpl::count_stats(uint64_t nn, hipStream_t s) {
+	fill_u32(dev_scalars.data() + 2, 0u, 4, s);
+	k_count_stats<<<std::min<unsigned>(grid_for(nn, 256), 240), 256, 0, s>>>(
+		counts.data(), nn, reinterpret_cast<unsigned long long*>(dev_scalars.data() + 4), dev_scalars.data() + 2);
+	MMD_LAUNCH_CHECK();
+}
+
+// The home-cell walk on the build grid (k_walk_home), writing uniform tile rows of
+// home_cap entries per slot and walking again once with a larger cap when a count
+// exceeds it. Returns false when it does not apply (no level below the build grid, or
+// rows beyond 32-bit offsets); `outside` reports a clamped coordinate (then the
+// caller walks the reference grid).
+bool PairlistImpl::walk_home(hipStream_t s, bool& outside) {
+	const int m = build_grid.m;
+	if (m < 1)
+		return false;
+	int h = 0;
+	while (h + 1 <= m - 1 && (n >> (3 * (h + 1))) >= 24)
+		++h;
+	const uint64_t tiles = (n + 63) / 64;
+	HomeParams P{};
+	P.spos = spos.data();
+	P.h = h;
+	P.s = m - h;
+	P.num_home = uint32_t(1) << (3 * h);
+	const int fine_cells = 1 << m;
+	const int span = 1 << (m - h);
+	const int k = 1024 / fine_cells;
+	double volume = 1.0;
+	for (int a = 0; a < 3; ++a) {
+		const bool per = box.periodic[a] != 0 && box.size[a] > 0.0f;
+		P.reach[a] = build_grid.r[a];
+		P.full[a] = (per && span + 2 * build_grid.r[a] >= fine_cells) ? 1 : 0;
+		const float extent = std::fmax(domain_max[a] - domain_min[a], 1e-6f);
+		volume *= double(extent);
+		P.cw[a] = float(double(extent) * k / 1023.0);
+		P.margin[a] = float(double(extent) / 1023.0 * 1.001 + 1e-5 * double(extent) + 1e-3);
+	}
+	P.rc2 = cutoff_squared;
+	P.box = make_devbox(box);
+	P.ex = excl;
+	if (home_cap == 0) {
+		// first build: the mean count of a uniform system, with room for its fluctuations
+		const double mean = double(n) / volume * 4.0 / 3.0 * 3.141592653589793 * double(cutoff) * cutoff * cutoff;
+		home_cap = uint32_t(std::min(1.6 * mean + 16.0, 4096.0) + 3) & ~3u;
+	}
+	const uint64_t home_cells = uint64_t(P.num_home);
+	home_begin.resize(home_cells);
+	home_end.resize(home_cells);
+	fill_u32(home_begin.data(), 0u, home_cells, s);
+	fill_u32(home_end.data(), 0u, home_cells, s);
+	k_cell_ranges<<<grid_for(n, 256), 256, 0, s>>>(sorter.codes.data(), n, 3 * (10 - h), home_begin.data(),
+													home_end.data());
+	MMD_LAUNCH_CHECK();
+	P.home_begin = home_begin.data();
+	P.home_end = home_end.data();
+	P.fine_begin = cell_begin.data();
+	P.fine_end = cell_end.data();
+	const unsigned groups = unsigned((P.num_home + kHomeWaves - 1) / kHomeWaves);
+	const unsigned grid = std::max(8u, (groups + 7u) / 8u * 8u);
+	for (int attempt = 0; attempt < 2; ++attempt) {
+		const uint64_t storage = tiles * 64 * uint64_t(home_cap);
+		if (storage >= (uint64_t(1) << 32))
+			return false;
+		if (storage > nbr.count)
+			nbr.resize(storage + storage / 8 + 1024);
+		P.cap = home_cap;
+		k_uniform_tile_base<<<grid_for(tiles + 1, 256), 256, 0, s>>>(tile_base.data(), tiles, 64u * home_cap);
+		MMD_LAUNCH_CHECK();
+		k_walk_home<true><<<grid, 256, 0, s>>>(P, counts.data(), nbr.data());
+		MMD_LAUNCH_CHECK();
+		count_stats(n, s);
+		uint32_t host[5] = {0, 0, 0, 0, 0};
+		MMD_HIP(hipMemcpyAsync(&host[0], sorter.outside_flag.data(), 4, hipMemcpyDeviceToHost, s));
+		MMD_HIP(hipMemcpyAsync(&host[1], dev_scalars.data() + 2, 16, hipMemcpyDeviceToHost, s));
+		MMD_HIP(hipStreamSynchronize(s));
+		if (host[0] != 0) {
+			outside = true;
+			return true;
+		}
+		max_neighbors = host[1];
+		std::memcpy(&nbr_entries, &host[3], 8);
+		if (max_neighbors <= home_cap) {
+			nbr_storage = storage;
+			home_level = h;
+			return true;
+		}
+		home_cap = (max_neighbors + max_neighbors / 8 + 4 + 3) & ~3u;
+	}
+	return false;
 }
 
 void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool canon_from_meta,
@@ -629,6 +979,7 @@ void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool 
 	built = true;
 	if (nn == 0) {
 		nbr_entries = 0;
+		nbr_storage = 0;
 		return;
 	}
 	// Morton domain (ZOrderPairlist.cpp:131-146, 368-411).
@@ -671,7 +1022,28 @@ void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool 
 	MMD_LAUNCH_CHECK();
 	choose_grids(nn);
 	bool use_ref = false;
-	for (int attempt = 0; attempt < 2; ++attempt) {
+	home_level = -1;
+	bool home_done = false;
+	if (ref_complete) {
+		const CellGrid& g = build_grid;
+		const uint64_t num_cells = uint64_t(1) << (3 * g.m);
+		cell_begin.resize(num_cells);
+		cell_end.resize(num_cells);
+		fill_u32(cell_begin.data(), 0u, num_cells, s);
+		fill_u32(cell_end.data(), 0u, num_cells, s);
+		k_cell_ranges<<<grid_for(nn, 256), 256, 0, s>>>(sorter.codes.data(), nn, 3 * (10 - g.m), cell_begin.data(),
+														 cell_end.data());
+		MMD_LAUNCH_CHECK();
+		bool outside = false;
+		home_done = walk_home(s, outside);
+		if (outside) {
+			// A coordinate outside the Morton domain was clamped: walk the reference grid.
+			home_done = false;
+			home_level = -1;
+			use_ref = true;
+		}
+	}
+	for (int attempt = 0; attempt < 2 && !home_done; ++attempt) {
 		if (!ref_complete)
 			use_ref = true;
 		const CellGrid& g = use_ref ? ref_grid : build_grid;
@@ -684,10 +1056,15 @@ void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool 
 														 cell_end.data());
 		MMD_LAUNCH_CHECK();
 		walk(false, use_ref, s);
-		exclusive_scan_u32(counts.data(), offsets.data(), nn, scan_scratch.data(), s);
-		uint32_t host[2] = {0, 0};
-		MMD_HIP(hipMemcpyAsync(&host[0], offsets.data() + nn, 4, hipMemcpyDeviceToHost, s));
+		const uint64_t tiles = (nn + 63) / 64;
+		k_tile_rows<<<grid_for(nn, 256), 256, 0, s>>>(counts.data(), nn, tile_rows.data());
+		MMD_LAUNCH_CHECK();
+		count_stats(nn, s);
+		exclusive_scan_u32(tile_rows.data(), tile_base.data(), tiles, scan_scratch.data(), s);
+		uint32_t host[6] = {0, 0, 0, 0, 0, 0};
+		MMD_HIP(hipMemcpyAsync(&host[0], tile_base.data() + tiles, 4, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipMemcpyAsync(&host[1], sorter.outside_flag.data(), 4, hipMemcpyDeviceToHost, s));
+		MMD_HIP(hipMemcpyAsync(&host[2], dev_scalars.data() + 2, 16, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipStreamSynchronize(s));
 		if (host[1] != 0 && !use_ref) {
 			// A coordinate outside the Morton domain was clamped: cell geometry no
@@ -695,7 +1072,15 @@ void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool 
 			use_ref = true;
 			continue;
 		}
-		nbr_entries = host[0];
+		uint64_t total = 0;
+		std::memcpy(&total, &host[4], 8);
+		max_neighbors = host[2];
+		// Each row is padded to a multiple of 4 within a tile of 64: the padded
+		// storage fits 32-bit offsets when this bound does.
+		if (uint64_t(max_neighbors + 3) * 64 * tiles >= (uint64_t(1) << 32))
+			raise(MARSMD_ERR_CAPACITY, "Pairlist: neighbour storage exceeds 2^32 entries");
+		nbr_entries = total;
+		nbr_storage = host[0];
 		break;
 	}
 	const uint64_t np = nbr_entries / 2;
@@ -707,13 +1092,11 @@ void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool 
 										   std::to_string(max_pairs_ceiling));
 		ref_max_pairs = std::min<uint64_t>(max_pairs_ceiling, np + np / 4);
 	}
-	if (nbr_entries > nbr.count)
-		nbr.resize(nbr_entries + nbr_entries / 8 + 1024);
-	walk(true, use_ref, s);
-	// statistics: max neighbours
-	fill_u32(dev_scalars.data() + 2, 0u, 1, s);
-	k_max_count<<<std::min<unsigned>(grid_for(nn, 256), 512), 256, 0, s>>>(offsets.data(), nn, dev_scalars.data() + 2);
-	MMD_LAUNCH_CHECK();
+	if (!home_done) {
+		if (nbr_storage > nbr.count)
+			nbr.resize(nbr_storage + nbr_storage / 8 + 1024);
+		walk(true, use_ref, s);
+	}
 	const auto t1 = std::chrono::high_resolution_clock::now();
 	last_build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 }
@@ -744,11 +1127,10 @@ const int32_t* PairlistImpl::neighbor_pairs(hipStream_t s) {
 	if (n > 0 && np > 0) {
 		half_counts.resize(n + 1);
 		half_offsets.resize(n + 1);
-		k_half_counts<<<grid_for(n, 256), 256, 0, s>>>(offsets.data(), nbr.data(), n, half_counts.data());
+		k_half_counts<<<grid_for(n, 256), 256, 0, s>>>(list(), n, half_counts.data());
 		MMD_LAUNCH_CHECK();
 		exclusive_scan_u32(half_counts.data(), half_offsets.data(), n, scan_scratch.data(), s);
-		k_write_pairs<<<grid_for(n, 256), 256, 0, s>>>(offsets.data(), nbr.data(), half_offsets.data(), spos.data(),
-														n, pairs.data());
+		k_write_pairs<<<grid_for(n, 256), 256, 0, s>>>(list(), half_offsets.data(), spos.data(), n, pairs.data());
 		MMD_LAUNCH_CHECK();
 	}
 	pairs_valid = true;
@@ -781,10 +1163,7 @@ marsmd_pairlist_stats PairlistImpl::stats() const {
 	st.num_particles = n;
 	st.num_pairs = num_pairs();
 	st.average_neighbors_per_particle = n > 0 ? double(float(st.num_pairs) / float(n)) : 0.0;
-	uint32_t mx = 0;
-	if (n > 0)
-		(void)hipMemcpy(&mx, dev_scalars.data() + 2, 4, hipMemcpyDeviceToHost);
-	st.max_neighbors_per_particle = mx;
+	st.max_neighbors_per_particle = n > 0 ? max_neighbors : 0;
 	st.build_time_ms = last_build_ms;
 	st.cell_grid_bits = ref_grid.m;
 	for (int a = 0; a < 3; ++a) {

@@ -35,6 +35,22 @@ struct ZSort {
 	void remap_indices(int32_t* idx, uint64_t count, hipStream_t s) const;
 };
 
+// Full neighbour list in wave tiles. Slots 64w .. 64w+63 form tile w, whose rows hold
+// R_w = 4 * ceil(max count in the tile / 4) entries per slot, stored as R_w / 4 groups
+// of 64 slots x 4 indices: entry k of slot t = 64w + l is
+//   nbr[tile_base[w] + (k / 4) * 256 + 4 l + k % 4].
+// One 16-byte load per lane therefore brings 4 neighbours and a wave's loads are
+// contiguous. A slot's last group is padded with the slot itself (a zero-distance
+// pair, which the force kernels skip like the reference's d < 1e-6 test).
+struct NbrList {
+	const uint32_t* count;
+	const uint32_t* tile_base;
+	const uint32_t* nbr;
+};
+__host__ __device__ inline uint64_t nbr_slot_index(uint64_t tile_base, uint32_t t, uint32_t k) {
+	return tile_base + uint64_t(k >> 2) * 256u + uint64_t(t & 63u) * 4u + (k & 3u);
+}
+
 struct PairlistImpl {
 	uint64_t max_particles = 0;
 	uint64_t ref_max_pairs = 0;     // Pairlist::max_pairs_ bookkeeping (grows x1.25)
@@ -49,9 +65,14 @@ struct PairlistImpl {
 	DevArray<float4> spos;
 	// build-grid cell ranges
 	DevArray<uint32_t> cell_begin, cell_end;
-	// CSR full neighbour list over sorted indices
-	DevArray<uint32_t> counts, offsets, nbr;
+	// full neighbour list over sorted indices in wave tiles (see NbrList)
+	DevArray<uint32_t> counts, tile_rows, tile_base, nbr;
+	// home-cell walk: home-level cell ranges and the per-slot capacity of its rows
+	DevArray<uint32_t> home_begin, home_end;
+	uint32_t home_cap = 0;
+	int home_level = -1; // level of the last home-cell walk (-1: count / scan / fill path)
 	uint64_t nbr_entries = 0; // = 2 * num_pairs
+	uint64_t nbr_storage = 0; // padded entries held by nbr
 	DevArray<uint8_t> scan_scratch;
 	// lazily materialised int2 pairs (canonical, x < y)
 	DevArray<int32_t> pairs;
@@ -64,7 +85,7 @@ struct PairlistImpl {
 	float domain_min[3]{}, domain_max[3]{};
 	double last_build_ms = 0.0;
 	uint32_t max_neighbors = 0;
-	// misc device scalars: [0] outside flag, [1] max displacement^2 bits, [2] max count
+	// misc device scalars: [1] max displacement^2 bits, [2] max count, [4..5] entries (u64)
 	DevArray<uint32_t> dev_scalars;
 	DevArray<float> dev_domain;
 	DevArray<uint32_t> ref_begin, ref_end, ref_neighbors;
@@ -81,6 +102,7 @@ struct PairlistImpl {
 	float max_displacement(const float4* pos, const uint32_t* sorted_of_slot, uint64_t n,
 						   hipStream_t s);
 	uint64_t num_pairs() const { return nbr_entries / 2; }
+	NbrList list() const { return NbrList{counts.data(), tile_base.data(), nbr.data()}; }
 	const int32_t* neighbor_pairs(hipStream_t s);
 	void reference_cells(hipStream_t s, uint64_t* num_cells);
 	marsmd_pairlist_stats stats() const;
@@ -88,6 +110,8 @@ struct PairlistImpl {
   private:
 	void choose_grids(uint64_t n);
 	void walk(bool fill, bool use_ref_grid, hipStream_t s);
+	bool walk_home(hipStream_t s, bool& outside);
+	void count_stats(uint64_t n, hipStream_t s);
 	void ensure(uint64_t n);
 };
 

@@ -67,58 +67,91 @@ struct StepParams {
 
 __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
 
-// Sum of the nonbonded pair terms on slot t from its neighbour list.
+// One pair term on slot t (PairNonbondedComputer, Pairwise.h:184-305) in canonical
+// orientation: r = wrap_diff(pos[higher canon] - pos[lower canon]), the lower takes -F.
+__device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t tt, const float4& q,
+										const NbTables& T, const float* __restrict__ knots, const uint32_t* tagm,
+										const DevBox& box, float cutoff2, bool energy, float& fx, float& fy,
+										float& fz, float& e) {
+	const uint32_t mj = fb(q.w);
+	const uint32_t cj = meta_canon(mj);
+	const uint32_t tj = meta_type(mj);
+	const bool t_is_x = ct < cj;
+	const float4 px = t_is_x ? p : q;
+	const float4 py = t_is_x ? q : p;
+	const uint32_t tx = t_is_x ? tt : tj;
+	const uint32_t ty = t_is_x ? tj : tt;
+	const uint32_t tag = tagm[tx * uint32_t(T.num_types) + ty];
+	if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
+		return;
+	const float3 r = wrap_diff(box, sub3(py, px));
+	const float d = sqrtf(len2(r));
+	float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+	if (d > 1e-6f) {
+		const float inv = 1.0f / d;
+		ux = inv * r.x;
+		uy = inv * r.y;
+		uz = inv * r.z;
+	}
+	if (d < 1e-6f)
+		return;
+	if (cutoff2 > 0.0f && d * d > cutoff2)
+		return;
+	const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
+	const FE v = table_compute(d, knots + T.offset[ti], T.size[ti], T.start[ti], T.step_inv[ti], false);
+	const float fm = 0.0f + v.f;
+	const float gx = fm * ux, gy = fm * uy, gz = fm * uz;
+	if (t_is_x) {
+		fx += -gx;
+		fy += -gy;
+		fz += -gz;
+	} else {
+		fx += gx;
+		fy += gy;
+		fz += gz;
+	}
+	if (energy)
+		e += (0.0f + v.e) * 0.5f;
+}
+
+// Sum of the nonbonded pair terms on slot t over its neighbour list, in list order.
+// A lane's list is read in groups of 4 indices (one 16-byte load) and pipelined three
+// deep: while group g is evaluated, the positions of group g+1 and the indices of
+// group g+2 are in flight. The padding entries of a row are t itself (skipped as d = 0).
 __device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const float4* __restrict__ pos,
-										   const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ nbr,
-										   const NbTables& T, const float* __restrict__ knots, const uint32_t* tagm,
-										   const DevBox& box, float cutoff2, bool energy) {
+										   const NbrList& L, const NbTables& T, const float* __restrict__ knots,
+										   const uint32_t* tagm, const DevBox& box, float cutoff2, bool energy) {
 	const uint32_t mt = fb(p.w);
 	const uint32_t ct = meta_canon(mt);
 	const uint32_t tt = meta_type(mt);
 	float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
-	const uint32_t k0 = offsets[t], k1 = offsets[t + 1];
-	for (uint32_t k = k0; k < k1; ++k) {
-		const uint32_t j = nbr[k];
-		const float4 q = pos[j];
-		const uint32_t mj = fb(q.w);
-		const uint32_t cj = meta_canon(mj);
-		const uint32_t tj = meta_type(mj);
-		const bool t_is_x = ct < cj;
-		const float4 px = t_is_x ? p : q;
-		const float4 py = t_is_x ? q : p;
-		const uint32_t tx = t_is_x ? tt : tj;
-		const uint32_t ty = t_is_x ? tj : tt;
-		const uint32_t tag = tagm[tx * uint32_t(T.num_types) + ty];
-		if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
-			continue;
-		const float3 r = wrap_diff(box, sub3(py, px));
-		const float d = sqrtf(len2(r));
-		float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-		if (d > 1e-6f) {
-			const float inv = 1.0f / d;
-			ux = inv * r.x;
-			uy = inv * r.y;
-			uz = inv * r.z;
+	const uint32_t ng = (L.count[t] + 3u) >> 2;
+	if (ng == 0)
+		return make_float4(fx, fy, fz, e);
+	const uint4* __restrict__ row = reinterpret_cast<const uint4*>(L.nbr + L.tile_base[t >> 6]) + (t & 63u);
+	const uint4 j0 = row[0];
+	uint4 jn = ng > 1 ? row[64] : j0;
+	float4 q0 = pos[j0.x], q1 = pos[j0.y], q2 = pos[j0.z], q3 = pos[j0.w];
+	for (uint32_t g = 0; g < ng; ++g) {
+		float4 n0 = q0, n1 = q1, n2 = q2, n3 = q3;
+		uint4 jnn = jn;
+		if (g + 1 < ng) {
+			n0 = pos[jn.x];
+			n1 = pos[jn.y];
+			n2 = pos[jn.z];
+			n3 = pos[jn.w];
+			if (g + 2 < ng)
+				jnn = row[uint64_t(g + 2) * 64u];
 		}
-		if (d < 1e-6f)
-			continue;
-		if (cutoff2 > 0.0f && d * d > cutoff2)
-			continue;
-		const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
-		const FE v = table_compute(d, knots + T.offset[ti], T.size[ti], T.start[ti], T.step_inv[ti], false);
-		const float fm = 0.0f + v.f;
-		const float gx = fm * ux, gy = fm * uy, gz = fm * uz;
-		if (t_is_x) {
-			fx += -gx;
-			fy += -gy;
-			fz += -gz;
-		} else {
-			fx += gx;
-			fy += gy;
-			fz += gz;
-		}
-		if (energy)
-			e += (0.0f + v.e) * 0.5f;
+		nb_pair(p, ct, tt, q0, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair(p, ct, tt, q1, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair(p, ct, tt, q2, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair(p, ct, tt, q3, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		q0 = n0;
+		q1 = n1;
+		q2 = n2;
+		q3 = n3;
+		jn = jnn;
 	}
 	return make_float4(fx, fy, fz, e);
 }
@@ -137,19 +170,16 @@ __device__ __forceinline__ const float* stage_tables(const NbTables& T, const fl
 	return T.lds ? lds_knots : knots_g;
 }
 
-__global__ __launch_bounds__(kBlock) void k_nb_forces(const float4* __restrict__ pos, uint64_t n,
-													  const uint32_t* __restrict__ offsets,
-													  const uint32_t* __restrict__ nbr, NbTables T,
-													  const float* __restrict__ knots_g, DevBox box, float cutoff2,
-													  int energy, float4* __restrict__ force) {
+__global__ __launch_bounds__(kBlock) void k_nb_forces(const float4* __restrict__ pos, uint64_t n, NbrList L,
+													  NbTables T, const float* __restrict__ knots_g, DevBox box,
+													  float cutoff2, int energy, float4* __restrict__ force) {
 	extern __shared__ float lds_knots[];
 	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
 	const float* knots = stage_tables(T, knots_g, lds_knots, lds_tag);
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
-	const float4 p = pos[t];
-	force[t] = nb_force(uint32_t(t), p, pos, offsets, nbr, T, knots, lds_tag, box, cutoff2, energy != 0);
+	force[t] = nb_force(uint32_t(t), pos[t], pos, L, T, knots, lds_tag, box, cutoff2, energy != 0);
 }
 
 __global__ void k_zero_force(float4* __restrict__ f, uint64_t n) {
@@ -276,49 +306,54 @@ __global__ __launch_bounds__(kBlock) void k_integrate_baoab(float4* __restrict__
 	mom[t] = mo;
 }
 
-// The fused step: nonbonded + bonded (from `force` when has_bonded) + PMF + integrator.
+// The fused step: nonbonded + bonded (from `force` when has_bonded) + PMF + integrator,
+// persistent over XCD-local tiles so the table knots are staged once per workgroup.
+// Bounded to 128 VGPRs (4 waves per SIMD; the tricubic sampler alone would take ~140
+// and leave 3): measured faster despite a few dwords of spill.
 template<int INTEG, bool ENERGY>
-__global__ __launch_bounds__(kBlock) void k_step(const float4* __restrict__ pos, float4* __restrict__ pos_next,
+__global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ pos, float4* __restrict__ pos_next,
 												 float4* __restrict__ mom, float4* __restrict__ force, uint64_t n,
-												 const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ nbr,
-												 NbTables T, const float* __restrict__ knots_g,
+												 NbrList L, NbTables T, const float* __restrict__ knots_g,
 												 marsmd_particle_types types,
 												 const marsmd_grid_view* __restrict__ grids, StepParams S,
 												 int has_bonded, int store_force) {
 	extern __shared__ float lds_knots[];
 	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
 	const float* knots = stage_tables(T, knots_g, lds_knots, lds_tag);
-	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (t >= n)
-		return;
-	const float4 p = pos[t];
-	float4 f = nb_force(uint32_t(t), p, pos, offsets, nbr, T, knots, lds_tag, S.box, S.cutoff2, ENERGY);
-	if (has_bonded) {
-		const float4 b = force[t];
-		f.x += b.x;
-		f.y += b.y;
-		f.z += b.z;
-		f.w += b.w;
-	}
-	float4 pn;
-	if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
-		pn = bd_update(p, f, types, S.has_grids ? grids : nullptr, S);
-	} else {
-		if (S.has_grids) {
-			const float4 g = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, grids, S.ef,
-													  S.scheme);
-			f.x += g.x;
-			f.y += g.y;
-			f.z += g.z;
-			f.w += g.w;
+	const TileRange R = xcd_tiles(uint32_t((n + kBlock - 1) / kBlock));
+	for (uint32_t tile = R.tile; tile < R.end; tile += R.stride) {
+		const uint64_t t = uint64_t(tile) * kBlock + threadIdx.x;
+		if (t >= n)
+			continue;
+		const float4 p = pos[t];
+		float4 f = nb_force(uint32_t(t), p, pos, L, T, knots, lds_tag, S.box, S.cutoff2, ENERGY);
+		if (has_bonded) {
+			const float4 b = force[t];
+			f.x += b.x;
+			f.y += b.y;
+			f.z += b.z;
+			f.w += b.w;
 		}
-		float4 mo = mom[t];
-		pn = baoab_update(p, mo, f, types, S);
-		mom[t] = mo;
+		float4 pn;
+		if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
+			pn = bd_update(p, f, types, S.has_grids ? grids : nullptr, S);
+		} else {
+			if (S.has_grids) {
+				const float4 g = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, grids,
+														  S.ef, S.scheme);
+				f.x += g.x;
+				f.y += g.y;
+				f.z += g.z;
+				f.w += g.w;
+			}
+			float4 mo = mom[t];
+			pn = baoab_update(p, mo, f, types, S);
+			mom[t] = mo;
+		}
+		pos_next[t] = pn;
+		if (ENERGY || store_force)
+			force[t] = f;
 	}
-	pos_next[t] = pn;
-	if (ENERGY || store_force)
-		force[t] = f;
 }
 
 // Permute slot-indexed state into the new sorted order:
@@ -433,9 +468,9 @@ uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 struct PatchImpl::Tables {
 	NbTables nb{};
 	DevArray<float> nb_knots;
-	std::vector<marsmd_table> bond_h, angle_h;
-	DevArray<float> bond_knots, angle_knots;
-	DevArray<marsmd_table> bond_dev, angle_dev;
+	std::vector<marsmd_table> bond_h, angle_h, dihedral_h;
+	DevArray<float> bond_knots, angle_knots, dihedral_knots;
+	DevArray<marsmd_table> bond_dev, angle_dev, dihedral_dev;
 };
 
 PatchImpl::PatchImpl(const marsmd_patch_desc& d) : tables(new Tables) {
@@ -504,6 +539,8 @@ PatchImpl::PatchImpl(const marsmd_patch_desc& d) : tables(new Tables) {
 	};
 	upload_tables(d.num_bond_tables, d.bond_tables, tables->bond_h, tables->bond_knots, tables->bond_dev);
 	upload_tables(d.num_angle_tables, d.angle_tables, tables->angle_h, tables->angle_knots, tables->angle_dev);
+	upload_tables(d.num_dihedral_tables, d.dihedral_tables, tables->dihedral_h, tables->dihedral_knots,
+				  tables->dihedral_dev);
 	// topology
 	num_bonds = d.num_bonds;
 	num_angles = d.num_angles;
@@ -513,6 +550,19 @@ PatchImpl::PatchImpl(const marsmd_patch_desc& d) : tables(new Tables) {
 		topo_max_index = std::max<int64_t>(topo_max_index, d.angles[k]);
 	for (uint64_t k = 0; k < 2 * d.num_exclusions; ++k)
 		topo_max_index = std::max<int64_t>(topo_max_index, d.exclusions[k]);
+	num_dihedrals = d.num_dihedrals;
+	num_restraints = d.num_restraints;
+	if (num_dihedrals && !d.dihedrals)
+		raise(MARSMD_ERR_VALUE, "Patch: num_dihedrals > 0 without dihedral indices");
+	if (num_restraints && (!d.restraint_ids || !d.restraint_anchors || !d.restraint_k))
+		raise(MARSMD_ERR_VALUE, "Patch: num_restraints > 0 without ids, anchors and spring constants");
+	for (uint64_t k = 0; k < 4 * num_dihedrals; ++k) {
+		if (d.dihedrals[k] < 0)
+			raise(MARSMD_ERR_VALUE, "Patch: negative dihedral index");
+		topo_max_index = std::max<int64_t>(topo_max_index, d.dihedrals[k]);
+	}
+	for (uint64_t k = 0; k < num_restraints; ++k)
+		topo_max_index = std::max<int64_t>(topo_max_index, d.restraint_ids[k]);
 	for (uint64_t k = 0; k < 2 * num_bonds; ++k)
 		if (d.bonds[k] < 0)
 			raise(MARSMD_ERR_VALUE, "Patch: negative bond index");
@@ -549,6 +599,28 @@ PatchImpl::PatchImpl(const marsmd_patch_desc& d) : tables(new Tables) {
 				raise(MARSMD_ERR_VALUE, "Patch: angle table index out of range");
 		MMD_HIP(hipMemcpy(angle_table.data(), at.data(), 4 * num_angles, hipMemcpyHostToDevice));
 	}
+	if (num_dihedrals) {
+		dihedrals_canon.resize(4 * num_dihedrals);
+		dihedrals_slot.resize(4 * num_dihedrals);
+		dihedral_table.resize(num_dihedrals);
+		MMD_HIP(hipMemcpy(dihedrals_canon.data(), d.dihedrals, 16 * num_dihedrals, hipMemcpyHostToDevice));
+		std::vector<int32_t> dt(num_dihedrals, 0);
+		if (d.dihedral_table)
+			std::copy(d.dihedral_table, d.dihedral_table + num_dihedrals, dt.begin());
+		for (auto v : dt)
+			if (v < 0 || v >= d.num_dihedral_tables)
+				raise(MARSMD_ERR_VALUE, "Patch: dihedral table index out of range");
+		MMD_HIP(hipMemcpy(dihedral_table.data(), dt.data(), 4 * num_dihedrals, hipMemcpyHostToDevice));
+	}
+	if (num_restraints) {
+		restraint_canon.resize(num_restraints);
+		restraint_slot.resize(num_restraints);
+		restraint_anchor.resize(num_restraints);
+		restraint_k.resize(num_restraints);
+		MMD_HIP(hipMemcpy(restraint_canon.data(), d.restraint_ids, 4 * num_restraints, hipMemcpyHostToDevice));
+		MMD_HIP(hipMemcpy(restraint_anchor.data(), d.restraint_anchors, 16 * num_restraints, hipMemcpyHostToDevice));
+		MMD_HIP(hipMemcpy(restraint_k.data(), d.restraint_k, 4 * num_restraints, hipMemcpyHostToDevice));
+	}
 	num_excl = d.num_exclusions;
 	if (num_excl) {
 		std::vector<int32_t> ex(2 * num_excl);
@@ -581,7 +653,28 @@ PatchImpl::PatchImpl(const marsmd_patch_desc& d) : tables(new Tables) {
 	pl->set_box(box);
 }
 
-PatchImpl::~PatchImpl() { delete tables; }
+PatchImpl::~PatchImpl() {
+	for (hipEvent_t e : timing_events)
+		(void)hipEventDestroy(e);
+	delete tables;
+}
+
+void PatchImpl::set_kernel_timing(bool enable) {
+	timing = enable;
+	timing_used = 0;
+}
+
+void PatchImpl::kernel_timing(double* total_ms, uint64_t* launches) {
+	double sum = 0.0;
+	for (size_t k = 0; k + 1 < timing_used; k += 2) {
+		MMD_HIP(hipEventSynchronize(timing_events[k + 1]));
+		float ms = 0.0f;
+		MMD_HIP(hipEventElapsedTime(&ms, timing_events[k], timing_events[k + 1]));
+		sum += double(ms);
+	}
+	*total_ms = sum;
+	*launches = timing_used / 2;
+}
 
 void PatchImpl::rebuild_exclusions(hipStream_t s) {
 	if (num_excl == 0 || n == 0)
@@ -689,6 +782,16 @@ void PatchImpl::build(float pairlist_cutoff, hipStream_t s) {
 																	  2 * num_bonds, slot_of_canon.data(), n);
 		MMD_LAUNCH_CHECK();
 	}
+	if (num_dihedrals) {
+		k_topo_to_slots<<<grid_for(4 * num_dihedrals, 256), 256, 0, s>>>(
+			dihedrals_canon.data(), dihedrals_slot.data(), 4 * num_dihedrals, slot_of_canon.data(), n);
+		MMD_LAUNCH_CHECK();
+	}
+	if (num_restraints) {
+		k_topo_to_slots<<<grid_for(num_restraints, 256), 256, 0, s>>>(restraint_canon.data(), restraint_slot.data(),
+																	   num_restraints, slot_of_canon.data(), n);
+		MMD_LAUNCH_CHECK();
+	}
 	if (num_angles) {
 		k_topo_to_slots<<<grid_for(4 * num_angles, 256), 256, 0, s>>>(angles_canon.data(), angles_slot.data(),
 																	   4 * num_angles, slot_of_canon.data(), n);
@@ -714,6 +817,29 @@ bool PatchImpl::rebuild_policy(float pairlist_cutoff, float cutoff, uint64_t ste
 
 unsigned PatchImpl::lds_bytes() const { return tables->nb.lds ? unsigned(tables->nb.total_knots) * 4u : 0u; }
 
+// Workgroups for a persistent launch over the 256-slot tiles: as many as are resident
+// at once (occupancy x CUs), a multiple of 8 (one share per XCD), at most the tiles.
+unsigned PatchImpl::persistent_grid(const void* kernel, uint64_t nn) {
+	const uint64_t tiles = (nn + kBlock - 1) / kBlock;
+	const unsigned lds = lds_bytes();
+	auto it = occupancy.find(kernel);
+	if (it == occupancy.end() || it->second.first != lds) {
+		int occ = 0;
+		MMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBlock, lds));
+		it = occupancy.insert_or_assign(kernel, std::make_pair(lds, std::max(occ, 1))).first;
+	}
+	if (num_cus == 0) {
+		int dev = 0;
+		MMD_HIP(hipGetDevice(&dev));
+		MMD_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+		num_cus = std::max(num_cus, 1);
+	}
+	uint64_t g = uint64_t(it->second.second) * uint64_t(num_cus);
+	g = std::min<uint64_t>(g, (tiles + 7) / 8 * 8);
+	g = std::max<uint64_t>((g + 7) / 8 * 8, 8);
+	return unsigned(g);
+}
+
 void PatchImpl::calculate_nonbonded(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period,
 									const float ef[3], int interp, bool energy, hipStream_t s) {
 	if (n == 0)
@@ -724,15 +850,14 @@ void PatchImpl::calculate_nonbonded(float pairlist_cutoff, float cutoff, uint64_
 	interpolation = interp;
 	rebuild_policy(pairlist_cutoff, cutoff, step, period, s);
 	const float cutoff2 = cutoff > 0.0f ? cutoff * cutoff : 0.0f;
-	k_nb_forces<<<grid_for(n, kBlock), kBlock, lds_bytes(), s>>>(pos[cur].data(), n, pl->offsets.data(),
-																 pl->nbr.data(), tables->nb, tables->nb_knots.data(),
-																 make_devbox(box), cutoff2, energy ? 1 : 0,
-																 force.data());
+	k_nb_forces<<<grid_for(n, kBlock), kBlock, lds_bytes(), s>>>(pos[cur].data(), n, pl->list(), tables->nb,
+																 tables->nb_knots.data(), make_devbox(box), cutoff2,
+																 energy ? 1 : 0, force.data());
 	MMD_LAUNCH_CHECK();
 }
 
 void PatchImpl::calculate_bonded(bool energy, hipStream_t s) {
-	if (n == 0 || (num_bonds == 0 && num_angles == 0))
+	if (n == 0 || num_bonded_terms() == 0)
 		return;
 	if (!built)
 		raise(MARSMD_ERR_RUNTIME, "Patch: calculate_bonded_forces before the first nonbonded pass");
@@ -744,6 +869,15 @@ void PatchImpl::calculate_bonded(bool energy, hipStream_t s) {
 		launch_tabulated_angles(angles_slot.data(), reinterpret_cast<const float*>(pos[cur].data()),
 								reinterpret_cast<float*>(force.data()), tables->angle_dev.data(), angle_table.data(),
 								nullptr, box, energy, num_angles, s);
+	if (num_dihedrals)
+		launch_tabulated_dihedrals(dihedrals_slot.data(), reinterpret_cast<const float*>(pos[cur].data()),
+								   reinterpret_cast<float*>(force.data()), tables->dihedral_dev.data(),
+								   dihedral_table.data(), nullptr, box, energy, num_dihedrals, s);
+	if (num_restraints)
+		launch_harmonic_restraints(restraint_slot.data(), reinterpret_cast<const float*>(pos[cur].data()),
+								   reinterpret_cast<float*>(force.data()),
+								   reinterpret_cast<const float*>(restraint_anchor.data()), restraint_k.data(), box,
+								   energy, num_restraints, s);
 }
 
 void PatchImpl::integrate(float dt, float kT, int integrator, uint64_t step, hipStream_t s) {
@@ -817,6 +951,10 @@ void PatchImpl::reorder(hipStream_t s) {
 	if (num_angles) {
 		k_remap_canon(angles_canon.data(), 4 * num_angles, s);
 	}
+	if (num_dihedrals)
+		k_remap_canon(dihedrals_canon.data(), 4 * num_dihedrals, s);
+	if (num_restraints)
+		k_remap_canon(restraint_canon.data(), num_restraints, s);
 	if (num_excl) {
 		k_relabel_pairs<<<grid_for(num_excl, 256), 256, 0, s>>>(excl_pairs.data(), num_excl, relabel.data(), n);
 		MMD_LAUNCH_CHECK();
@@ -839,7 +977,7 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 		return;
 	const int64_t rp = P.reorder_period > 0 ? P.reorder_period : int64_t(P.rebuild_period);
 	const float cutoff2 = P.cutoff > 0.0f ? P.cutoff * P.cutoff : 0.0f;
-	const bool has_bonded = (num_bonds + num_angles) > 0;
+	const bool has_bonded = num_bonded_terms() > 0;
 	const unsigned g = grid_for(n, kBlock);
 	for (uint64_t step = first_step + 1; step <= first_step + num_steps; ++step) {
 		// ParticleReorderManager::maybe_reorder (ReorderManager.cpp:8-25)
@@ -871,26 +1009,27 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 			S.ef[a] = P.electric_field[a];
 		const int store = (step == first_step + num_steps) ? 1 : 0;
 		float4* pnext = pos[1 - cur].data();
-		if (P.integrator == MARSMD_INTEGRATOR_BROWNIAN) {
-			if (energy)
-				k_step<MARSMD_INTEGRATOR_BROWNIAN, true><<<g, kBlock, lds_bytes(), s>>>(
-					pos[cur].data(), pnext, mom.data(), force.data(), n, pl->offsets.data(), pl->nbr.data(), tables->nb,
-					tables->nb_knots.data(), types, grids_dev.data(), S, has_bonded, store);
-			else
-				k_step<MARSMD_INTEGRATOR_BROWNIAN, false><<<g, kBlock, lds_bytes(), s>>>(
-					pos[cur].data(), pnext, mom.data(), force.data(), n, pl->offsets.data(), pl->nbr.data(), tables->nb,
-					tables->nb_knots.data(), types, grids_dev.data(), S, has_bonded, store);
-		} else {
-			if (energy)
-				k_step<MARSMD_INTEGRATOR_LANGEVIN, true><<<g, kBlock, lds_bytes(), s>>>(
-					pos[cur].data(), pnext, mom.data(), force.data(), n, pl->offsets.data(), pl->nbr.data(), tables->nb,
-					tables->nb_knots.data(), types, grids_dev.data(), S, has_bonded, store);
-			else
-				k_step<MARSMD_INTEGRATOR_LANGEVIN, false><<<g, kBlock, lds_bytes(), s>>>(
-					pos[cur].data(), pnext, mom.data(), force.data(), n, pl->offsets.data(), pl->nbr.data(), tables->nb,
-					tables->nb_knots.data(), types, grids_dev.data(), S, has_bonded, store);
-			deferred_kick = true;
+		const bool bd = P.integrator == MARSMD_INTEGRATOR_BROWNIAN;
+		auto kern = bd ? (energy ? k_step<MARSMD_INTEGRATOR_BROWNIAN, true> : k_step<MARSMD_INTEGRATOR_BROWNIAN, false>)
+					   : (energy ? k_step<MARSMD_INTEGRATOR_LANGEVIN, true> : k_step<MARSMD_INTEGRATOR_LANGEVIN, false>);
+		const unsigned grid = persistent_grid(reinterpret_cast<const void*>(kern), n);
+		if (timing) {
+			while (timing_events.size() < timing_used + 2) {
+				hipEvent_t e;
+				MMD_HIP(hipEventCreate(&e));
+				timing_events.push_back(e);
+			}
+			MMD_HIP(hipEventRecord(timing_events[timing_used], s));
 		}
+		kern<<<grid, kBlock, lds_bytes(), s>>>(pos[cur].data(), pnext, mom.data(), force.data(), n, pl->list(),
+											   tables->nb, tables->nb_knots.data(), types, grids_dev.data(), S,
+											   has_bonded, store);
+		if (timing) {
+			MMD_HIP(hipEventRecord(timing_events[timing_used + 1], s));
+			timing_used += 2;
+		}
+		if (!bd)
+			deferred_kick = true;
 		MMD_LAUNCH_CHECK();
 		cur = 1 - cur;
 	}

@@ -1,7 +1,9 @@
 // patch.hpp — host class behind marsmd_patch (see patch.hip).
 #pragma once
 
+#include <map>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "pairlist.hpp"
@@ -19,6 +21,11 @@ struct PatchImpl {
 	int64_t topo_max_index = -1;
 	DevArray<int32_t> bonds_canon, bonds_slot, bond_table;
 	DevArray<int32_t> angles_canon, angles_slot, angle_table; // 4 ints per angle
+	uint64_t num_dihedrals = 0, num_restraints = 0;
+	DevArray<int32_t> dihedrals_canon, dihedrals_slot, dihedral_table; // 4 ints per dihedral
+	DevArray<int32_t> restraint_canon, restraint_slot;
+	DevArray<float4> restraint_anchor;
+	DevArray<float> restraint_k;
 	DevArray<int32_t> excl_pairs;
 	DevArray<uint32_t> excl_offsets, excl_deg, excl_nbrs;
 	DevArray<uint8_t> excl_scan;
@@ -37,6 +44,13 @@ struct PatchImpl {
 	bool built = false, force_rebuild = false, deferred_kick = false, bonded_prepared = false;
 	float efield[3] = {0, 0, 0};
 	int interpolation = MARSMD_INTERP_LINEAR;
+	// persistent launches: kernel -> (dynamic LDS bytes, resident workgroups per CU)
+	std::map<const void*, std::pair<unsigned, int>> occupancy;
+	int num_cus = 0;
+	// launch timing of the fused step kernel (events recorded around each launch)
+	bool timing = false;
+	std::vector<hipEvent_t> timing_events; // pairs: start, stop
+	size_t timing_used = 0;
 
 	explicit PatchImpl(const marsmd_patch_desc& d);
 	~PatchImpl();
@@ -52,13 +66,17 @@ struct PatchImpl {
 	void finish_deferred_kick(float dt, hipStream_t s);
 	void reorder(hipStream_t s);
 	void run(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s);
+	void set_kernel_timing(bool enable);
+	void kernel_timing(double* total_ms, uint64_t* launches);
 
   private:
 	void build(float pairlist_cutoff, hipStream_t s);
 	bool rebuild_policy(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period, hipStream_t s);
 	void rebuild_exclusions(hipStream_t s);
 	void k_remap_canon(int32_t* idx, uint64_t count, hipStream_t s);
+	uint64_t num_bonded_terms() const { return num_bonds + num_angles + num_dihedrals + num_restraints; }
 	unsigned lds_bytes() const;
+	unsigned persistent_grid(const void* kernel, uint64_t n);
 };
 
 } // namespace mmd

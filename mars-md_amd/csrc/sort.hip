@@ -5,7 +5,7 @@
 // (src/PatchOperation/ZOrderKernels/CUDASort.cu:13-98; stability contract pinned by
 // Unit_test/System/zorder_tests.cpp:139-151). One pass per 8-bit digit:
 //   1. k_rs_hist     per-tile digit histogram (wave-aggregated LDS counts),
-//   2. k_rs_scan     one-block exclusive scan over [digit][tile] (digit-major),
+//   2. exclusive scan over [digit][tile] (digit-major; reduce / spine / apply),
 //   3. k_rs_scatter  per-tile stable ranks from wave64 ballot peer masks, then
 //                    scatter to offset[digit][tile] + rank.
 // A tile is 4 waves x 16 rows x 64 lanes = 4096 keys; wave w owns the contiguous
@@ -182,9 +182,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __re
 		block_sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t* __restrict__ in, uint64_t n,
+// `in` may alias `out` (in-place scan): each thread reads its items before writing them.
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t* in, uint64_t n,
 														   const uint32_t* __restrict__ block_offsets,
-														   uint32_t* __restrict__ out) {
+														   uint32_t* out) {
 	__shared__ uint32_t wsum[5];
 	const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
 	uint32_t v[kScanItems];
@@ -220,7 +221,7 @@ inline uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
 
 uint64_t radix_sort_scratch_bytes(uint64_t n) {
 	const uint64_t tiles = (n + kRsTile - 1) / kRsTile;
-	return align256(n * 4) * 2 + align256(tiles * kRsBuckets * 4 + 4);
+	return align256(n * 4) * 2 + align256(tiles * kRsBuckets * 4 + 4) + scan_scratch_bytes(tiles * kRsBuckets);
 }
 
 void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
@@ -239,6 +240,7 @@ void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
 	uint32_t* tmp_k = reinterpret_cast<uint32_t*>(sp);
 	uint32_t* tmp_v = reinterpret_cast<uint32_t*>(sp + align256(n * 4));
 	uint32_t* hist = reinterpret_cast<uint32_t*>(sp + 2 * align256(n * 4));
+	void* scan_tmp = sp + 2 * align256(n * 4) + align256(uint64_t(tiles) * kRsBuckets * 4 + 4);
 	const int passes = (key_bits + 7) / 8;
 	const uint32_t* src_k = keys_in;
 	const uint32_t* src_v = vals_in;
@@ -249,8 +251,7 @@ void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
 		const int shift = 8 * p;
 		k_rs_hist<<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, hist, tiles);
 		MMD_LAUNCH_CHECK();
-		k_scan_one_block<<<1, 1024, 0, s>>>(hist, uint64_t(tiles) * kRsBuckets, nullptr);
-		MMD_LAUNCH_CHECK();
+		exclusive_scan_u32(hist, hist, uint64_t(tiles) * kRsBuckets, scan_tmp, s);
 		k_rs_scatter<<<tiles, kRsBlock, 0, s>>>(src_k, src_v, dk, dv, n, shift, hist, tiles);
 		MMD_LAUNCH_CHECK();
 		src_k = dk;

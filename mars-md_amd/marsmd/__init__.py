@@ -24,6 +24,8 @@ from .core import (
     launch_pairwise_nonbonded,
     launch_resolve_pair_tables,
     launch_tabulated_angles,
+    launch_tabulated_dihedrals,
+    launch_harmonic_restraints,
     launch_tabulated_bonds,
     make_pair_tag,
     make_types,
@@ -38,5 +40,6 @@ __all__ = [
     "SystemSpec", "TabulatedPotential", "ZOrderPairlist", "ZOrderSort", "create_pairlist", "device_grid_views",
     "exclusion_csr_from_pairs", "launch_BAOAB", "launch_BAOAB_LastUpdate", "launch_BD", "launch_PMF",
     "launch_pairwise_nonbonded", "launch_resolve_pair_tables", "launch_tabulated_angles", "launch_tabulated_bonds",
+    "launch_tabulated_dihedrals", "launch_harmonic_restraints",
     "lib", "make_pair_tag", "make_types", "particle_view", "philox_draw",
 ]

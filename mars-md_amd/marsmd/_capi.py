@@ -136,6 +136,15 @@ class PatchDesc(C.Structure):
         ("exclusions", C.c_void_p),
         ("num_grids", C.c_int32),
         ("grids", C.c_void_p),
+        ("num_dihedrals", C.c_uint64),
+        ("dihedrals", C.c_void_p),
+        ("dihedral_table", C.c_void_p),
+        ("num_dihedral_tables", C.c_int32),
+        ("dihedral_tables", C.c_void_p),
+        ("num_restraints", C.c_uint64),
+        ("restraint_ids", C.c_void_p),
+        ("restraint_anchors", C.c_void_p),
+        ("restraint_k", C.c_void_p),
     ]
 
 
@@ -191,6 +200,8 @@ PROTOTYPES = {
                                                 _U32, _P]),
     "marsmd_launch_tabulated_bonds": (_I32, [_P, _P, _P, _P, _P, _P, C.POINTER(Box), _I32, _U64, _P]),
     "marsmd_launch_tabulated_angles": (_I32, [_P, _P, _P, _P, _P, _P, C.POINTER(Box), _I32, _U64, _P]),
+    "marsmd_launch_tabulated_dihedrals": (_I32, [_P, _P, _P, _P, _P, _P, C.POINTER(Box), _I32, _U64, _P]),
+    "marsmd_launch_harmonic_restraints": (_I32, [_P, _P, _P, _P, _P, C.POINTER(Box), _I32, _U64, _P]),
     "marsmd_launch_bd": (_I32, [C.POINTER(ParticleView), C.POINTER(ParticleTypes), _F, _U64, _F, _U64, C.POINTER(Box),
                                 _U64, _U32, _P,
                                 _FP3, _I32, _P]),
@@ -211,6 +222,8 @@ PROTOTYPES = {
     "marsmd_patch_reorder_particles": (_I32, [_P, _P]),
     "marsmd_patch_get_pairlist": (_I32, [_P, _PP]),
     "marsmd_patch_run": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, _P]),
+    "marsmd_patch_set_kernel_timing": (_I32, [_P, _I32]),
+    "marsmd_patch_get_kernel_timing": (_I32, [_P, C.POINTER(C.c_double), C.POINTER(_U64)]),
 }
 
 LIB_PATH = Path(__file__).resolve().parent / "libmarsmd_hip.so"

@@ -70,6 +70,18 @@ def cosine_angle_table(k=2.0, knots=256) -> TabulatedPotential:
     return TabulatedPotential(u.astype(np.float32), float(start), float(step), is_periodic=True)
 
 
+def cosine_dihedral_table(k=1.0, n=3, delta=0.0, knots=361) -> TabulatedPotential:
+    """U(phi) = k (1 + cos(n phi - delta)) on [-180, 180] degrees, the abscissa converted
+    to radians as Table::convert_angular_abscissa_to_radians (Tables.h:109-119);
+    periodic, as dihedral tables are (DeviceBondedInteraction.h:245-255)."""
+    step_deg = np.float32(360.0 / (knots - 1))
+    ph = -180.0 + np.arange(knots) * float(step_deg)
+    u = k * (1.0 + np.cos(n * np.deg2rad(ph) - delta))
+    start = np.float32(-180.0) * DEG_TO_RAD
+    step = np.float32(step_deg * DEG_TO_RAD)
+    return TabulatedPotential(u.astype(np.float32), float(start), float(step), is_periodic=True)
+
+
 def uniform_positions(n: int, L, seed: int, min_sep: float = 3.0, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
     """Uniform positions with rejection of candidates within min_sep of an earlier
     accepted bead (minimum image). Deterministic for a seed."""
@@ -136,12 +148,15 @@ def config0(n: int = 10_000, L: float = 250.0, seed: int = 42, steps: int = 1000
     return spec
 
 
-def config1(chains: int = 200, beads: int = 500, L: float = 800.0, seed: int = 42, steps: int = 1000) -> SystemSpec:
+def config1(chains: int = 200, beads: int = 500, L: float = 800.0, seed: int = 42, steps: int = 1000,
+            dihedrals: bool = False, restraints: bool = False) -> SystemSpec:
     """Bead-spring polymers: random-walk chains (step r0 = 5 A, a bead is redrawn when it
     lands within 3 A of the bead two back), two types in blocks of 50 beads, harmonic
     bond table U = k/2 (r - r0)^2 (512 knots on [0, 20]), cosine angle table k(1 + cos)
     (256 knots, degrees), WCA+DH for 3 type pairs with charges (+0.5, -0.5), 1-2 and 1-3
-    exclusions, BD, reorder period 20."""
+    exclusions, BD, reorder period 20. Optionally dihedrals along every chain (k = 1,
+    n = 3 cosine table, 361 knots) and harmonic restraints (k = 0.5) pinning every 50th
+    bead to its initial position."""
     n = chains * beads
     r0 = 5.0
     starts = rng.uniform(seed, rng.STREAM_CHAINS, chains, draw=0)[:, :3].astype(np.float64) * L
@@ -189,6 +204,17 @@ def config1(chains: int = 200, beads: int = 500, L: float = 800.0, seed: int = 4
         seed=seed,
         name=f"bd-polymers-{chains}x{beads}-L{L:g}",
     )
+    if dihedrals:
+        spec.dihedrals = np.stack([idx[:, :-3].ravel(), idx[:, 1:-2].ravel(), idx[:, 2:-1].ravel(),
+                                   idx[:, 3:].ravel()], axis=1).astype(np.int32)
+        spec.dihedral_table = np.zeros(len(spec.dihedrals), np.int32)
+        spec.dihedral_tables = [cosine_dihedral_table()]
+        spec.name += "-dihedrals"
+    if restraints:
+        spec.restraint_ids = np.arange(0, n, 50, dtype=np.int32)
+        spec.restraint_anchors = pos[spec.restraint_ids].astype(np.float32)
+        spec.restraint_k = np.full(len(spec.restraint_ids), 0.5, np.float32)
+        spec.name += "-restraints"
     spec.steps = steps
     return spec
 

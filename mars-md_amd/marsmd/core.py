@@ -425,6 +425,22 @@ def launch_tabulated_angles(angles4, pos, force_energy, tables: DeviceTables, ta
              _ptr(table_indices), _ptr(forms), C.byref(box.to_c()), 1 if get_energy else 0, num_angles, _stream(stream))
 
 
+def launch_tabulated_dihedrals(dihedrals4, pos, force_energy, tables: DeviceTables, table_indices, forms,
+                               box: PeriodicBox, get_energy: bool, num_dihedrals: int, stream=None):
+    """launch_tabulated_dihedrals (DeviceBondedInteraction.h:548-573); dihedrals4 is an
+    int32 tensor of {i, j, k, l} records, the tables periodic in radians."""
+    api.call("marsmd_launch_tabulated_dihedrals", _ptr(dihedrals4), _ptr(pos), _ptr(force_energy), tables.ptr,
+             _ptr(table_indices), _ptr(forms), C.byref(box.to_c()), 1 if get_energy else 0, num_dihedrals,
+             _stream(stream))
+
+
+def launch_harmonic_restraints(particle_ids, pos, force_energy, anchors4, spring_constants, box: PeriodicBox,
+                               get_energy: bool, num_restraints: int, stream=None):
+    """launch_harmonic_restraints (DeviceBondedInteraction.h:595-620): anchors4 float4 records."""
+    api.call("marsmd_launch_harmonic_restraints", _ptr(particle_ids), _ptr(pos), _ptr(force_energy), _ptr(anchors4),
+             _ptr(spring_constants), C.byref(box.to_c()), 1 if get_energy else 0, num_restraints, _stream(stream))
+
+
 def _ef(e):
     return (C.c_float * 3)(*(map(float, e) if e is not None else (0.0, 0.0, 0.0)))
 

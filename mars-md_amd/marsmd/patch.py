@@ -83,6 +83,16 @@ class Patch:
         p = (params or self.spec.params).to_c()
         api.call("marsmd_patch_run", self._h, C.byref(p), first_step, num_steps, _stream(stream))
 
+    def set_kernel_timing(self, enable: bool = True):
+        """Record HIP events around each fused step kernel launched by run()."""
+        api.call("marsmd_patch_set_kernel_timing", self._h, 1 if enable else 0)
+
+    def kernel_timing(self):
+        """(summed duration in ms, launches) of the fused step kernels timed so far."""
+        ms, k = C.c_double(0.0), C.c_uint64(0)
+        api.call("marsmd_patch_get_kernel_timing", self._h, C.byref(ms), C.byref(k))
+        return ms.value, k.value
+
     def maybe_reorder(self, step: int, params: Optional[RunParams] = None, stream=None) -> bool:
         """ParticleReorderManager::maybe_reorder (ReorderManager.cpp:8-25) with the
         reorderPeriod fallback of SimManager::execute_force_calculation (:343-349)."""

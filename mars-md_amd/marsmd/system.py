@@ -68,6 +68,12 @@ class SystemSpec:
     angle_tables: List[TabulatedPotential] = field(default_factory=list)
     exclusions: Optional[np.ndarray] = None  # (E, 2)
     grids: List[Grid] = field(default_factory=list)
+    dihedrals: Optional[np.ndarray] = None  # (D, 4)
+    dihedral_table: Optional[np.ndarray] = None
+    dihedral_tables: List[TabulatedPotential] = field(default_factory=list)  # periodic, radians
+    restraint_ids: Optional[np.ndarray] = None  # (R,) particle per harmonic restraint
+    restraint_anchors: Optional[np.ndarray] = None  # (R, 3)
+    restraint_k: Optional[np.ndarray] = None  # (R,)
     seed: int = 42
     patch_id: int = 0
     name: str = ""
@@ -145,6 +151,21 @@ class SystemSpec:
         if self.exclusions is not None and len(self.exclusions):
             d.num_exclusions = len(self.exclusions)
             d.exclusions = arr(self.exclusions, np.int32)
+        if self.dihedrals is not None and len(self.dihedrals):
+            d.num_dihedrals = len(self.dihedrals)
+            d.dihedrals = arr(self.dihedrals, np.int32)
+            dt = self.dihedral_table if self.dihedral_table is not None else np.zeros(len(self.dihedrals))
+            d.dihedral_table = arr(dt, np.int32)
+        d.num_dihedral_tables = len(self.dihedral_tables)
+        d.dihedral_tables = tables(self.dihedral_tables)
+        if self.restraint_ids is not None and len(self.restraint_ids):
+            r = len(self.restraint_ids)
+            d.num_restraints = r
+            d.restraint_ids = arr(self.restraint_ids, np.int32)
+            anchors = np.zeros((r, 4), np.float32)
+            anchors[:, :3] = np.asarray(self.restraint_anchors, np.float32)[:, :3]
+            d.restraint_anchors = arr(anchors, np.float32)
+            d.restraint_k = arr(self.restraint_k, np.float32)
         if self.grids:
             recs = (api.GridView * len(self.grids))()
             for i, g in enumerate(self.grids):

@@ -352,6 +352,130 @@ inline AngleTerm<T> angle_term(const Box& box, const V4<T>& pa, const V4<T>& pb,
 	return out;
 }
 
+// FP32 two-argument arctangent shared with the device kernels (device_common.hpp
+// atan2_series): t = min(|x|,|y|)/max(|x|,|y|); for t > tan(pi/12),
+// atan(t) = pi/6 + atan((sqrt(3) t - 1)/(t + sqrt(3))); then the Maclaurin series of
+// atan to the t^13 term; octant/quadrant from |y| > |x| and the signs (signed zeros as
+// IEEE atan2).
+inline float atan2_series(float y, float x) {
+	const float ax = std::fabs(x), ay = std::fabs(y);
+	const float mx = ax > ay ? ax : ay;
+	const float mn = ax > ay ? ay : ax;
+	float t = mx == 0.0f ? 0.0f : mn / mx;
+	float base = 0.0f;
+	if (t > 0.267949194f) {
+		t = (t * 1.73205081f - 1.0f) / (t + 1.73205081f);
+		base = 0.523598776f;
+	}
+	const float z = t * t;
+	float p = 7.69230769e-02f;
+	p = p * z - 9.09090909e-02f;
+	p = p * z + 1.11111111e-01f;
+	p = p * z - 1.42857143e-01f;
+	p = p * z + 2.00000000e-01f;
+	p = p * z - 3.33333333e-01f;
+	float r = base + (t + t * z * p);
+	if (ay > ax)
+		r = 1.57079637f - r;
+	const bool xneg = x < 0.0f || (x == 0.0f && 1.0f / x < 0.0f);
+	if (xneg)
+		r = 3.14159274f - r;
+	const bool yneg = y < 0.0f || (y == 0.0f && 1.0f / y < 0.0f);
+	return yneg ? -r : r;
+}
+inline float atan2_t(float y, float x) { return atan2_series(y, x); }
+inline double atan2_t(double y, double x) { return std::atan2(y, x); }
+
+template<class T>
+inline V4<T> cross3(const V4<T>& v, const V4<T>& w) {
+	return V4<T>{v.y * w.z - v.z * w.y, v.z * w.x - v.x * w.z, v.x * w.y - v.y * w.x, T(0)};
+}
+template<class T>
+inline T dot3(const V4<T>& v, const V4<T>& w) {
+	return v.x * w.x + v.y * w.y + v.z * w.z;
+}
+// Vector3::rLength2 (src/Types/Vector3.h:233-236).
+template<class T>
+inline T rlen2(const V4<T>& v) {
+	const T l2 = length2<T>(v);
+	return l2 != T(0) ? T(1) / l2 : T(0);
+}
+
+// DihedralGeometry (src/Interactions/Bonded/BondGeometry.h:56-87) and
+// TabulatedDihedralComputer (BondComputer.h:296-343): f[0..3] on i, j, k, l and U/4.
+template<class T>
+struct DihedralTerm {
+	V4<T> f[4];
+	T quarter_energy{0};
+	T phi{0};
+};
+template<class T>
+inline DihedralTerm<T> dihedral_term(const Box& box, const V4<T>& p0, const V4<T>& p1, const V4<T>& p2,
+									 const V4<T>& p3, const Table<T>& tab) {
+	const V4<T> ab = wrap_diff<T>(box, sub<T>(p1, p0));
+	const V4<T> bc = wrap_diff<T>(box, sub<T>(p2, p1));
+	const V4<T> cd = wrap_diff<T>(box, sub<T>(p3, p2));
+	const V4<T> c_abc = cross3<T>(ab, bc);
+	const V4<T> c_bcd = cross3<T>(bc, cd);
+	const V4<T> c_x = cross3<T>(bc, c_abc);
+	const T cos_phi = dot3<T>(c_abc, c_bcd) / (std::sqrt(length2<T>(c_abc)) * std::sqrt(length2<T>(c_bcd)));
+	const T sin_phi = dot3<T>(c_x, c_bcd) / (std::sqrt(length2<T>(c_x)) * std::sqrt(length2<T>(c_bcd)));
+	const T phi = atan2_t(sin_phi, cos_phi);
+	const T bcl = std::sqrt(length2<T>(bc));
+	const T r_abc = rlen2<T>(c_abc), r_bcd = rlen2<T>(c_bcd), r_bc = rlen2<T>(bc);
+	const T s1 = -bcl * r_abc;
+	const T s3 = -bcl * r_bcd;
+	const V4<T> f1{s1 * c_abc.x, s1 * c_abc.y, s1 * c_abc.z, T(0)};
+	const V4<T> f3{s3 * c_bcd.x, s3 * c_bcd.y, s3 * c_bcd.z, T(0)};
+	const T a2 = -(dot3<T>(ab, bc) * r_bc);
+	const T b2 = dot3<T>(bc, cd) * r_bc;
+	const V4<T> f2{a2 * f1.x - b2 * f3.x, a2 * f1.y - b2 * f3.y, a2 * f1.z - b2 * f3.z, T(0)};
+	const bool degenerate =
+		(length2<T>(ab) * length2<T>(bc) * r_abc > T(100)) || (length2<T>(bc) * length2<T>(cd) * r_bcd > T(100));
+	const FE<T> fe = table_compute<T>(phi, tab);
+	T mag = degenerate ? T(0) : fe.f;
+	if (mag > T(1000))
+		mag = T(1000);
+	else if (mag < T(-1000))
+		mag = T(-1000);
+	const V4<T> g1{f1.x * mag, f1.y * mag, f1.z * mag, T(0)};
+	const V4<T> g2{f2.x * mag, f2.y * mag, f2.z * mag, T(0)};
+	const V4<T> g3{f3.x * mag, f3.y * mag, f3.z * mag, T(0)};
+	DihedralTerm<T> out;
+	out.f[0] = g1;
+	out.f[1] = V4<T>{g2.x - g1.x, g2.y - g1.y, g2.z - g1.z, T(0)};
+	out.f[2] = V4<T>{g3.x - g2.x, g3.y - g2.y, g3.z - g2.z, T(0)};
+	out.f[3] = V4<T>{-g3.x, -g3.y, -g3.z, T(0)};
+	out.quarter_energy = fe.e * T(0.25);
+	out.phi = phi;
+	return out;
+}
+
+// HarmonicRestraintComputer (BondComputer.h:109-129): CalcDistance from the anchor
+// (Interactions.h:42-53) and AnalyticalForceComputer<0> (Bonded/Analytical.h:18-30)
+// at r0 = 0. Full U on the particle.
+template<class T>
+struct RestraintTerm {
+	V4<T> f;
+	T energy{0};
+};
+template<class T>
+inline RestraintTerm<T> restraint_term(const Box& box, const V4<T>& p, const V4<T>& anchor, T k) {
+	const V4<T> r = wrap_diff<T>(box, sub<T>(p, anchor));
+	const T d = std::sqrt(length2<T>(r));
+	V4<T> u{};
+	if (d > T(1e-6)) {
+		const T inv = T(1) / d;
+		u = V4<T>{r.x * inv, r.y * inv, r.z * inv, T(0)};
+	}
+	const T dr = d - T(0);
+	const T force = -k * dr;
+	RestraintTerm<T> out;
+	out.f = V4<T>{u.x * force, u.y * force, u.z * force, T(0)};
+	out.energy = T(0.5) * k * dr * dr;
+	return out;
+}
+
 // ---------------------------------------------------------------------- grids
 // BaseGridView sampling (src/Types/BaseGridDevice.h:218-475).
 template<class T>

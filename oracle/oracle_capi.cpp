@@ -368,6 +368,12 @@ struct Sim {
 	std::vector<int32_t> bond_table;
 	std::vector<std::array<int, 3>> angles;
 	std::vector<int32_t> angle_table;
+	std::vector<Table<float>> dihedral_tables;
+	std::vector<std::array<int, 4>> dihedrals;
+	std::vector<int32_t> dihedral_table;
+	std::vector<int32_t> restraint_ids;
+	std::vector<V4f> restraint_anchors;
+	std::vector<float> restraint_k;
 	std::vector<std::pair<int, int>> excl_pairs;
 	std::vector<std::vector<float>> grid_data;
 	std::vector<Grid<float>> grids;
@@ -454,6 +460,8 @@ void sim_pair_forces(Sim& s, float cutoff, bool compute_energy) {
 	}
 }
 
+void sim_dihedral_restraint_forces(Sim& s, bool compute_energy);
+
 void sim_bonded_forces(Sim& s, bool compute_energy) {
 	for (size_t b = 0; b < s.bonds.size(); ++b) {
 		const int x = s.bonds[b].first, y = s.bonds[b].second;
@@ -497,6 +505,47 @@ void sim_bonded_forces(Sim& s, bool compute_energy) {
 				s.force[ids[q]].t += at.third_energy;
 				s.force64[ids[q]].t += at.third_energy;
 			}
+		}
+	}
+	sim_dihedral_restraint_forces(s, compute_energy);
+}
+
+// Dihedrals and harmonic restraints of Patch::calculate_bonded_forces (Patch.cpp:208-236).
+void sim_dihedral_restraint_forces(Sim& s, bool compute_energy) {
+	for (size_t d = 0; d < s.dihedrals.size(); ++d) {
+		const auto& q = s.dihedrals[d];
+		const DihedralTerm<float> dt = dihedral_term<float>(s.box, s.pos[size_t(q[0])], s.pos[size_t(q[1])],
+															 s.pos[size_t(q[2])], s.pos[size_t(q[3])],
+															 s.dihedral_tables[size_t(s.dihedral_table[d])]);
+		for (int k = 0; k < 4; ++k) {
+			const size_t id = size_t(q[size_t(k)]);
+			s.force[id].x += dt.f[k].x;
+			s.force[id].y += dt.f[k].y;
+			s.force[id].z += dt.f[k].z;
+			s.force64[id].x += dt.f[k].x;
+			s.force64[id].y += dt.f[k].y;
+			s.force64[id].z += dt.f[k].z;
+			if (compute_energy) {
+				s.force[id].t += dt.quarter_energy;
+				s.force64[id].t += dt.quarter_energy;
+			}
+		}
+	}
+	for (size_t r = 0; r < s.restraint_ids.size(); ++r) {
+		const int id = s.restraint_ids[r];
+		if (id < 0)
+			continue;
+		const RestraintTerm<float> rt = restraint_term<float>(s.box, s.pos[size_t(id)], s.restraint_anchors[r],
+															   s.restraint_k[r]);
+		s.force[size_t(id)].x += rt.f.x;
+		s.force[size_t(id)].y += rt.f.y;
+		s.force[size_t(id)].z += rt.f.z;
+		s.force64[size_t(id)].x += rt.f.x;
+		s.force64[size_t(id)].y += rt.f.y;
+		s.force64[size_t(id)].z += rt.f.z;
+		if (compute_energy) {
+			s.force[size_t(id)].t += rt.energy;
+			s.force64[size_t(id)].t += rt.energy;
 		}
 	}
 }
@@ -571,6 +620,12 @@ void sim_reorder(Sim& s) {
 	for (auto& a : s.angles)
 		for (int q = 0; q < 3; ++q)
 			a[q] = int(inv[size_t(a[q])]);
+	for (auto& d : s.dihedrals)
+		for (int q = 0; q < 4; ++q)
+			d[size_t(q)] = int(inv[size_t(d[size_t(q)])]);
+	for (auto& r : s.restraint_ids)
+		if (r >= 0)
+			r = int(inv[size_t(r)]);
 	for (auto& e : s.excl_pairs) {
 		e.first = int(inv[size_t(e.first)]);
 		e.second = int(inv[size_t(e.second)]);
@@ -1071,6 +1126,19 @@ void* orc_sim_create(const marsmd_patch_desc* d) {
 		}
 		for (uint64_t k = 0; k < d->num_exclusions; ++k)
 			s->excl_pairs.emplace_back(d->exclusions[2 * k], d->exclusions[2 * k + 1]);
+		for (int k = 0; k < d->num_dihedral_tables; ++k)
+			s->dihedral_tables.push_back(make_table<float>(d->dihedral_tables[k]));
+		for (uint64_t k = 0; k < d->num_dihedrals; ++k) {
+			s->dihedrals.push_back({d->dihedrals[4 * k], d->dihedrals[4 * k + 1], d->dihedrals[4 * k + 2],
+									d->dihedrals[4 * k + 3]});
+			s->dihedral_table.push_back(d->dihedral_table ? d->dihedral_table[k] : 0);
+		}
+		for (uint64_t k = 0; k < d->num_restraints; ++k) {
+			s->restraint_ids.push_back(d->restraint_ids[k]);
+			s->restraint_anchors.push_back(V4f{d->restraint_anchors[4 * k], d->restraint_anchors[4 * k + 1],
+											   d->restraint_anchors[4 * k + 2], 0.0f});
+			s->restraint_k.push_back(d->restraint_k[k]);
+		}
 		for (int k = 0; k < d->num_grids; ++k) {
 			const marsmd_grid_view& g = d->grids[k];
 			const size_t cnt = size_t(g.dims[0] * g.dims[1] * g.dims[2]);
@@ -1232,6 +1300,61 @@ int orc_sim_get_topology(void* h, int32_t* bonds, int32_t* angles, int32_t* excl
 extern "C" {
 // The shared FP32 arccosine of the angle term.
 float orc_acos(float x) { return acos_series(x); }
+// The shared FP32 arctangent of the dihedral term.
+float orc_atan2(float y, float x) { return atan2_series(y, x); }
+
+// Dihedrals (4 ints each) and harmonic restraints on explicit arrays, as
+// launch_tabulated_dihedrals / launch_harmonic_restraints. mode 1: FP32 terms summed
+// in FP64; mode 2: FP64 terms (libm atan2). out: n x 4 doubles (force, energy) and,
+// when phi_out is not null, each dihedral's angle.
+int orc_dihedral_restraint_forces(const int32_t* dihedrals, const int32_t* dihedral_table, uint64_t num_dihedrals,
+								  const marsmd_table* dtabs, const int32_t* restraint_ids, const float* anchors4,
+								  const float* ks, uint64_t num_restraints, const float* pos4, const marsmd_box* box,
+								  int32_t mode, uint64_t n, double* out, double* phi_out) {
+	const Box b = make_box(*box);
+	const V4f* p = as_v4(pos4);
+	std::fill(out, out + 4 * n, 0.0);
+	auto add = [&](int i, double fx, double fy, double fz, double e) {
+		out[4 * i + 0] += fx;
+		out[4 * i + 1] += fy;
+		out[4 * i + 2] += fz;
+		out[4 * i + 3] += e;
+	};
+	auto v64 = [&](int i) { return V4<double>{p[i].x, p[i].y, p[i].z, 0.0}; };
+	for (uint64_t k = 0; k < num_dihedrals; ++k) {
+		const int q[4] = {dihedrals[4 * k], dihedrals[4 * k + 1], dihedrals[4 * k + 2], dihedrals[4 * k + 3]};
+		const marsmd_table& t = dtabs[dihedral_table ? dihedral_table[k] : 0];
+		if (mode == 2) {
+			const DihedralTerm<double> dt =
+				dihedral_term<double>(b, v64(q[0]), v64(q[1]), v64(q[2]), v64(q[3]), make_table<double>(t));
+			for (int a = 0; a < 4; ++a)
+				add(q[a], dt.f[a].x, dt.f[a].y, dt.f[a].z, dt.quarter_energy);
+			if (phi_out)
+				phi_out[k] = dt.phi;
+			continue;
+		}
+		const DihedralTerm<float> dt = dihedral_term<float>(b, p[q[0]], p[q[1]], p[q[2]], p[q[3]], make_table<float>(t));
+		for (int a = 0; a < 4; ++a)
+			add(q[a], dt.f[a].x, dt.f[a].y, dt.f[a].z, dt.quarter_energy);
+		if (phi_out)
+			phi_out[k] = dt.phi;
+	}
+	for (uint64_t k = 0; k < num_restraints; ++k) {
+		const int id = restraint_ids[k];
+		if (id < 0)
+			continue;
+		const V4f anc{anchors4[4 * k], anchors4[4 * k + 1], anchors4[4 * k + 2], 0.0f};
+		if (mode == 2) {
+			const RestraintTerm<double> rt = restraint_term<double>(
+				b, v64(id), V4<double>{anc.x, anc.y, anc.z, 0.0}, double(ks[k]));
+			add(id, rt.f.x, rt.f.y, rt.f.z, rt.energy);
+			continue;
+		}
+		const RestraintTerm<float> rt = restraint_term<float>(b, p[id], anc, ks[k]);
+		add(id, rt.f.x, rt.f.y, rt.f.z, rt.energy);
+	}
+	return 0;
+}
 // Stash the field and interpolation order the integrator applies (the state
 // calculate_nonbonded_forces records, Patch.cpp:71-73).
 int orc_sim_set_field(void* h, const float ef[3], int32_t interp) {

@@ -58,6 +58,9 @@ def lib():
             "orc_sim_set_state": (I32, [P, P, P, P, P, P, U64, I32]),
             "orc_sim_set_field": (I32, [P, P, I32]),
             "orc_acos": (F, [F]),
+            "orc_atan2": (F, [F, F]),
+            "orc_dihedral_restraint_forces": (I32, [P, P, U64, P, P, P, P, U64, P, C.POINTER(api.Box), I32, U64, P,
+                                                    P]),
             "orc_sim_calculate_nonbonded_forces": (I32, [P, F, F, U64, U64, P, I32, I32]),
             "orc_sim_calculate_bonded_forces": (I32, [P, I32]),
             "orc_sim_integrate_motion": (I32, [P, F, F, I32, U64]),
@@ -274,6 +277,34 @@ def bonded_forces(pos4, box, bonds=None, bond_table=None, bond_tables=(), angles
                                    C.addressof(arec), len(angle_tables), C.byref(b), mode, len(pos4),
                                    out.ctypes.data))
     return out
+
+
+def atan2_32(y: float, x: float) -> float:
+    """The FP32 arctangent shared by the dihedral kernel and the oracle."""
+    return lib().orc_atan2(y, x)
+
+
+def dihedral_restraint_forces(pos4, box, dihedrals=None, dihedral_table=None, dihedral_tables=(),
+                              restraint_ids=None, restraint_anchors=None, restraint_k=None, mode=1):
+    """launch_tabulated_dihedrals + launch_harmonic_restraints on explicit arrays.
+    Returns (n x 4 float64 force/energy, per-dihedral angle)."""
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    dih = np.zeros((0, 4), np.int32) if dihedrals is None else np.ascontiguousarray(dihedrals, np.int32)
+    dt = np.zeros(len(dih), np.int32) if dihedral_table is None else np.ascontiguousarray(dihedral_table, np.int32)
+    rid = np.zeros(0, np.int32) if restraint_ids is None else np.ascontiguousarray(restraint_ids, np.int32)
+    anc = np.zeros((len(rid), 4), np.float32)
+    if len(rid):
+        anc[:, :3] = np.asarray(restraint_anchors, np.float32)[:, :3]
+    kk = np.zeros(len(rid), np.float32) if restraint_k is None else np.ascontiguousarray(restraint_k, np.float32)
+    drec, keep = table_records(list(dihedral_tables))
+    out = np.zeros((len(pos4), 4), np.float64)
+    phi = np.zeros(max(len(dih), 1), np.float64)
+    b = box_c(box)
+    _check(lib().orc_dihedral_restraint_forces(dih.ctypes.data, dt.ctypes.data, len(dih), C.addressof(drec),
+                                               rid.ctypes.data, anc.ctypes.data, kk.ctypes.data, len(rid),
+                                               pos4.ctypes.data, C.byref(b), mode, len(pos4), out.ctypes.data,
+                                               phi.ctypes.data))
+    return out, phi[:len(dih)]
 
 
 def grid_sample(grid, pos4, scheme, bc=None, fp64=False):

@@ -73,3 +73,22 @@ def test_struct_layouts_match_c(tmp_path):
         assert int(got[cname]) == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, f"{cname}.{fname}"
+
+
+def test_cpp_interface_compiles(tmp_path):
+    """mars-md_amd/host/MARS.hpp (the reference's MARS:: classes over the C ABI)
+    compiles with the host compiler alone, and its enums keep the reference's values
+    (IntegratorType SimParam.h:160, InterpolationOrder GridTerm.h:36)."""
+    src = tmp_path / "t.cpp"
+    src.write_text(
+        '#include "MARS.hpp"\n'
+        "static_assert(int(MARS::IntegratorType::Langevin) == 0, \"\");\n"
+        "static_assert(int(MARS::IntegratorType::Brownian) == 1, \"\");\n"
+        "static_assert(int(MARS::InterpolationOrder::Linear) == MARSMD_INTERP_LINEAR, \"\");\n"
+        "static_assert(int(MARS::InterpolationOrder::Cubic) == MARSMD_INTERP_CUBIC, \"\");\n"
+        "static_assert(int(MARS::ExceptionType::RuntimeError) == 9, \"\");\n"
+        "int main() { return 0; }\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", f"-I{ROOT / 'mars-md_amd' / 'host'}",
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

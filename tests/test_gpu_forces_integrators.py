@@ -207,9 +207,15 @@ def _canonical_topology(spec, ids):
     canon_of_id = np.empty(len(ids), np.int64)
     canon_of_id[ids] = np.arange(len(ids))
     out = {}
-    for name in ("bonds", "angles", "exclusions"):
+    for name in ("bonds", "angles", "exclusions", "dihedrals"):
         arr = getattr(spec, name)
         out[name] = None if arr is None else canon_of_id[np.asarray(arr)].astype(np.int32)
+    r = spec.restraint_ids
+    if r is None:
+        out["restraint_ids"] = None
+    else:
+        r = np.asarray(r)
+        out["restraint_ids"] = np.where(r >= 0, canon_of_id[np.maximum(r, 0)], -1).astype(np.int32)
     return out
 
 
@@ -245,6 +251,10 @@ def _check_step(cuda, patch, spec, step, check_pairs=True):
     if topo["bonds"] is not None:
         fref += orc.bonded_forces(g["pos"], spec.box, topo["bonds"], spec.bond_table, spec.bond_tables,
                                   topo["angles"], spec.angle_table, spec.angle_tables, mode=1)
+    if topo["dihedrals"] is not None or topo["restraint_ids"] is not None:
+        fref += orc.dihedral_restraint_forces(g["pos"], spec.box, topo["dihedrals"], spec.dihedral_table,
+                                              spec.dihedral_tables, topo["restraint_ids"], spec.restraint_anchors,
+                                              spec.restraint_k, mode=1)[0]
     assert rel_force_error(g["force"], fref) <= FORCE_TOL, f"force error at step {step}"
     assert abs(g["force"][:, 3].sum() - fref[:, 3].sum()) <= 1e-5 * np.abs(fref[:, 3]).sum() + 1e-6
     # integrator replay

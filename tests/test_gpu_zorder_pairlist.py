@@ -213,3 +213,42 @@ def test_needs_update_and_overflow_bookkeeping(cuda):
     nobox = marsmd.ZOrderPairlist(n)
     with pytest.raises(marsmd.MarsError):
         nobox.build_pairlist(d, n, 12.0)
+
+
+def test_dense_cluster_outgrows_first_row_capacity(cuda):
+    # 6000 beads packed in a 45 A cube inside a sparse 500 A box: the first build's row
+    # capacity (sized for the mean density) is far too small, so the walk runs again
+    # with rows sized to the measured maximum; the pair set must not notice
+    rs = np.random.default_rng(21)
+    L = 500.0
+    dense = rs.uniform(200.0, 245.0, size=(6000, 3))
+    sparse = rs.uniform(0.0, L, size=(3000, 3))
+    pos = np.zeros((9000, 4), np.float32)
+    pos[:, :3] = np.vstack([dense, sparse])
+    box = PeriodicBox((L, L, L))
+    pl, gp = _gpu_pairs(cuda, pos, box, 14.0)
+    ref = orc.build_pairlist(pos, box, 14.0)
+    assert np.array_equal(gp, ref["pairs"].astype(np.int64))
+    deg = np.bincount(ref["pairs"].ravel(), minlength=len(pos))
+    assert pl.get_statistics().max_neighbors_per_particle == deg.max()
+
+
+def test_rebuild_keeps_pair_set_and_degrees(cuda):
+    rs = np.random.default_rng(5)
+    n, L = 20000, 150.0
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rs.uniform(0, L, size=(n, 3))
+    box = PeriodicBox((L, L, L))
+    pl = marsmd.ZOrderPairlist(n, 1 << 40)
+    pl.set_periodic_box(box)
+    d = _dev(cuda, pos)
+    for rebuild in range(2):  # the second build reuses the first build's row capacity
+        pl.build_pairlist(d, n, 16.0)
+        cuda.cuda.synchronize()
+        gp = sorted_pairs(pl.get_neighbor_pairs().cpu().numpy())
+        ref = orc.build_pairlist(pos, box, 16.0)
+        assert np.array_equal(gp, ref["pairs"].astype(np.int64)), f"build {rebuild}"
+        deg = np.bincount(ref["pairs"].ravel(), minlength=n)
+        st = pl.get_statistics()
+        assert st.max_neighbors_per_particle == deg.max()
+        assert st.num_pairs == len(ref["pairs"])
