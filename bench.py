@@ -66,6 +66,95 @@ def cpu_baseline(spec, steps: int):
                       f"host {platform.processor() or platform.machine()} nproc={os.cpu_count()}"}
 
 
+def copy_probe(torch, n: int = 1 << 28, reps: int = 3):
+    """A plain streaming kernel of known size: y = -x over n float32 (4n bytes read and
+    4n written per launch). Profiled with the same counters as the step kernel, it gives
+    the factor between counter readings and bytes moved."""
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe": {"elements": n, "bytes_read_per_launch": 4 * n,
+                                     "bytes_written_per_launch": 4 * n, "launches": reps}}))
+
+
+def bd_frames(n: int, L: float, snapshots: int, seed: int = 42, warmup: int = 100, stride: int = 10):
+    """Brownian-dynamics frames of the 4-type WCA+DH system (the force-matching
+    workload's snapshots), in global-id order, from the GPU Patch."""
+    import numpy as np
+
+    import marsmd
+    from marsmd import configs, rng
+    from marsmd.system import RunParams, SystemSpec
+    from marsmd.core import ParticleType
+    from marsmd import _capi as api
+
+    tables, tm = configs.fm_tables()
+    type_ids = np.minimum((rng.uniform(seed, rng.STREAM_POSITIONS ^ 0x4, n)[:, 0] * 4).astype(np.int32), 3)
+    q = [1.0, -1.0, 0.5, -0.5]
+    spec = SystemSpec(box=marsmd.PeriodicBox((L, L, L)),
+                      types=[ParticleType(mass=1.0, charge=qi, diffusion=(150.0,) * 3) for qi in q],
+                      positions=configs.uniform_positions(n, L, seed), type_ids=type_ids,
+                      params=RunParams(cutoff=30.0, pairlist_cutoff=35.0, rebuild_period=20, timestep=1e-4,
+                                       kT=configs.KT, integrator=api.INTEGRATOR_BROWNIAN),
+                      nb_tables=tables, term_matrix=tm, seed=seed, name="fm-bd")
+    patch = marsmd.Patch(spec)
+    patch.run(warmup)
+    out = np.zeros((snapshots, n, 3), np.float32)
+    step = warmup
+    for s in range(snapshots):
+        patch.run(stride, first_step=step)
+        step += stride
+        h = patch.copy_particles_to_host()
+        out[s, h["id"]] = h["pos"][:, :3]
+    return out, type_ids
+
+
+def bench_fm(args, torch):
+    """Gradient evaluations per second of force matching: forces of 64 snapshots of
+    100k beads for 10 tables x 512 knots, MSE loss, FP64 adjoint, Adam step."""
+    from marsmd import configs
+    from marsmd.fm import ForceMatching, force_matching_loss
+
+    n, L, S = 100_000, 1000.0, 64
+    frames, type_ids = bd_frames(n, L, S)
+    w = configs.config4(n=n, L=L, snapshots=S, frames=frames)
+    assert (w.type_ids == type_ids).all()
+    fm = ForceMatching(w.positions, w.type_ids, w.box, w.tables, w.term_matrix, w.cutoff)
+    target = fm.forces(fm.knots(w.target_tables))[..., :3].clone()
+    knots = fm.knots().clone().requires_grad_(True)
+    opt = torch.optim.Adam([knots], lr=0.02)
+
+    def one():
+        opt.zero_grad()
+        loss = force_matching_loss(knots, fm, target)
+        loss.backward()
+        opt.step()
+        return loss
+
+    first = one().item()
+    for _ in range(max(args.warmup // 4, 2)):
+        one()
+    K = max(args.steps // 10, 10)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(K):
+        loss = one()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    print(json.dumps({
+        "metric": "force-matching gradient evaluations/s (64 BD snapshots x 100k beads, 10 tables x 512 knots)",
+        "value": K / (ms / 1e3), "unit": "grad evals/s", "n_gpus": 1, "steps": K, "warmup": max(args.warmup // 4, 2),
+        "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded BD frames)",
+        "config": {"workload": w.name, "snapshots": S, "beads": n, "tables": len(w.tables), "knots_per_table": 512,
+                   "pairs_total": fm.num_pairs()},
+        "loss_first": first, "loss_last": loss.item()}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +164,12 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=15)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["langevin", "fm"], default="langevin",
+                    help="langevin: the per-step path at 1M beads; fm: force-matching gradient evaluations "
+                         "(64 BD snapshots x 100k beads, 10 tables x 512 knots)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a streaming kernel of known size (reads and writes 1 GiB each), the "
+                         "scale for hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -83,6 +178,13 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_probe:
+        copy_probe(torch)
+        return
+    if args.workload == "fm":
+        bench_fm(args, torch)
+        return
 
     import marsmd
     from marsmd import configs
@@ -111,6 +213,12 @@ def main():
     value = spec.n * args.steps / (total_ms / 1e3)
     acc = algorithmic_bytes(spec)
     t_kernel = kernel_ms / max(launches, 1) * 1e-3
+    traffic, traffic_src = None, None
+    pmc = ROOT / "profiles" / "k_step_langevin_1M_grid256_pmc.json"
+    if pmc.exists():
+        rec = json.loads(pmc.read_text())
+        if rec.get("workload") == spec.name:
+            traffic, traffic_src = rec["hbm_bytes_per_launch"], str(pmc.relative_to(ROOT))
     achieved = acc["step_kernel_bytes_per_launch"] / t_kernel
     out = {
         "metric": "particle-steps/s at 1M beads",
@@ -133,7 +241,8 @@ def main():
         "b_alg_per_particle_step": acc["b_alg_per_particle_step"],
         "roofline": {"bound": "hbm", "kernel": "k_step (fused pair force + PMF + BAOAB)",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_unit": "B/launch",
+                     "traffic_source": traffic_src,
                      "avg_launch_us": t_kernel * 1e6, "launches": launches,
                      "other_ms_per_step": (total_ms - kernel_ms) / args.steps},
     }

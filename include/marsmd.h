@@ -432,6 +432,41 @@ marsmd_status marsmd_patch_run(marsmd_patch* p, const marsmd_run_params* params,
 marsmd_status marsmd_patch_set_kernel_timing(marsmd_patch* p, int32_t enable);
 marsmd_status marsmd_patch_get_kernel_timing(marsmd_patch* p, double* total_ms, uint64_t* launches);
 
+/* ------------------------------------------------------------------ force matching
+ * No counterpart in the reference, which has no gradient path: the forces of a fixed
+ * set of configurations (snapshots) under tabulated pair potentials whose knots are
+ * trainable, and the adjoint dL/dknot of a loss of those forces in FP64. The force is
+ * the Patch's pair term (PairNonbondedComputer, Pairwise.h:184-305) on each snapshot's
+ * own pairlist, built once at creation and kept in HBM. */
+typedef struct marsmd_fm marsmd_fm;
+
+typedef struct marsmd_fm_desc {
+    marsmd_box box;
+    uint64_t num_particles;        /* per snapshot                                   */
+    uint64_t num_snapshots;
+    const float* positions;        /* device, num_snapshots x num_particles float4    */
+    const int32_t* type_ids;       /* device, num_particles (the same every snapshot) */
+    int32_t num_types;
+    const uint32_t* term_matrix;   /* host, num_types^2 pair tags (make_pair_tag)     */
+    int32_t num_tables;
+    const marsmd_table* tables;    /* host: start, step_inv, size (non-periodic); .pot unused */
+    float cutoff;                  /* force cutoff (d^2 > cutoff^2 skipped)           */
+    float pairlist_cutoff;         /* pairs listed at d <= pairlist_cutoff (0: cutoff) */
+} marsmd_fm_desc;
+
+marsmd_status marsmd_fm_create(const marsmd_fm_desc* desc, marsmd_fm** out);
+marsmd_status marsmd_fm_destroy(marsmd_fm* fm);
+/* Listed pairs summed over the snapshots. */
+marsmd_status marsmd_fm_num_pairs(marsmd_fm* fm, uint64_t* num_pairs);
+/* Forces for the given knots (device FP32, all tables concatenated in table order):
+ * force_energy4 is num_snapshots x num_particles float4 {F, energy}, each snapshot in
+ * its own particle order. */
+marsmd_status marsmd_fm_forces(marsmd_fm* fm, const float* knots, float* force_energy4, marsmd_stream stream);
+/* dL/dknot (device FP64, one per knot) for dL/dF = grad_force4 (device, the layout of
+ * force_energy4; .w ignored). Overwrites grad_knots. */
+marsmd_status marsmd_fm_adjoint(marsmd_fm* fm, const float* knots, const float* grad_force4, double* grad_knots,
+                                marsmd_stream stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

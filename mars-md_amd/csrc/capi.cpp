@@ -1,5 +1,6 @@
 // capi.cpp — the extern "C" boundary declared in include/marsmd.h. Every entry
 // point converts exceptions into a marsmd_status and a thread-local message.
+#include "fm.hpp"
 #include "launchers.hpp"
 #include "pairlist.hpp"
 #include "patch.hpp"
@@ -442,6 +443,53 @@ marsmd_status marsmd_patch_get_kernel_timing(marsmd_patch* p, double* total_ms, 
 		need(total_ms, "total_ms");
 		need(launches, "launches");
 		p->impl.kernel_timing(total_ms, launches);
+	});
+}
+
+// ------------------------------------------------------------------ force matching
+struct marsmd_fm {
+	mmd::FMImpl impl;
+	explicit marsmd_fm(const marsmd_fm_desc& d) : impl(d) {}
+};
+
+marsmd_status marsmd_fm_create(const marsmd_fm_desc* desc, marsmd_fm** out) {
+	return guard([&] {
+		need(desc, "desc");
+		need(out, "out");
+		*out = nullptr;
+		*out = new marsmd_fm(*desc);
+	});
+}
+
+marsmd_status marsmd_fm_destroy(marsmd_fm* fm) {
+	return guard([&] { delete fm; });
+}
+
+marsmd_status marsmd_fm_num_pairs(marsmd_fm* fm, uint64_t* num_pairs) {
+	return guard([&] {
+		need(fm, "fm");
+		need(num_pairs, "num_pairs");
+		*num_pairs = fm->impl.num_pairs;
+	});
+}
+
+marsmd_status marsmd_fm_forces(marsmd_fm* fm, const float* knots, float* force_energy4, marsmd_stream stream) {
+	return guard([&] {
+		need(fm, "fm");
+		need(knots, "knots");
+		need(force_energy4, "force_energy4");
+		fm->impl.forces(knots, force_energy4, st(stream));
+	});
+}
+
+marsmd_status marsmd_fm_adjoint(marsmd_fm* fm, const float* knots, const float* grad_force4, double* grad_knots,
+								marsmd_stream stream) {
+	return guard([&] {
+		need(fm, "fm");
+		need(knots, "knots");
+		need(grad_force4, "grad_force4");
+		need(grad_knots, "grad_knots");
+		fm->impl.adjoint(knots, grad_force4, grad_knots, st(stream));
 	});
 }
 

@@ -1,0 +1,245 @@
+// fm.hip — force matching of tabulated pair potentials on gfx950: the forces of a fixed
+// set of configurations (snapshots) for given table knots, and the adjoint dL/dknot of a
+// loss of those forces, accumulated in FP64.
+//
+// The reference has no gradient path; this is the library's own. The forward force is
+// the Patch's pair term (nb_pair: PairNonbondedComputer, Pairwise.h:184-305, with
+// TabulatedPotential::compute, TabulatedPotential.h:19-52) over each snapshot's own
+// pairlist (ZOrderPairlist semantics), i.e. calculate_nonbonded_forces on that
+// configuration. A table force is linear in its knots,
+//   fm = -(U[h+1] - U[h]) step_inv      (h = the bin of the pair's distance),
+// so with G = dL/dF each pair (x = lower canonical index, y = higher) contributes
+//   dL/dfm = (G_y - G_x) . u,   dL/dU[h] += dL/dfm step_inv,   dL/dU[h+1] -= dL/dfm step_inv;
+// a distance past the last knot of a (non-periodic) table carries no force and no
+// gradient. Each pair is visited once, from its lower canonical end.
+#include "fm.hpp"
+
+#include "nbforce.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace mmd {
+
+namespace {
+
+constexpr int kFmBlock = 256;
+constexpr int kFmLdsGrad = 7936; // knots whose gradient a workgroup accumulates in LDS (62 KB)
+
+__global__ __launch_bounds__(kFmBlock) void k_fm_forces(const float4* __restrict__ spos,
+														const uint32_t* __restrict__ sorted_idx, NbrList L,
+														uint64_t n, NbTables T, const float* __restrict__ knots_g,
+														DevBox box, float cutoff2, float4* __restrict__ out) {
+	extern __shared__ float lds_knots[];
+	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+	const float* knots = stage_tables(T, knots_g, lds_knots, lds_tag);
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const float4 f = nb_force(uint32_t(t), spos[t], spos, L, T, knots, lds_tag, box, cutoff2, true);
+	out[sorted_idx[t]] = f;
+}
+
+// dL/dknot of one snapshot: per-workgroup FP64 sums in LDS (or global FP64 atomics when
+// the tables exceed the LDS budget), written to partial[blockIdx.x][knot].
+template<bool LDS>
+__global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restrict__ spos, NbrList L, uint64_t n,
+														 NbTables T, const float* __restrict__ knots_g, DevBox box,
+														 float cutoff2, const float4* __restrict__ grad,
+														 double* __restrict__ partial) {
+	extern __shared__ double lds_grad[];
+	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+	const int nt2 = T.num_types * T.num_types;
+	for (int i = threadIdx.x; i < nt2; i += blockDim.x)
+		lds_tag[i] = T.tag[i];
+	double* acc = LDS ? lds_grad : partial;
+	if (LDS)
+		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
+			lds_grad[i] = 0.0;
+	__syncthreads();
+	for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += uint64_t(gridDim.x) * blockDim.x) {
+		const float4 p = spos[t];
+		const uint32_t mt = __float_as_uint(p.w);
+		const uint32_t ct = meta_canon(mt);
+		const uint32_t tt = meta_type(mt);
+		const float4 gx = grad[ct];
+		const uint32_t cnt = L.count[t];
+		const uint64_t base = L.tile_base[t >> 6];
+		for (uint32_t k = 0; k < cnt; ++k) {
+			const uint32_t j = L.nbr[nbr_slot_index(base, uint32_t(t), k)];
+			const float4 q = spos[j];
+			const uint32_t mj = __float_as_uint(q.w);
+			const uint32_t cj = meta_canon(mj);
+			if (!(ct < cj))
+				continue; // the pair is taken from its lower canonical end
+			const uint32_t tag = lds_tag[tt * uint32_t(T.num_types) + meta_type(mj)];
+			if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
+				continue;
+			const float3 r = wrap_diff(box, sub3(q, p));
+			const float d = sqrtf(len2(r));
+			if (d < 1e-6f)
+				continue;
+			if (cutoff2 > 0.0f && d * d > cutoff2)
+				continue;
+			const float inv = 1.0f / d;
+			const float ux = inv * r.x, uy = inv * r.y, uz = inv * r.z;
+			const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
+			const float w = (d - T.start[ti]) * T.step_inv[ti];
+			int h = int(floorf(w));
+			if (h >= T.size[ti] - 1)
+				continue;
+			if (h < 0)
+				h = 0;
+			const float4 gy = grad[cj];
+			const double dfm = (double(gy.x) - double(gx.x)) * double(ux) + (double(gy.y) - double(gx.y)) * double(uy) +
+							   (double(gy.z) - double(gx.z)) * double(uz);
+			const double g = dfm * double(T.step_inv[ti]);
+			const int k0 = T.offset[ti] + h;
+			if (LDS) {
+				atomicAdd(&acc[k0], g);
+				atomicAdd(&acc[k0 + 1], -g);
+			} else {
+				atomicAdd(&acc[k0], g);
+				atomicAdd(&acc[k0 + 1], -g);
+			}
+		}
+	}
+	if (LDS) {
+		__syncthreads();
+		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
+			partial[uint64_t(blockIdx.x) * uint64_t(T.total_knots) + uint64_t(i)] = lds_grad[i];
+	}
+}
+
+// out[k] += sum over workgroups of partial[b][k], in workgroup order.
+__global__ void k_fm_reduce(const double* __restrict__ partial, uint32_t blocks, uint32_t total,
+							double* __restrict__ out) {
+	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= total)
+		return;
+	double s = 0.0;
+	for (uint32_t b = 0; b < blocks; ++b)
+		s += partial[uint64_t(b) * total + k];
+	out[k] += s;
+}
+
+__global__ void k_fill_f64(double* p, uint64_t n) {
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = 0.0;
+}
+
+} // namespace
+
+struct FMImpl::Snap {
+	DevArray<float4> spos;
+	DevArray<uint32_t> sorted_idx, counts, tile_base, nbr;
+};
+
+FMImpl::FMImpl(const marsmd_fm_desc& d) : tables(new NbTables{}) {
+	if (d.num_types < 1 || d.num_types > MARSMD_MAX_TYPES)
+		raise(MARSMD_ERR_VALUE, "force matching: num_types must be in [1, 16]");
+	if (d.num_tables < 1 || d.num_tables > kMaxTables)
+		raise(MARSMD_ERR_VALUE, "force matching: num_tables must be in [1, 32]");
+	if (d.num_particles == 0 || d.num_snapshots == 0 || !d.positions || !d.type_ids)
+		raise(MARSMD_ERR_VALUE, "force matching: snapshots, particles, positions and types are required");
+	if (d.num_particles >= kMaxParticles)
+		raise(MARSMD_ERR_VALUE, "force matching: particle count must be below 2^24");
+	n = d.num_particles;
+	S = d.num_snapshots;
+	box = d.box;
+	cutoff2 = d.cutoff > 0.0f ? d.cutoff * d.cutoff : 0.0f;
+	NbTables& T = *tables;
+	T.num_types = d.num_types;
+	T.num_tables = d.num_tables;
+	T.enabled = MARSMD_PAIR_TERM_TABULATED;
+	for (int i = 0; i < d.num_types * d.num_types; ++i)
+		T.tag[i] = d.term_matrix ? d.term_matrix[i] : 0u;
+	int total = 0;
+	for (int k = 0; k < d.num_tables; ++k) {
+		const marsmd_table& t = d.tables[k];
+		if (t.size < 2 || t.is_periodic)
+			raise(MARSMD_ERR_VALUE, "force matching: tables need >= 2 knots and are not periodic");
+		T.start[k] = t.start;
+		T.step_inv[k] = t.step_inv;
+		T.size[k] = int(t.size);
+		T.offset[k] = total;
+		total += int(t.size);
+	}
+	for (int i = 0; i < d.num_types * d.num_types; ++i)
+		if ((T.tag[i] & MARSMD_PAIR_TERM_TABULATED) && int(T.tag[i] >> MARSMD_PAIR_TABLE_SHIFT) >= d.num_tables)
+			raise(MARSMD_ERR_VALUE, "force matching: a pair tag names a missing table");
+	T.total_knots = total;
+	T.lds = total <= kLdsKnots ? 1 : 0;
+	PairlistImpl pl(n, ~uint64_t(0));
+	pl.set_box(box);
+	const float plc = d.pairlist_cutoff > 0.0f ? d.pairlist_cutoff : d.cutoff;
+	for (uint64_t s = 0; s < S; ++s) {
+		pl.build(reinterpret_cast<const float4*>(d.positions) + s * n, nullptr, false, d.type_ids, n, plc, nullptr);
+		Snap* sn = new Snap;
+		snaps.push_back(sn);
+		const uint64_t tiles = (n + 63) / 64;
+		sn->spos.resize(n);
+		sn->sorted_idx.resize(n);
+		sn->counts.resize(n);
+		sn->tile_base.resize(tiles + 1);
+		sn->nbr.resize(std::max<uint64_t>(pl.nbr_storage, 1));
+		MMD_HIP(hipMemcpyAsync(sn->spos.data(), pl.spos.data(), 16 * n, hipMemcpyDeviceToDevice, nullptr));
+		MMD_HIP(hipMemcpyAsync(sn->sorted_idx.data(), pl.sorter.sorted_idx.data(), 4 * n, hipMemcpyDeviceToDevice,
+							   nullptr));
+		MMD_HIP(hipMemcpyAsync(sn->counts.data(), pl.counts.data(), 4 * n, hipMemcpyDeviceToDevice, nullptr));
+		MMD_HIP(hipMemcpyAsync(sn->tile_base.data(), pl.tile_base.data(), 4 * (tiles + 1), hipMemcpyDeviceToDevice,
+							   nullptr));
+		if (pl.nbr_storage)
+			MMD_HIP(hipMemcpyAsync(sn->nbr.data(), pl.nbr.data(), 4 * pl.nbr_storage, hipMemcpyDeviceToDevice, nullptr));
+		num_pairs += pl.num_pairs();
+	}
+	MMD_HIP(hipDeviceSynchronize());
+}
+
+FMImpl::~FMImpl() {
+	for (Snap* s : snaps)
+		delete s;
+	delete tables;
+}
+
+void FMImpl::forces(const float* knots, float* out4, hipStream_t s) {
+	const unsigned lds = tables->lds ? unsigned(tables->total_knots) * 4u : 0u;
+	for (uint64_t k = 0; k < S; ++k) {
+		Snap& sn = *snaps[k];
+		const NbrList L{sn.counts.data(), sn.tile_base.data(), sn.nbr.data()};
+		k_fm_forces<<<grid_for(n, kFmBlock), kFmBlock, lds, s>>>(sn.spos.data(), sn.sorted_idx.data(), L, n, *tables,
+																 knots, make_devbox(box), cutoff2,
+																 reinterpret_cast<float4*>(out4) + k * n);
+		MMD_LAUNCH_CHECK();
+	}
+}
+
+void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStream_t s) {
+	const uint32_t total = uint32_t(tables->total_knots);
+	const bool lds = total <= uint32_t(kFmLdsGrad);
+	const unsigned blocks = lds ? unsigned(std::min<uint64_t>(grid_for(n, kFmBlock), 512)) : 1u;
+	partial.resize(uint64_t(blocks) * total);
+	k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(out, total);
+	MMD_LAUNCH_CHECK();
+	for (uint64_t k = 0; k < S; ++k) {
+		Snap& sn = *snaps[k];
+		const NbrList L{sn.counts.data(), sn.tile_base.data(), sn.nbr.data()};
+		const float4* g = reinterpret_cast<const float4*>(grad4) + k * n;
+		if (lds) {
+			k_fm_adjoint<true><<<blocks, kFmBlock, total * 8u, s>>>(sn.spos.data(), L, n, *tables, knots,
+																	make_devbox(box), cutoff2, g, partial.data());
+		} else {
+			k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), total);
+			MMD_LAUNCH_CHECK();
+			k_fm_adjoint<false><<<grid_for(n, kFmBlock), kFmBlock, 0, s>>>(sn.spos.data(), L, n, *tables, knots,
+																		   make_devbox(box), cutoff2, g,
+																		   partial.data());
+		}
+		MMD_LAUNCH_CHECK();
+		k_fm_reduce<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), blocks, total, out);
+		MMD_LAUNCH_CHECK();
+	}
+}
+
+} // namespace mmd

@@ -1,0 +1,33 @@
+// fm.hpp — host class behind marsmd_fm (see fm.hip).
+#pragma once
+
+#include <vector>
+
+#include "pairlist.hpp"
+
+namespace mmd {
+
+struct NbTables;
+
+struct FMImpl {
+	struct Snap;
+	uint64_t n = 0, S = 0, num_pairs = 0;
+	marsmd_box box{};
+	float cutoff2 = 0.0f;
+	NbTables* tables = nullptr;
+	std::vector<Snap*> snaps;
+	DevArray<double> partial;
+
+	explicit FMImpl(const marsmd_fm_desc& d);
+	~FMImpl();
+	FMImpl(const FMImpl&) = delete;
+	FMImpl& operator=(const FMImpl&) = delete;
+
+	// Forces of every snapshot for the given knots (device FP32, tables concatenated):
+	// out4 is S x n float4 records {F, energy} in each snapshot's particle order.
+	void forces(const float* knots, float* out4, hipStream_t s);
+	// dL/dknot (device FP64) for dL/dF = grad4 (S x n float4, .w ignored).
+	void adjoint(const float* knots, const float* grad4, double* out, hipStream_t s);
+};
+
+} // namespace mmd

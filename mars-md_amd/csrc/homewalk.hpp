@@ -1,0 +1,247 @@
+// homewalk.hpp — the home-cell pairlist walk (see k_walk_home) and the reference's
+// canonical pair predicate, shared by the pairlist (pairlist.hip) and the replica batch
+// (batch.hip).
+#pragma once
+
+#include "pairlist.hpp"
+
+namespace mmd {
+
+namespace {
+
+// Canonical pair predicate of ZOrderCellNeighborKernel for sorted lo < hi
+// (ZOrderNeighbor.h:204-221): wrap_diff(pos_hi - pos_lo)^2 <= rc^2, not excluded by
+// lo's CSR row or a shared rigid body.
+__device__ __forceinline__ bool pair_predicate(const DevBox& box, float rc2, const marsmd_exclusion_view& ex,
+											   const float4& plo, const float4& phi) {
+	const float3 dr = wrap_diff(box, sub3(phi, plo));
+	if (!(len2(dr) <= rc2))
+		return false;
+	const int a = int(meta_canon(__float_as_uint(plo.w)));
+	const int b = int(meta_canon(__float_as_uint(phi.w)));
+	if (ex.rigid_body_id) {
+		const int body = ex.rigid_body_id[a];
+		if (body >= 0 && ex.rigid_body_id[b] == body)
+			return false;
+	}
+	if (ex.offsets && a >= 0 && uint64_t(a) < ex.num_particles) {
+		const int e0 = ex.offsets[a], e1 = ex.offsets[a + 1];
+		for (int e = e0; e < e1; ++e)
+			if (ex.excluded[e] == b)
+				return false;
+	}
+	return true;
+}
+// ------------------------------------------------------------ home-cell walk
+// One wave per home cell: a Morton cell of the home level h, whose particles are
+// contiguous in sorted order. The wave enumerates the fine cells (level m = h + s)
+// within `reach` fine cells of the home cell, keeps those whose box can lie within rc
+// of the home cell's box (both widened by the quantisation margin, so the test is
+// conservative across the periodic seam), stores their particle ranges as segments
+// in LDS, then streams the candidates 64 at a time (one coalesced load per lane) and
+// tests each against the wave's home particles, the candidate broadcast with
+// readlane. Predicate and orientation are k_walk's (the reference's), so the pair set
+// is the same; a particle's list is in candidate order. Entries past `cap` are not
+// written (the host sees the count and walks again with a larger cap).
+constexpr int kHomeWaves = 4;
+constexpr int kSegMax = 448;
+
+struct HomeParams {
+	const float4* spos;
+	const uint32_t* home_begin;
+	const uint32_t* home_end;
+	const uint32_t* fine_begin;
+	const uint32_t* fine_end;
+	uint32_t num_home;
+	int h;         // home level
+	int s;         // fine level m = h + s
+	int reach[3];  // fine cells enumerated beyond the home cell's span
+	int full[3];   // periodic axis enumerated whole at the fine level
+	float cw[3];   // fine cell width (quantised)
+	float margin[3];
+	float rc2;
+	DevBox box;
+	marsmd_exclusion_view ex;
+	uint32_t cap;  // entries per slot the uniform tile rows hold
+};
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+	const int lane = threadIdx.x & 63;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t t = __shfl_up(v, o, 64);
+		if (lane >= o)
+			v += t;
+	}
+	return v;
+}
+__device__ __forceinline__ uint32_t bcast_u32(uint32_t v, uint32_t lane) {
+	return uint32_t(__builtin_amdgcn_readlane(int(v), int(lane)));
+}
+__device__ __forceinline__ float bcast_f32(float v, uint32_t lane) {
+	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), int(lane)));
+}
+// LDS written by some lanes of this wave and read by others: order the accesses.
+__device__ __forceinline__ void wave_lds_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Tests the candidates of segments [0, nseg) (ntot particles) against this lane's home
+// particle t (when valid) and records the hits.
+template<bool FILL>
+__device__ __forceinline__ void home_test_segments(const HomeParams& P, const uint32_t* seg_b, uint32_t* seg_pre,
+												   uint32_t nseg, uint32_t ntot, bool valid, uint32_t t,
+												   const float4& p, uint32_t& cnt, uint32_t* __restrict__ nbr) {
+	const int lane = threadIdx.x & 63;
+	if (lane == 0)
+		seg_pre[nseg] = ntot;
+	wave_lds_sync();
+	const uint64_t row = (uint64_t(t) >> 6) * 64u * P.cap;
+	for (uint32_t q0 = 0; q0 < ntot; q0 += 64) {
+		const uint32_t qi = q0 + uint32_t(lane);
+		uint32_t j = 0xFFFFFFFFu;
+		float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		if (qi < ntot) {
+			uint32_t lo = 0, hi = nseg; // seg_pre[lo] <= qi < seg_pre[hi]
+			while (hi - lo > 1) {
+				const uint32_t mid = (lo + hi) >> 1;
+				if (seg_pre[mid] <= qi)
+					lo = mid;
+				else
+					hi = mid;
+			}
+			j = seg_b[lo] + (qi - seg_pre[lo]);
+			q = P.spos[j];
+		}
+		const uint32_t m = min(64u, ntot - q0);
+		for (uint32_t u = 0; u < m; ++u) {
+			const uint32_t ju = bcast_u32(j, u);
+			const float4 qu = make_float4(bcast_f32(q.x, u), bcast_f32(q.y, u), bcast_f32(q.z, u), bcast_f32(q.w, u));
+			if (valid && ju != t) {
+				const bool t_lo = ju > t; // operands selected, one predicate evaluation
+				const float4 plo = t_lo ? p : qu;
+				const float4 phi = t_lo ? qu : p;
+				const bool in = pair_predicate(P.box, P.rc2, P.ex, plo, phi);
+				if (in) {
+					if (FILL && cnt < P.cap)
+						nbr[nbr_slot_index(row, t, cnt)] = ju;
+					++cnt;
+				}
+			}
+		}
+	}
+	wave_lds_sync();
+}
+
+template<bool FILL>
+__device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg_b, uint32_t* seg_pre,
+							   uint32_t* __restrict__ counts, uint32_t* __restrict__ nbr) {
+	const int lane = threadIdx.x & 63;
+	const uint32_t hb = P.home_begin[home], he = P.home_end[home];
+	if (hb >= he)
+		return;
+	const int hc[3] = {int(compact_by3(home >> 2)), int(compact_by3(home >> 1)), int(compact_by3(home))};
+	const int fine_cells = 1 << (P.h + P.s);
+	const int span = 1 << P.s;
+	int lo[3], nax[3];
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		if (P.full[a]) {
+			lo[a] = 0;
+			nax[a] = fine_cells;
+		} else {
+			int l = hc[a] * span - P.reach[a];
+			int hgh = hc[a] * span + span - 1 + P.reach[a];
+			if (!P.box.periodic[a]) { // open axis: only cells that exist (the reach can be huge
+				l = max(l, 0);        // when the extent is degenerate)
+				hgh = min(hgh, fine_cells - 1);
+			}
+			lo[a] = l;
+			nax[a] = hgh - l + 1;
+		}
+	}
+	const int total = nax[0] * nax[1] * nax[2];
+	const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+	for (uint32_t c0 = hb; c0 < he; c0 += 64) {
+		const uint32_t t = c0 + uint32_t(lane);
+		const bool valid = t < he;
+		const float4 p = valid ? P.spos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		uint32_t cnt = 0;
+		uint32_t nseg = 0, ntot = 0; // wave-uniform
+		for (int cb = 0; cb < total; cb += 64) {
+			const int k = cb + lane;
+			uint32_t b = 0, e = 0;
+			if (k < total) {
+				const int iv[3] = {k % nax[0], (k / nax[0]) % nax[1], k / (nax[0] * nax[1])};
+				float gap2 = 0.0f;
+				bool keep = true;
+				uint32_t code = 0;
+#pragma unroll
+				for (int a = 0; a < 3; ++a) {
+					int u = lo[a] + iv[a];
+					if (!P.full[a]) {
+						const int h0 = hc[a] * span;
+						const int apart = u < h0 ? h0 - u - 1 : (u >= h0 + span ? u - h0 - span : 0);
+						const float g = fmaxf(0.0f, float(apart) * P.cw[a] - 2.0f * P.margin[a]);
+						gap2 += g * g;
+						if (P.box.periodic[a])
+							u = (u + fine_cells) & (fine_cells - 1);
+						else if (u < 0 || u >= fine_cells)
+							keep = false;
+					}
+					code |= split_by3(uint32_t(u) & 1023u) << (2 - a);
+				}
+				if (keep && gap2 <= P.rc2) {
+					b = P.fine_begin[code];
+					e = P.fine_end[code];
+				}
+			}
+			const uint32_t c = e > b ? e - b : 0u;
+			const uint64_t mask = __ballot(c > 0);
+			const uint32_t nnew = uint32_t(__popcll(mask));
+			if (nseg + nnew > uint32_t(kSegMax)) {
+				home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr);
+				nseg = 0;
+				ntot = 0;
+			}
+			const uint32_t incl = wave_inclusive_scan(c);
+			const uint32_t sum = bcast_u32(incl, 63);
+			if (c > 0) {
+				const uint32_t r = nseg + uint32_t(__popcll(mask & lt_mask));
+				seg_b[r] = b;
+				seg_pre[r] = ntot + incl - c;
+			}
+			nseg += nnew;
+			ntot += sum;
+		}
+		home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr);
+		if (valid) {
+			counts[t] = cnt;
+			if (FILL && cnt <= P.cap) {
+				const uint64_t row = (uint64_t(t) >> 6) * 64u * P.cap;
+				for (uint32_t k = cnt; (k & 3u) != 0; ++k)
+					nbr[nbr_slot_index(row, t, k)] = t;
+			}
+		}
+	}
+}
+
+template<bool FILL>
+__global__ __launch_bounds__(256) void k_walk_home(HomeParams P, uint32_t* __restrict__ counts,
+												   uint32_t* __restrict__ nbr) {
+	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
+	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
+	const int wv = threadIdx.x >> 6;
+	const TileRange R = xcd_tiles((P.num_home + kHomeWaves - 1) / kHomeWaves);
+	for (uint32_t g = R.tile; g < R.end; g += R.stride) {
+		const uint32_t home = g * kHomeWaves + uint32_t(wv);
+		if (home < P.num_home)
+			walk_home_cell<FILL>(P, home, seg_b[wv], seg_pre[wv], counts, nbr);
+	}
+}
+
+} // namespace
+
+} // namespace mmd

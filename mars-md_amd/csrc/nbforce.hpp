@@ -1,0 +1,130 @@
+// nbforce.hpp — the nonbonded pair term over the wave-tiled neighbour list, shared by
+// the Patch kernels (patch.hip) and force matching (fm.hip).
+#pragma once
+
+#include "pairlist.hpp"
+
+namespace mmd {
+
+constexpr int kMaxTables = 32;
+constexpr int kLdsKnots = 15360; // 60 KB of table knots staged in LDS
+
+struct NbTables {
+	int num_tables;
+	int num_types;
+	uint32_t enabled;
+	int lds; // knots staged in LDS
+	int total_knots;
+	float start[kMaxTables];
+	float step_inv[kMaxTables];
+	int size[kMaxTables];
+	int offset[kMaxTables];
+	uint32_t tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+};
+
+__device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
+
+// One pair term on slot t (PairNonbondedComputer, Pairwise.h:184-305) in canonical
+// orientation: r = wrap_diff(pos[higher canon] - pos[lower canon]), the lower takes -F.
+__device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t tt, const float4& q,
+										const NbTables& T, const float* __restrict__ knots, const uint32_t* tagm,
+										const DevBox& box, float cutoff2, bool energy, float& fx, float& fy,
+										float& fz, float& e) {
+	const uint32_t mj = fb(q.w);
+	const uint32_t cj = meta_canon(mj);
+	const uint32_t tj = meta_type(mj);
+	const bool t_is_x = ct < cj;
+	const float4 px = t_is_x ? p : q;
+	const float4 py = t_is_x ? q : p;
+	const uint32_t tx = t_is_x ? tt : tj;
+	const uint32_t ty = t_is_x ? tj : tt;
+	const uint32_t tag = tagm[tx * uint32_t(T.num_types) + ty];
+	if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
+		return;
+	const float3 r = wrap_diff(box, sub3(py, px));
+	const float d = sqrtf(len2(r));
+	float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+	if (d > 1e-6f) {
+		const float inv = 1.0f / d;
+		ux = inv * r.x;
+		uy = inv * r.y;
+		uz = inv * r.z;
+	}
+	if (d < 1e-6f)
+		return;
+	if (cutoff2 > 0.0f && d * d > cutoff2)
+		return;
+	const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
+	const FE v = table_compute(d, knots + T.offset[ti], T.size[ti], T.start[ti], T.step_inv[ti], false);
+	const float fm = 0.0f + v.f;
+	const float gx = fm * ux, gy = fm * uy, gz = fm * uz;
+	if (t_is_x) {
+		fx += -gx;
+		fy += -gy;
+		fz += -gz;
+	} else {
+		fx += gx;
+		fy += gy;
+		fz += gz;
+	}
+	if (energy)
+		e += (0.0f + v.e) * 0.5f;
+}
+
+// Sum of the nonbonded pair terms on slot t over its neighbour list, in list order.
+// A lane's list is read in groups of 4 indices (one 16-byte load) and pipelined three
+// deep: while group g is evaluated, the positions of group g+1 and the indices of
+// group g+2 are in flight. The padding entries of a row are t itself (skipped as d = 0).
+__device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const float4* __restrict__ pos,
+										   const NbrList& L, const NbTables& T, const float* __restrict__ knots,
+										   const uint32_t* tagm, const DevBox& box, float cutoff2, bool energy) {
+	const uint32_t mt = fb(p.w);
+	const uint32_t ct = meta_canon(mt);
+	const uint32_t tt = meta_type(mt);
+	float fx = 0.0f, fy = 0.0f, fz = 0.0f, e = 0.0f;
+	const uint32_t ng = (L.count[t] + 3u) >> 2;
+	if (ng == 0)
+		return make_float4(fx, fy, fz, e);
+	const uint4* __restrict__ row = reinterpret_cast<const uint4*>(L.nbr + L.tile_base[t >> 6]) + (t & 63u);
+	const uint4 j0 = row[0];
+	uint4 jn = ng > 1 ? row[64] : j0;
+	float4 q0 = pos[j0.x], q1 = pos[j0.y], q2 = pos[j0.z], q3 = pos[j0.w];
+	for (uint32_t g = 0; g < ng; ++g) {
+		float4 n0 = q0, n1 = q1, n2 = q2, n3 = q3;
+		uint4 jnn = jn;
+		if (g + 1 < ng) {
+			n0 = pos[jn.x];
+			n1 = pos[jn.y];
+			n2 = pos[jn.z];
+			n3 = pos[jn.w];
+			if (g + 2 < ng)
+				jnn = row[uint64_t(g + 2) * 64u];
+		}
+		nb_pair(p, ct, tt, q0, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair(p, ct, tt, q1, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair(p, ct, tt, q2, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair(p, ct, tt, q3, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		q0 = n0;
+		q1 = n1;
+		q2 = n2;
+		q3 = n3;
+		jn = jnn;
+	}
+	return make_float4(fx, fy, fz, e);
+}
+
+// Stage table knots and the tag matrix in LDS.
+__device__ __forceinline__ const float* stage_tables(const NbTables& T, const float* __restrict__ knots_g,
+													 float* lds_knots, uint32_t* lds_tag) {
+	const int nt2 = T.num_types * T.num_types;
+	for (int i = threadIdx.x; i < nt2; i += blockDim.x)
+		lds_tag[i] = T.tag[i];
+	if (T.lds) {
+		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
+			lds_knots[i] = knots_g[i];
+	}
+	__syncthreads();
+	return T.lds ? lds_knots : knots_g;
+}
+
+} // namespace mmd

@@ -148,6 +148,22 @@ class PatchDesc(C.Structure):
     ]
 
 
+class FMDesc(C.Structure):
+    _fields_ = [
+        ("box", Box),
+        ("num_particles", C.c_uint64),
+        ("num_snapshots", C.c_uint64),
+        ("positions", C.c_void_p),
+        ("type_ids", C.c_void_p),
+        ("num_types", C.c_int32),
+        ("term_matrix", C.c_void_p),
+        ("num_tables", C.c_int32),
+        ("tables", C.c_void_p),
+        ("cutoff", C.c_float),
+        ("pairlist_cutoff", C.c_float),
+    ]
+
+
 class RunParams(C.Structure):
     _fields_ = [
         ("pairlist_cutoff", C.c_float),
@@ -224,6 +240,11 @@ PROTOTYPES = {
     "marsmd_patch_run": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, _P]),
     "marsmd_patch_set_kernel_timing": (_I32, [_P, _I32]),
     "marsmd_patch_get_kernel_timing": (_I32, [_P, C.POINTER(C.c_double), C.POINTER(_U64)]),
+    "marsmd_fm_create": (_I32, [C.POINTER(FMDesc), C.POINTER(_P)]),
+    "marsmd_fm_destroy": (_I32, [_P]),
+    "marsmd_fm_num_pairs": (_I32, [_P, C.POINTER(_U64)]),
+    "marsmd_fm_forces": (_I32, [_P, _P, _P, _P]),
+    "marsmd_fm_adjoint": (_I32, [_P, _P, _P, _P, _P]),
 }
 
 LIB_PATH = Path(__file__).resolve().parent / "libmarsmd_hip.so"

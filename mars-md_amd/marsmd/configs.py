@@ -16,6 +16,7 @@ Every constant the workload descriptions leave open is pinned here once:
 from __future__ import annotations
 
 import math
+from dataclasses import dataclass
 from typing import List, Optional
 
 import numpy as np
@@ -277,3 +278,60 @@ def config2(n: int = 1_000_000, L: float = 2000.0, seed: int = 42, steps: int = 
     )
     spec.steps = steps
     return spec
+
+
+@dataclass
+class FMWorkload:
+    """A force-matching problem: snapshots, the starting tables and the tables whose
+    forces are the targets."""
+    name: str
+    box: PeriodicBox
+    positions: np.ndarray  # (S, n, 3) float32
+    type_ids: np.ndarray  # (n,) int32
+    tables: List[TabulatedPotential]
+    target_tables: List[TabulatedPotential]
+    term_matrix: np.ndarray  # (T, T) uint32 pair tags
+    cutoff: float
+
+
+def fm_tables(knots: int = 512, rmax: float = 30.0):
+    """4 types (charges +1, -1, +0.5, -0.5) and one WCA+DH table of `knots` knots on
+    [0, rmax] per type pair (10 tables); returns (tables, tag matrix)."""
+    q = [1.0, -1.0, 0.5, -0.5]
+    tables = []
+    tm = np.zeros((4, 4), np.uint32)
+    k = 0
+    for i in range(4):
+        for j in range(i, 4):
+            tables.append(pair_table(qq=q[i] * q[j], rmax=rmax, knots=knots))
+            tm[i, j] = tm[j, i] = make_pair_tag(api.PAIR_TERM_TABULATED, k)
+            k += 1
+    return tables, tm
+
+
+def perturbed_tables(tables, sigma: float, seed: int):
+    """Every knot but the last (the force must still vanish past the cutoff) plus a
+    seeded Gaussian of standard deviation sigma (kcal/mol)."""
+    g = np.random.default_rng(seed)
+    out = []
+    for t in tables:
+        noise = g.normal(0.0, sigma, len(t.pot))
+        noise[-1] = 0.0
+        out.append(TabulatedPotential((t.pot.astype(np.float64) + noise).astype(np.float32), t.start, t.step))
+    return out
+
+
+def config4(n: int = 100_000, L: float = 1000.0, snapshots: int = 64, knots: int = 512, seed: int = 42,
+            perturbation: float = 0.2, frames: Optional[np.ndarray] = None) -> FMWorkload:
+    """Force matching of 10 tables x 512 knots over 64 snapshots of 100k beads (4 types,
+    L = 1000 A, cutoff 30 A). `frames` (S, n, 3) supplies the snapshots (BD frames);
+    without it they are independent uniform configurations (rejection at 3 A). Targets
+    are the forces of the tables plus a seeded 0.2 kcal/mol Gaussian on every knot."""
+    tables, tm = fm_tables(knots)
+    type_ids = np.minimum((rng.uniform(seed, rng.STREAM_POSITIONS ^ 0x4, n)[:, 0] * 4).astype(np.int32), 3)
+    if frames is None:
+        frames = np.stack([uniform_positions(n, L, seed + 1000 * (s + 1)) for s in range(snapshots)])
+    frames = np.asarray(frames, np.float32)[..., :3]
+    return FMWorkload(name=f"force-matching-{frames.shape[0]}x{n}-beads-4types-L{L:g}-10x{knots}-knots",
+                      box=PeriodicBox((L, L, L)), positions=frames, type_ids=type_ids, tables=tables,
+                      target_tables=perturbed_tables(tables, perturbation, seed), term_matrix=tm, cutoff=30.0)

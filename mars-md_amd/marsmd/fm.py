@@ -1,0 +1,135 @@
+"""Force matching of tabulated pair potentials on the GPU, with the adjoint behind
+torch.autograd.
+
+The reference has no gradient path; this is the library's own. `ForceMatching` holds a
+fixed set of configurations (snapshots) and their pairlists in HBM; `forces(knots)` is
+calculate_nonbonded_forces of every snapshot for the given table knots, and
+`adjoint(knots, dL/dF)` is dL/dknot accumulated in FP64 (a table force is linear in its
+knots, so the adjoint is exact: see fm.hip). `TabulatedPairForces` wraps both as a
+torch.autograd.Function.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _capi as api
+from .core import PeriodicBox, TabulatedPotential, _stream, _torch
+
+
+class ForceMatching:
+    def __init__(self, positions, type_ids, box: PeriodicBox, tables: Sequence[TabulatedPotential], term_matrix,
+                 cutoff: float, pairlist_cutoff: float | None = None):
+        torch = _torch()
+        pos = np.asarray(positions, np.float32)
+        if pos.ndim != 3 or pos.shape[2] not in (3, 4):
+            raise ValueError("positions must be (snapshots, particles, 3 or 4)")
+        S, n = pos.shape[0], pos.shape[1]
+        p4 = np.zeros((S, n, 4), np.float32)
+        p4[..., :3] = pos[..., :3]
+        self.S, self.n = S, n
+        self.tables = list(tables)
+        self.sizes = [len(t.pot) for t in self.tables]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.total_knots = int(self.offsets[-1])
+        self._pos = torch.from_numpy(p4).cuda()
+        self._types = torch.from_numpy(np.ascontiguousarray(type_ids, np.int32)).cuda()
+        tm = np.ascontiguousarray(term_matrix, np.uint32)
+        recs = (api.Table * len(self.tables))()
+        for i, t in enumerate(self.tables):
+            recs[i] = t.to_c(0)
+        d = api.FMDesc()
+        d.box = box.to_c()
+        d.num_particles = n
+        d.num_snapshots = S
+        d.positions = self._pos.data_ptr()
+        d.type_ids = self._types.data_ptr()
+        d.num_types = tm.shape[0]
+        d.term_matrix = tm.ctypes.data
+        d.num_tables = len(self.tables)
+        d.tables = C.addressof(recs)
+        d.cutoff = float(cutoff)
+        d.pairlist_cutoff = float(pairlist_cutoff if pairlist_cutoff is not None else cutoff)
+        self._h = C.c_void_p()
+        api.call("marsmd_fm_create", C.byref(d), C.byref(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            api.lib().marsmd_fm_destroy(h)
+            self._h = None
+
+    def knots(self, tables: Sequence[TabulatedPotential] | None = None):
+        """The concatenated FP32 knots of `tables` (default: the ones given at creation)."""
+        torch = _torch()
+        tabs = self.tables if tables is None else tables
+        return torch.from_numpy(np.concatenate([np.asarray(t.pot, np.float32) for t in tabs])).cuda()
+
+    def num_pairs(self) -> int:
+        v = C.c_uint64(0)
+        api.call("marsmd_fm_num_pairs", self._h, C.byref(v))
+        return v.value
+
+    def forces(self, knots, stream=None):
+        """(S, n, 4) float32: force and energy of every snapshot for these knots."""
+        torch = _torch()
+        k = knots.detach().to(device="cuda", dtype=torch.float32).contiguous()
+        if k.numel() != self.total_knots:
+            raise ValueError(f"expected {self.total_knots} knots, got {k.numel()}")
+        out = torch.empty((self.S, self.n, 4), dtype=torch.float32, device="cuda")
+        api.call("marsmd_fm_forces", self._h, k.data_ptr(), out.data_ptr(), _stream(stream))
+        return out
+
+    def adjoint(self, knots, grad_forces, stream=None):
+        """dL/dknot (float64, total knots) for dL/dF = grad_forces (S, n, 3 or 4)."""
+        torch = _torch()
+        k = knots.detach().to(device="cuda", dtype=torch.float32).contiguous()
+        g = grad_forces.detach().to(device="cuda", dtype=torch.float32)
+        if g.shape[-1] == 3:
+            g4 = torch.zeros((self.S, self.n, 4), dtype=torch.float32, device="cuda")
+            g4[..., :3] = g
+            g = g4
+        g = g.contiguous()
+        out = torch.empty(self.total_knots, dtype=torch.float64, device="cuda")
+        api.call("marsmd_fm_adjoint", self._h, k.data_ptr(), g.data_ptr(), out.data_ptr(), _stream(stream))
+        return out
+
+
+def _function():
+    torch = _torch()
+
+    class TabulatedPairForces(torch.autograd.Function):
+        """F(knots) of every snapshot; backward is the FP64 adjoint."""
+
+        @staticmethod
+        def forward(ctx, knots, fm: ForceMatching):
+            ctx.fm = fm
+            ctx.save_for_backward(knots)
+            return fm.forces(knots)[..., :3]
+
+        @staticmethod
+        def backward(ctx, grad_f):
+            (knots,) = ctx.saved_tensors
+            gk = ctx.fm.adjoint(knots, grad_f)
+            return gk.to(knots.dtype), None
+
+    return TabulatedPairForces
+
+
+_FN = None
+
+
+def tabulated_pair_forces(knots, fm: ForceMatching):
+    """Differentiable forces (S, n, 3) of the snapshots of `fm` for `knots`."""
+    global _FN
+    if _FN is None:
+        _FN = _function()
+    return _FN.apply(knots, fm)
+
+
+def force_matching_loss(knots, fm: ForceMatching, target):
+    """Mean squared force error over snapshots, particles and components."""
+    f = tabulated_pair_forces(knots, fm)
+    return ((f.double() - target.double()) ** 2).mean()

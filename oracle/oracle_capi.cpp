@@ -1365,3 +1365,104 @@ int orc_sim_set_field(void* h, const float ef[3], int32_t interp) {
 	return 0;
 }
 } // extern "C"
+
+// ---- force matching of tabulated pair tables (the gradient path of this library; the
+// reference has none). Per snapshot: the oracle's pair set at `cutoff`; each pair's
+// term as PairNonbondedComputer (Pairwise.h:184-305) forms it -- FP32 wrap_diff,
+// distance, unit vector and table bin (TabulatedPotential.h:19-52, non-periodic) --
+// but with FP64 knots and FP64 sums, so the forces are linear in the knots exactly:
+//   fm = -(U[h+1] - U[h]) step_inv.
+// Loss L = mean over snapshots, particles and components of (F - F_target)^2 and
+// dL/dU by the chain rule through that linear map (each pair once, x = lower index):
+//   dL/dfm = (G_y - G_x) . u, G = dL/dF;  dL/dU[h] += dL/dfm step_inv,
+//   dL/dU[h+1] -= dL/dfm step_inv.
+// knots64: all tables' knots, concatenated in table order. Any output may be NULL.
+extern "C" int orc_fm_loss_grad(uint64_t S, uint64_t n, const float* pos4, const int32_t* types, int32_t num_types,
+								const uint32_t* term_matrix, uint32_t enabled, int32_t num_tables,
+								const marsmd_table* tables, const double* knots64, const marsmd_box* box,
+								float cutoff, const double* target, double* forces_out, double* loss_out,
+								double* grad_out) {
+	try {
+		const Box b = make_box(*box);
+		std::vector<size_t> off(size_t(num_tables) + 1, 0);
+		for (int k = 0; k < num_tables; ++k) {
+			if (tables[k].is_periodic)
+				return -1;
+			off[size_t(k) + 1] = off[size_t(k)] + tables[k].size;
+		}
+		if (grad_out)
+			std::fill(grad_out, grad_out + off[size_t(num_tables)], 0.0);
+		const float cut2 = cutoff * cutoff;
+		const double norm = 1.0 / (double(S) * double(n) * 3.0);
+		double loss = 0.0;
+		std::vector<double> F(3 * n), G(3 * n);
+		struct Hit {
+			int x, y;
+			size_t k0;
+			double sinv;
+			double ux, uy, uz;
+		};
+		std::vector<Hit> hits;
+		for (uint64_t s = 0; s < S; ++s) {
+			const V4f* p = as_v4(pos4 + 4 * n * s);
+			PairlistResult r;
+			build_pairlist(p, n, b, cutoff, marsmd_exclusion_view{}, ~uint64_t(0), r, false);
+			std::fill(F.begin(), F.end(), 0.0);
+			hits.clear();
+			for (const auto& pr : r.pairs) {
+				const int x = pr.first, y = pr.second;
+				const uint32_t tag = term_matrix[size_t(types[x]) * size_t(num_types) + size_t(types[y])];
+				if (!(tag & enabled & MARSMD_PAIR_TERM_TABULATED))
+					continue;
+				const V4f dr = wrap_diff<float>(b, sub<float>(p[y], p[x]));
+				const float d = std::sqrt(length2<float>(dr));
+				if (d < 1e-6f)
+					continue;
+				if (cut2 > 0.0f && d * d > cut2)
+					continue;
+				const float inv = 1.0f / d;
+				const float ux = inv * dr.x, uy = inv * dr.y, uz = inv * dr.z;
+				const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
+				const marsmd_table& t = tables[ti];
+				const float w = (d - t.start) * t.step_inv;
+				int h = int(std::floor(w));
+				if (h >= int(t.size) - 1)
+					continue; // past the last knot: no force
+				if (h < 0)
+					h = 0;
+				const size_t k0 = off[size_t(ti)] + size_t(h);
+				const double fm = -(knots64[k0 + 1] - knots64[k0]) * double(t.step_inv);
+				F[3 * size_t(x) + 0] += -fm * ux;
+				F[3 * size_t(x) + 1] += -fm * uy;
+				F[3 * size_t(x) + 2] += -fm * uz;
+				F[3 * size_t(y) + 0] += fm * ux;
+				F[3 * size_t(y) + 1] += fm * uy;
+				F[3 * size_t(y) + 2] += fm * uz;
+				hits.push_back(Hit{x, y, k0, double(t.step_inv), ux, uy, uz});
+			}
+			if (forces_out)
+				std::copy(F.begin(), F.end(), forces_out + 3 * n * s);
+			if (!target)
+				continue;
+			const double* T = target + 3 * n * s;
+			for (size_t i = 0; i < 3 * n; ++i) {
+				const double diff = F[i] - T[i];
+				loss += diff * diff * norm;
+				G[i] = 2.0 * diff * norm;
+			}
+			if (grad_out)
+				for (const Hit& hit : hits) {
+					const size_t X = 3 * size_t(hit.x), Y = 3 * size_t(hit.y);
+					const double dfm =
+						(G[Y] - G[X]) * hit.ux + (G[Y + 1] - G[X + 1]) * hit.uy + (G[Y + 2] - G[X + 2]) * hit.uz;
+					grad_out[hit.k0] += dfm * hit.sinv;
+					grad_out[hit.k0 + 1] -= dfm * hit.sinv;
+				}
+		}
+		if (loss_out)
+			*loss_out = loss;
+		return 0;
+	} catch (...) {
+		return -1;
+	}
+}

@@ -59,6 +59,8 @@ def lib():
             "orc_sim_set_field": (I32, [P, P, I32]),
             "orc_acos": (F, [F]),
             "orc_atan2": (F, [F, F]),
+            "orc_fm_loss_grad": (I32, [U64, U64, P, P, I32, P, C.c_uint32, I32, P, P, C.POINTER(api.Box), F, P, P,
+                                       C.POINTER(C.c_double), P]),
             "orc_dihedral_restraint_forces": (I32, [P, P, U64, P, P, P, P, U64, P, C.POINTER(api.Box), I32, U64, P,
                                                     P]),
             "orc_sim_calculate_nonbonded_forces": (I32, [P, F, F, U64, U64, P, I32, I32]),
@@ -305,6 +307,32 @@ def dihedral_restraint_forces(pos4, box, dihedrals=None, dihedral_table=None, di
                                                pos4.ctypes.data, C.byref(b), mode, len(pos4), out.ctypes.data,
                                                phi.ctypes.data))
     return out, phi[:len(dih)]
+
+
+def fm_loss_grad(positions, type_ids, term_matrix, tables, knots64, box, cutoff, target=None):
+    """The oracle's force-matching forces, loss and dL/dknot (FP32 pair geometry, FP64
+    knots and sums). positions (S, n, 3|4); knots64 all tables' knots concatenated;
+    target (S, n, 3) or None. Returns (forces (S, n, 3), loss, grad)."""
+    pos = np.asarray(positions, np.float32)
+    S, n = pos.shape[0], pos.shape[1]
+    p4 = np.zeros((S, n, 4), np.float32)
+    p4[..., :3] = pos[..., :3]
+    types = np.ascontiguousarray(type_ids, np.int32)
+    tm = np.ascontiguousarray(term_matrix, np.uint32)
+    recs, keep = table_records(list(tables))
+    k64 = np.ascontiguousarray(knots64, np.float64)
+    forces = np.zeros((S, n, 3), np.float64)
+    grad = np.zeros(len(k64), np.float64)
+    loss = C.c_double(0.0)
+    tgt = None if target is None else np.ascontiguousarray(target, np.float64)
+    b = box_c(box)
+    rc = lib().orc_fm_loss_grad(S, n, p4.ctypes.data, types.ctypes.data, tm.shape[0], tm.ctypes.data,
+                                api.PAIR_TERM_TABULATED, len(tables), C.addressof(recs), k64.ctypes.data, C.byref(b),
+                                float(cutoff), None if tgt is None else tgt.ctypes.data, forces.ctypes.data,
+                                C.byref(loss), grad.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("orc_fm_loss_grad failed")
+    return forces, loss.value, grad
 
 
 def grid_sample(grid, pos4, scheme, bc=None, fp64=False):

@@ -53,6 +53,7 @@ STRUCTS = {
     "marsmd_pairlist_stats": api.PairlistStats,
     "marsmd_patch_desc": api.PatchDesc,
     "marsmd_run_params": api.RunParams,
+    "marsmd_fm_desc": api.FMDesc,
 }
 
 
