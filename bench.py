@@ -79,6 +79,68 @@ def copy_probe(torch, n: int = 1 << 28, reps: int = 3):
                                      "bytes_written_per_launch": 4 * n, "launches": reps}}))
 
 
+def bench_batch(args, torch):
+    """Aggregate particle-steps/s of 512 independent BD replicas (N log-uniform in
+    [500, 20000], density 1e-4, config-0 table) stepped together, and of the same
+    replicas stepped one after another as separate Patches."""
+    import marsmd
+    from marsmd import configs
+    from marsmd.batch import Batch
+
+    specs = configs.config3()
+    total = sum(s.n for s in specs)
+    K = args.steps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # the default launch (eager, every workgroup that fits), then the autotuned one on a
+    # fresh batch: the autotuner's steps are the second batch's warmup
+    b = Batch(specs)
+    b.run(args.warmup)
+    torch.cuda.synchronize()
+    e0.record()
+    b.run(K, first_step=args.warmup)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_default = e0.elapsed_time(e1)
+    del b
+    b = Batch(specs)
+    tune_steps = max(args.warmup, 160)
+    choice = b.autotune(tune_steps, cache=False)
+    torch.cuda.synchronize()
+    b.set_kernel_timing(True)
+    e0.record()
+    b.run(K, first_step=tune_steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    kms, launches = b.kernel_timing()
+    # the same replicas as separate Patches, one after another on the same stream
+    patches = [marsmd.Patch(s) for s in specs]
+    for p in patches:
+        p.run(args.warmup)
+    torch.cuda.synchronize()
+    Kseq = min(K, 40)
+    e0.record()
+    for p in patches:
+        p.run(Kseq, first_step=args.warmup)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_seq = e0.elapsed_time(e1)
+    value = total * K / (ms / 1e3)
+    seq = total * Kseq / (ms_seq / 1e3)
+    print(json.dumps({
+        "metric": "aggregate particle-steps/s, 512 replicas", "value": value, "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": K, "warmup": tune_steps, "ms_per_step": ms / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded)",
+        "config": {"workload": "bd-512-replicas-500..20000-beads-density1e-4", "replicas": len(specs),
+                   "beads": total, "integrator": "brownian", "cutoff_A": 30.0, "skin_A": 5.0, "rebuild_period": 20},
+        "launch": {"blocks_per_cu": choice["blocks_per_cu"], "use_graph": choice["use_graph"],
+                   "tuned_ms_per_step": choice["ms_per_step"]},
+        "default_launch_ms_per_step": ms_default / K,
+        "step_kernel_avg_us": kms / max(launches, 1) * 1e3,
+        "sequential_patches": {"value": seq, "steps": Kseq, "ms_per_step": ms_seq / Kseq},
+        "speedup_over_sequential": value / seq}))
+
+
 def bd_frames(n: int, L: float, snapshots: int, seed: int = 42, warmup: int = 100, stride: int = 10):
     """Brownian-dynamics frames of the 4-type WCA+DH system (the force-matching
     workload's snapshots), in global-id order, from the GPU Patch."""
@@ -164,9 +226,10 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=15)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["langevin", "fm"], default="langevin",
-                    help="langevin: the per-step path at 1M beads; fm: force-matching gradient evaluations "
-                         "(64 BD snapshots x 100k beads, 10 tables x 512 knots)")
+    ap.add_argument("--workload", choices=["langevin", "batch", "fm"], default="langevin",
+                    help="langevin: the per-step path at 1M beads; batch: 512 BD replicas (500..20000 beads) "
+                         "stepped together; fm: force-matching gradient evaluations (64 BD snapshots x 100k "
+                         "beads, 10 tables x 512 knots)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a streaming kernel of known size (reads and writes 1 GiB each), the "
                          "scale for hardware byte counters")
@@ -184,6 +247,9 @@ def main():
         return
     if args.workload == "fm":
         bench_fm(args, torch)
+        return
+    if args.workload == "batch":
+        bench_batch(args, torch)
         return
 
     import marsmd

@@ -432,6 +432,70 @@ marsmd_status marsmd_patch_run(marsmd_patch* p, const marsmd_run_params* params,
 marsmd_status marsmd_patch_set_kernel_timing(marsmd_patch* p, int32_t enable);
 marsmd_status marsmd_patch_get_kernel_timing(marsmd_patch* p, double* total_ms, uint64_t* launches);
 
+/* ------------------------------------------------------------------ replica batch
+ * Many independent replicas of a periodic nonbonded system, each one the reference's
+ * Patch (Patch.h:48-645) driven by SimManager::run's per-step sequence
+ * (SimManager.cpp:271-296, 337-504), stepped together: one launch per step over every
+ * replica and one batched rebuild for all. Replica r's id enters its seed mix as the
+ * patch id (Patch.cpp:252-254). The replicas share particle types and pair tables and
+ * carry no bonded topology, exclusions or grids; each box is periodic on all axes. No
+ * counterpart in the reference, which steps patches one at a time. */
+typedef struct marsmd_batch marsmd_batch;
+
+typedef struct marsmd_batch_desc {
+    uint64_t num_replicas;
+    const uint64_t* num_particles; /* host, per replica                                   */
+    const marsmd_box* boxes;       /* host, per replica                                   */
+    const uint64_t* patch_ids;     /* host, per replica (seed-mix word); NULL: 0, 1, ...  */
+    marsmd_particle_types types;   /* shared by the replicas (no grid terms)             */
+    const uint32_t* term_matrix;   /* host, num_types^2 pair tags                        */
+    uint32_t enabled_terms;
+    int32_t num_nb_tables;
+    const marsmd_table* nb_tables; /* host records; .pot host pointers to knots          */
+} marsmd_batch_desc;
+
+marsmd_status marsmd_batch_create(const marsmd_batch_desc* desc, marsmd_batch** out);
+marsmd_status marsmd_batch_destroy(marsmd_batch* b);
+marsmd_status marsmd_batch_set_base_seed(marsmd_batch* b, uint64_t seed);
+/* Every replica's particles, replica after replica, each in its own canonical order
+ * (host arrays; mom may be NULL). Positions must lie inside their replica's box. */
+marsmd_status marsmd_batch_copy_particles_from_host(marsmd_batch* b, const float* pos4, const float* mom4,
+                                                    const int32_t* type_ids, uint64_t total);
+marsmd_status marsmd_batch_copy_particles_to_host(marsmd_batch* b, float* pos4, float* mom4, float* force_energy4,
+                                                  int32_t* type_ids, int32_t* ids, uint64_t total);
+/* Steps first_step+1 .. first_step+num_steps of every replica (marsmd_patch_run). */
+marsmd_status marsmd_batch_run(marsmd_batch* b, const marsmd_run_params* params, uint64_t first_step,
+                               uint64_t num_steps, marsmd_stream stream);
+/* Patch::finish_deferred_kick (Patch.cpp:354-373) of every replica. */
+marsmd_status marsmd_batch_finish_deferred_kick(marsmd_batch* b, float dt, marsmd_stream stream);
+/* The current pair set of one replica: int2 {x < y} in its canonical indices (device
+ * pointer, valid until the next call); *num_pairs on the host. */
+marsmd_status marsmd_batch_get_neighbor_pairs(marsmd_batch* b, uint64_t replica, const int32_t** pairs,
+                                              uint64_t* num_pairs, marsmd_stream stream);
+/* Launch timing of the batched step kernel (as marsmd_patch_set/get_kernel_timing):
+ * event pairs around each launch or replayed segment; *steps counts the steps they
+ * span. */
+marsmd_status marsmd_batch_set_kernel_timing(marsmd_batch* b, int32_t enable);
+marsmd_status marsmd_batch_get_kernel_timing(marsmd_batch* b, double* total_ms, uint64_t* steps);
+
+/* Launch configuration of the batched step (no counterpart in the reference). Results
+ * do not depend on it: each particle's step is one lane's work whatever the grid, and
+ * a replayed segment launches the same kernels with the same arguments. */
+typedef struct marsmd_batch_launch {
+    int32_t blocks_per_cu; /* persistent step-kernel workgroups per CU; 0: as many as fit */
+    int32_t use_graph;     /* replay the steps between host decisions (rebuild checks,
+                              reorders, energy steps, a run's last step) as hipGraphs */
+} marsmd_batch_launch;
+
+marsmd_status marsmd_batch_set_launch(marsmd_batch* b, const marsmd_batch_launch* launch);
+marsmd_status marsmd_batch_get_launch(marsmd_batch* b, marsmd_batch_launch* launch);
+/* Steps first_step+1 .. first_step+num_steps like marsmd_batch_run while timing every
+ * candidate launch configuration on runs of steps (two passes over the candidates),
+ * then keeps the fastest: *best is set to it and *best_ms_per_step to its time. */
+marsmd_status marsmd_batch_autotune(marsmd_batch* b, const marsmd_run_params* params, uint64_t first_step,
+                                    uint64_t num_steps, marsmd_batch_launch* best, double* best_ms_per_step,
+                                    marsmd_stream stream);
+
 /* ------------------------------------------------------------------ force matching
  * No counterpart in the reference, which has no gradient path: the forces of a fixed
  * set of configurations (snapshots) under tabulated pair potentials whose knots are

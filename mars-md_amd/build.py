@@ -18,7 +18,7 @@ OUT_DIR = HERE / "marsmd"
 LIB = OUT_DIR / "libmarsmd_hip.so"
 BUILD = HERE / "_build"
 
-SOURCES = ["sort.hip", "pairlist.hip", "launchers.hip", "patch.hip", "fm.hip", "capi.cpp"]
+SOURCES = ["sort.hip", "pairlist.hip", "launchers.hip", "patch.hip", "fm.hip", "batch.hip", "capi.cpp"]
 
 HIP_FLAGS = [
     "-O3",

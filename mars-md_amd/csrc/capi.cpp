@@ -1,5 +1,6 @@
 // capi.cpp — the extern "C" boundary declared in include/marsmd.h. Every entry
 // point converts exceptions into a marsmd_status and a thread-local message.
+#include "batch.hpp"
 #include "fm.hpp"
 #include "launchers.hpp"
 #include "pairlist.hpp"
@@ -443,6 +444,123 @@ marsmd_status marsmd_patch_get_kernel_timing(marsmd_patch* p, double* total_ms, 
 		need(total_ms, "total_ms");
 		need(launches, "launches");
 		p->impl.kernel_timing(total_ms, launches);
+	});
+}
+
+// ------------------------------------------------------------------ replica batch
+struct marsmd_batch {
+	mmd::BatchImpl impl;
+	explicit marsmd_batch(const marsmd_batch_desc& d) : impl(d) {}
+};
+
+marsmd_status marsmd_batch_create(const marsmd_batch_desc* desc, marsmd_batch** out) {
+	return guard([&] {
+		need(desc, "desc");
+		need(out, "out");
+		*out = nullptr;
+		*out = new marsmd_batch(*desc);
+	});
+}
+
+marsmd_status marsmd_batch_destroy(marsmd_batch* b) {
+	return guard([&] { delete b; });
+}
+
+marsmd_status marsmd_batch_set_base_seed(marsmd_batch* b, uint64_t seed) {
+	return guard([&] {
+		need(b, "batch");
+		b->impl.base_seed = seed;
+	});
+}
+
+marsmd_status marsmd_batch_copy_particles_from_host(marsmd_batch* b, const float* pos4, const float* mom4,
+													const int32_t* type_ids, uint64_t total) {
+	return guard([&] {
+		need(b, "batch");
+		need(pos4, "pos4");
+		need(type_ids, "type_ids");
+		b->impl.copy_from_host(pos4, mom4, type_ids, total);
+	});
+}
+
+marsmd_status marsmd_batch_copy_particles_to_host(marsmd_batch* b, float* pos4, float* mom4, float* force_energy4,
+												  int32_t* type_ids, int32_t* ids, uint64_t total) {
+	return guard([&] {
+		need(b, "batch");
+		b->impl.copy_to_host(pos4, mom4, force_energy4, type_ids, ids, total);
+	});
+}
+
+marsmd_status marsmd_batch_run(marsmd_batch* b, const marsmd_run_params* params, uint64_t first_step,
+							   uint64_t num_steps, marsmd_stream stream) {
+	return guard([&] {
+		need(b, "batch");
+		need(params, "params");
+		b->impl.run(*params, first_step, num_steps, st(stream));
+	});
+}
+
+marsmd_status marsmd_batch_finish_deferred_kick(marsmd_batch* b, float dt, marsmd_stream stream) {
+	return guard([&] {
+		need(b, "batch");
+		b->impl.finish_deferred_kick(dt, st(stream));
+	});
+}
+
+marsmd_status marsmd_batch_get_neighbor_pairs(marsmd_batch* b, uint64_t replica, const int32_t** pairs,
+											  uint64_t* num_pairs, marsmd_stream stream) {
+	return guard([&] {
+		need(b, "batch");
+		need(pairs, "pairs");
+		need(num_pairs, "num_pairs");
+		*pairs = b->impl.neighbor_pairs(replica, num_pairs, st(stream));
+	});
+}
+
+marsmd_status marsmd_batch_set_kernel_timing(marsmd_batch* b, int32_t enable) {
+	return guard([&] {
+		need(b, "batch");
+		b->impl.set_kernel_timing(enable != 0);
+	});
+}
+
+marsmd_status marsmd_batch_get_kernel_timing(marsmd_batch* b, double* total_ms, uint64_t* steps) {
+	return guard([&] {
+		need(b, "batch");
+		need(total_ms, "total_ms");
+		need(steps, "steps");
+		b->impl.kernel_timing(total_ms, steps);
+	});
+}
+
+marsmd_status marsmd_batch_set_launch(marsmd_batch* b, const marsmd_batch_launch* launch) {
+	return guard([&] {
+		need(b, "batch");
+		need(launch, "launch");
+		b->impl.set_launch(launch->blocks_per_cu, launch->use_graph != 0);
+	});
+}
+
+marsmd_status marsmd_batch_get_launch(marsmd_batch* b, marsmd_batch_launch* launch) {
+	return guard([&] {
+		need(b, "batch");
+		need(launch, "launch");
+		launch->blocks_per_cu = b->impl.blocks_per_cu;
+		launch->use_graph = b->impl.use_graph ? 1 : 0;
+	});
+}
+
+marsmd_status marsmd_batch_autotune(marsmd_batch* b, const marsmd_run_params* params, uint64_t first_step,
+									uint64_t num_steps, marsmd_batch_launch* best, double* best_ms_per_step,
+									marsmd_stream stream) {
+	return guard([&] {
+		need(b, "batch");
+		need(params, "params");
+		need(best, "best");
+		need(best_ms_per_step, "best_ms_per_step");
+		b->impl.autotune(*params, first_step, num_steps, st(stream), best_ms_per_step);
+		best->blocks_per_cu = b->impl.blocks_per_cu;
+		best->use_graph = b->impl.use_graph ? 1 : 0;
 	});
 }
 

@@ -45,6 +45,7 @@ __device__ __forceinline__ bool pair_predicate(const DevBox& box, float rc2, con
 // written (the host sees the count and walks again with a larger cap).
 constexpr int kHomeWaves = 4;
 constexpr int kSegMax = 448;
+constexpr int kSegSort = 512; // power of two >= kSegMax
 
 struct HomeParams {
 	const float4* spos;
@@ -63,7 +64,37 @@ struct HomeParams {
 	DevBox box;
 	marsmd_exclusion_view ex;
 	uint32_t cap;  // entries per slot the uniform tile rows hold
+	// When the reference cell grid misses pairs at the periodic seam, only the pairs its
+	// stencil reaches are kept: cells (quantised coordinate >> ref_shift) within
+	// ref_r per axis, wrapped on periodic axes; ref_full marks axes it enumerates whole.
+	int ref_filter;
+	int ref_shift;
+	int ref_cells;
+	int ref_r[3];
+	int ref_full[3];
+	float dom_lo[3], dom_hi[3];
 };
+
+// Whether the reference grid's stencil (BuildCellNeighborsKernel, ZOrderNeighbor.h:
+// 55-139) links the cells of a and b. The relation is symmetric, so it is the pair
+// set of ZOrderCellNeighborKernel whichever of the two emits.
+__device__ __forceinline__ bool in_reference_stencil(const HomeParams& P, const float4& a, const float4& b) {
+	const float pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+	bool clamped = false;
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		if (P.ref_full[k])
+			continue;
+		const int ca = int(morton_quantize(pa[k], P.dom_lo[k], P.dom_hi[k], clamped) >> P.ref_shift);
+		const int cb = int(morton_quantize(pb[k], P.dom_lo[k], P.dom_hi[k], clamped) >> P.ref_shift);
+		int d = ca > cb ? ca - cb : cb - ca;
+		if (P.box.periodic[k])
+			d = min(d, P.ref_cells - d);
+		if (d > P.ref_r[k])
+			return false;
+	}
+	return true;
+}
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
 	const int lane = threadIdx.x & 63;
@@ -123,7 +154,9 @@ __device__ __forceinline__ void home_test_segments(const HomeParams& P, const ui
 				const bool t_lo = ju > t; // operands selected, one predicate evaluation
 				const float4 plo = t_lo ? p : qu;
 				const float4 phi = t_lo ? qu : p;
-				const bool in = pair_predicate(P.box, P.rc2, P.ex, plo, phi);
+				bool in = pair_predicate(P.box, P.rc2, P.ex, plo, phi);
+				if (in && P.ref_filter)
+					in = in_reference_stencil(P, plo, phi);
 				if (in) {
 					if (FILL && cnt < P.cap)
 						nbr[nbr_slot_index(row, t, cnt)] = ju;
@@ -135,8 +168,60 @@ __device__ __forceinline__ void home_test_segments(const HomeParams& P, const ui
 	wave_lds_sync();
 }
 
+// Orders segments [0, nseg) (ntot candidates) by first slot. The candidates are then
+// tested in slot order, so each lane's row lists its neighbours in slot (Morton) order
+// and, at one row entry, the lanes of a wave read positions that lie close together.
+// Bitonic sort of {first slot, length} keys in LDS, then the prefix is rebuilt.
+__device__ __forceinline__ void sort_segments(uint32_t* seg_b, uint32_t* seg_pre, uint64_t* key, uint32_t nseg,
+											  uint32_t ntot) {
+	const uint32_t lane = threadIdx.x & 63u;
+	if (nseg < 2)
+		return;
+	uint32_t np2 = 2;
+	while (np2 < nseg)
+		np2 <<= 1;
+	for (uint32_t i = lane; i < np2; i += 64) {
+		uint64_t k = ~uint64_t(0);
+		if (i < nseg) {
+			const uint32_t end = i + 1 < nseg ? seg_pre[i + 1] : ntot;
+			k = (uint64_t(seg_b[i]) << 32) | uint64_t(end - seg_pre[i]);
+		}
+		key[i] = k;
+	}
+	wave_lds_sync();
+	for (uint32_t size = 2; size <= np2; size <<= 1)
+		for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+			for (uint32_t i = lane; i < np2; i += 64) {
+				const uint32_t x = i ^ stride;
+				if (x > i) {
+					const uint64_t a = key[i], b = key[x];
+					if ((a > b) == ((i & size) == 0)) {
+						key[i] = b;
+						key[x] = a;
+					}
+				}
+			}
+			wave_lds_sync();
+		}
+	const uint32_t per = (nseg + 63u) >> 6;
+	const uint32_t i0 = lane * per;
+	uint32_t local = 0;
+	for (uint32_t u = 0; u < per; ++u)
+		if (i0 + u < nseg)
+			local += uint32_t(key[i0 + u]);
+	uint32_t run = wave_inclusive_scan(local) - local;
+	for (uint32_t u = 0; u < per; ++u)
+		if (i0 + u < nseg) {
+			const uint64_t k = key[i0 + u];
+			seg_b[i0 + u] = uint32_t(k >> 32);
+			seg_pre[i0 + u] = run;
+			run += uint32_t(k);
+		}
+	wave_lds_sync();
+}
+
 template<bool FILL>
-__device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg_b, uint32_t* seg_pre,
+__device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg_b, uint32_t* seg_pre, uint64_t* seg_k,
 							   uint32_t* __restrict__ counts, uint32_t* __restrict__ nbr) {
 	const int lane = threadIdx.x & 63;
 	const uint32_t hb = P.home_begin[home], he = P.home_end[home];
@@ -202,6 +287,7 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 			const uint64_t mask = __ballot(c > 0);
 			const uint32_t nnew = uint32_t(__popcll(mask));
 			if (nseg + nnew > uint32_t(kSegMax)) {
+				sort_segments(seg_b, seg_pre, seg_k, nseg, ntot);
 				home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr);
 				nseg = 0;
 				ntot = 0;
@@ -216,6 +302,7 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 			nseg += nnew;
 			ntot += sum;
 		}
+		sort_segments(seg_b, seg_pre, seg_k, nseg, ntot);
 		home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr);
 		if (valid) {
 			counts[t] = cnt;
@@ -233,12 +320,13 @@ __global__ __launch_bounds__(256) void k_walk_home(HomeParams P, uint32_t* __res
 												   uint32_t* __restrict__ nbr) {
 	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
 	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
+	__shared__ uint64_t seg_k[kHomeWaves][kSegSort];
 	const int wv = threadIdx.x >> 6;
 	const TileRange R = xcd_tiles((P.num_home + kHomeWaves - 1) / kHomeWaves);
 	for (uint32_t g = R.tile; g < R.end; g += R.stride) {
 		const uint32_t home = g * kHomeWaves + uint32_t(wv);
 		if (home < P.num_home)
-			walk_home_cell<FILL>(P, home, seg_b[wv], seg_pre[wv], counts, nbr);
+			walk_home_cell<FILL>(P, home, seg_b[wv], seg_pre[wv], seg_k[wv], counts, nbr);
 	}
 }
 

@@ -711,6 +711,20 @@ bool PairlistImpl::walk_home(hipStream_t s, bool& outside) {
 	P.rc2 = cutoff_squared;
 	P.box = make_devbox(box);
 	P.ex = excl;
+	if (!ref_complete) {
+		// keep exactly the pairs the reference grid's stencil reaches
+		const int cells = 1 << ref_grid.m;
+		P.ref_filter = 1;
+		P.ref_shift = 10 - ref_grid.m;
+		P.ref_cells = cells;
+		for (int a = 0; a < 3; ++a) {
+			const bool per = box.periodic[a] != 0 && box.size[a] > 0.0f;
+			P.ref_r[a] = ref_grid.r[a];
+			P.ref_full[a] = (per && 2 * ref_grid.r[a] + 1 >= cells) ? 1 : 0;
+			P.dom_lo[a] = domain_min[a];
+			P.dom_hi[a] = domain_max[a];
+		}
+	}
 	if (home_cap == 0) {
 		// first build: the mean count of a uniform system, with room for its fluctuations
 		const double mean = double(n) / volume * 4.0 / 3.0 * 3.141592653589793 * double(cutoff) * cutoff * cutoff;
@@ -821,7 +835,7 @@ void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool 
 	bool use_ref = false;
 	home_level = -1;
 	bool home_done = false;
-	if (ref_complete) {
+	{
 		bool outside = false;
 		home_done = walk_home(s, outside);
 		if (outside) {

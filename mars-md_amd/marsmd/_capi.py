@@ -148,6 +148,24 @@ class PatchDesc(C.Structure):
     ]
 
 
+class BatchDesc(C.Structure):
+    _fields_ = [
+        ("num_replicas", C.c_uint64),
+        ("num_particles", C.c_void_p),
+        ("boxes", C.c_void_p),
+        ("patch_ids", C.c_void_p),
+        ("types", ParticleTypes),
+        ("term_matrix", C.c_void_p),
+        ("enabled_terms", C.c_uint32),
+        ("num_nb_tables", C.c_int32),
+        ("nb_tables", C.c_void_p),
+    ]
+
+
+class BatchLaunch(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int32), ("use_graph", C.c_int32)]
+
+
 class FMDesc(C.Structure):
     _fields_ = [
         ("box", Box),
@@ -240,6 +258,20 @@ PROTOTYPES = {
     "marsmd_patch_run": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, _P]),
     "marsmd_patch_set_kernel_timing": (_I32, [_P, _I32]),
     "marsmd_patch_get_kernel_timing": (_I32, [_P, C.POINTER(C.c_double), C.POINTER(_U64)]),
+    "marsmd_batch_create": (_I32, [C.POINTER(BatchDesc), C.POINTER(_P)]),
+    "marsmd_batch_destroy": (_I32, [_P]),
+    "marsmd_batch_set_base_seed": (_I32, [_P, _U64]),
+    "marsmd_batch_copy_particles_from_host": (_I32, [_P, _P, _P, _P, _U64]),
+    "marsmd_batch_copy_particles_to_host": (_I32, [_P, _P, _P, _P, _P, _P, _U64]),
+    "marsmd_batch_run": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, _P]),
+    "marsmd_batch_finish_deferred_kick": (_I32, [_P, C.c_float, _P]),
+    "marsmd_batch_get_neighbor_pairs": (_I32, [_P, _U64, C.POINTER(_P), C.POINTER(_U64), _P]),
+    "marsmd_batch_set_kernel_timing": (_I32, [_P, _I32]),
+    "marsmd_batch_get_kernel_timing": (_I32, [_P, C.POINTER(C.c_double), C.POINTER(_U64)]),
+    "marsmd_batch_set_launch": (_I32, [_P, C.POINTER(BatchLaunch)]),
+    "marsmd_batch_get_launch": (_I32, [_P, C.POINTER(BatchLaunch)]),
+    "marsmd_batch_autotune": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, C.POINTER(BatchLaunch),
+                                     C.POINTER(C.c_double), _P]),
     "marsmd_fm_create": (_I32, [C.POINTER(FMDesc), C.POINTER(_P)]),
     "marsmd_fm_destroy": (_I32, [_P]),
     "marsmd_fm_num_pairs": (_I32, [_P, C.POINTER(_U64)]),

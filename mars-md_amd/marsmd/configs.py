@@ -149,6 +149,25 @@ def config0(n: int = 10_000, L: float = 250.0, seed: int = 42, steps: int = 1000
     return spec
 
 
+def config3(replicas: int = 512, n_min: int = 500, n_max: int = 20_000, density: float = 1e-4,
+            seed: int = 42) -> List[SystemSpec]:
+    """Independent replicas of the config-0 system: N_r log-uniform in [n_min, n_max]
+    (seeded), L_r = (N_r / density)^(1/3), the config-0 table and BD; replica r has patch
+    id r (its seed-mix word) and positions of its own (uniform, rejection at 3 A); all
+    replicas share the base seed."""
+    u = rng.uniform(seed, rng.STREAM_POSITIONS ^ 0x3, replicas)[:, 0].astype(np.float64)
+    ns = np.round(np.exp(np.log(n_min) + u * (np.log(n_max) - np.log(n_min)))).astype(np.int64)
+    specs = []
+    for r, n in enumerate(ns):
+        L = float((n / density) ** (1.0 / 3.0))
+        spec = config0(n=int(n), L=L, seed=seed)
+        spec.positions = uniform_positions(int(n), L, seed + 7919 * (r + 1))
+        spec.patch_id = r
+        spec.name = f"replica{r}-bd-{n}-beads-L{L:.1f}"
+        specs.append(spec)
+    return specs
+
+
 def config1(chains: int = 200, beads: int = 500, L: float = 800.0, seed: int = 42, steps: int = 1000,
             dihedrals: bool = False, restraints: bool = False) -> SystemSpec:
     """Bead-spring polymers: random-walk chains (step r0 = 5 A, a bead is redrawn when it

@@ -54,6 +54,8 @@ STRUCTS = {
     "marsmd_patch_desc": api.PatchDesc,
     "marsmd_run_params": api.RunParams,
     "marsmd_fm_desc": api.FMDesc,
+    "marsmd_batch_desc": api.BatchDesc,
+    "marsmd_batch_launch": api.BatchLaunch,
 }
 
 

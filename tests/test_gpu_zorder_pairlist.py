@@ -6,7 +6,7 @@ import pytest
 import marsmd
 from marsmd import PeriodicBox, configs
 from oracle import pyoracle as orc
-from tests.conftest import sorted_pairs
+from tests.conftest import seam_pair_positions, sorted_pairs
 
 pytestmark = pytest.mark.gpu
 
@@ -252,3 +252,21 @@ def test_rebuild_keeps_pair_set_and_degrees(cuda):
         st = pl.get_statistics()
         assert st.max_neighbors_per_particle == deg.max()
         assert st.num_pairs == len(ref["pairs"])
+
+
+def test_incomplete_reference_grid_keeps_the_reference_pair_set(cuda):
+    # a pair buffer of 70000 selects 16 cells of 10.0006 A with r = 1 at rc = 10, whose
+    # stencil misses the planted pairs across the periodic seam: the list must miss them too
+    n, L, rc = 20000, 160.01, 10.0
+    pos = seam_pair_positions(n, L)
+    box = PeriodicBox((L, L, L))
+    pl = marsmd.ZOrderPairlist(70000, 1 << 62)
+    pl.set_periodic_box(box)
+    for rebuild in range(2):
+        pl.build_pairlist(_dev(cuda, pos), n, rc)
+        cuda.cuda.synchronize()
+        gp = sorted_pairs(pl.get_neighbor_pairs().cpu().numpy())
+        ref = orc.build_pairlist(pos, box, rc, max_pairs=70000)
+        assert ref["grid"][:2] == (4, (1, 1, 1))
+        assert not {(2 * i, 2 * i + 1) for i in range(120)} & {tuple(p) for p in gp.tolist()}
+        assert np.array_equal(gp, ref["pairs"].astype(np.int64)), f"build {rebuild}"

@@ -10,6 +10,7 @@ import pytest
 
 from marsmd import PeriodicBox, configs
 from oracle import pyoracle as orc
+from tests.conftest import seam_pair_positions
 
 
 def test_morton_closed_forms():
@@ -157,3 +158,40 @@ def test_config0_pair_count_density():
     rho = 3000 / 170.0 ** 3
     expect = 0.5 * 3000 * rho * 4.0 / 3.0 * np.pi * 35.0 ** 3
     assert abs(len(res["pairs"]) - expect) / expect < 0.05
+
+
+def _compact_by3(v):
+    v = v & 0x09249249
+    v = (v ^ (v >> 2)) & 0x030C30C3
+    v = (v ^ (v >> 4)) & 0x0300F00F
+    v = (v ^ (v >> 8)) & 0x030000FF
+    v = (v ^ (v >> 16)) & 0x000003FF
+    return v
+
+
+def reference_cells(pos, L, m):
+    codes = orc.morton_encode(pos, [0, 0, 0], [L, L, L])
+    return np.stack([_compact_by3(codes >> s) for s in (2, 1, 0)], axis=1).astype(np.int64) >> (10 - m), codes
+
+
+def test_incomplete_reference_grid_drops_exactly_the_seam_pairs():
+    # 16 cells of 10.0006 A per axis with r = 1 at rc = 10: the quantised top cell is
+    # short (63 of 64 quanta), so the +-1 stencil misses pairs across the periodic seam
+    # (BuildCellNeighborsKernel, ZOrderNeighbor.h:55-139). The pair set is then every
+    # within-cutoff pair whose cells are at most r apart per axis, wrapped.
+    n, L, rc = 20000, 160.01, 10.0
+    pos = seam_pair_positions(n, L)
+    box = PeriodicBox((L, L, L))
+    res = orc.build_pairlist(pos, box, rc, max_pairs=70000)
+    m, r, _ = res["grid"]
+    assert (m, r) == (4, (1, 1, 1))
+    brute = orc.build_pairlist(pos, box, rc, max_pairs=70000, brute_force=True)["pairs"]
+    cell, codes = reference_cells(pos, L, m)
+    assert np.array_equal(np.sort(codes), np.sort(res["codes"]))
+    d = np.abs(cell[brute[:, 0]] - cell[brute[:, 1]])
+    d = np.minimum(d, (1 << m) - d)
+    keep = (d <= 1).all(axis=1)
+    planted = {(2 * i, 2 * i + 1) for i in range(120)}
+    dropped = {tuple(p) for p in brute[~keep]}
+    assert planted <= dropped and len(dropped) < len(brute) // 100
+    assert np.array_equal(res["pairs"], brute[keep])
