@@ -52,18 +52,31 @@ def algorithmic_bytes(spec) -> dict:
 
 
 def cpu_baseline(spec, steps: int):
-    """The oracle (single-threaded C++ restatement of the reference functors) on the
-    same workload: one untimed step (pairlist build), then `steps` timed steps."""
+    """The oracle (C++ restatement of the reference functors) on the same workload: one
+    untimed step (the pairlist build), then `steps` timed steps on one thread and
+    `steps` more with OpenMP over the host's CPU share (per-particle loops split, pair
+    forces into per-thread buffers summed in thread order). All timed steps lie before
+    the first rebuild check (step 21), so no rebuild is timed."""
     from oracle import pyoracle
 
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+    threads = max(1, min(threads, 16))
+    steps = max(1, min(steps, 9))
     sim = pyoracle.Sim(spec)
     sim.run(1)
     t0 = time.perf_counter()
     sim.run(steps, first_step=1)
-    dt = time.perf_counter() - t0
-    return {"value": spec.n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps after the step-1 pairlist build, {spec.n} beads, same workload; "
-                      f"host {platform.processor() or platform.machine()} nproc={os.cpu_count()}"}
+    dt1 = time.perf_counter() - t0
+    sim.set_threads(threads)
+    t0 = time.perf_counter()
+    sim.run(steps, first_step=1 + steps)
+    dtn = time.perf_counter() - t0
+    host = f"host {platform.processor() or platform.machine()} nproc={os.cpu_count()}"
+    return {"value": spec.n * steps / dtn, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"steps {2 + steps}..{1 + 2 * steps} of {spec.n} beads (same workload) with {threads} OpenMP "
+                      f"threads, after the step-1 pairlist build; {host}",
+            "single_thread": {"value": spec.n * steps / dt1, "unit": "particle-steps/s", "cores": 1,
+                              "sample": f"steps 2..{1 + steps}, one thread"}}
 
 
 def copy_probe(torch, n: int = 1 << 28, reps: int = 3):
@@ -224,7 +237,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=15)
+    ap.add_argument("--cpu-steps", type=int, default=9)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["langevin", "batch", "fm"], default="langevin",
                     help="langevin: the per-step path at 1M beads; batch: 512 BD replicas (500..20000 beads) "

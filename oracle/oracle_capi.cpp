@@ -25,6 +25,7 @@
 #include "../include/marsmd.h"
 
 #include <cstdio>
+#include <omp.h>
 #include <map>
 #include <memory>
 #include <string>
@@ -388,6 +389,11 @@ struct Sim {
 	PairlistResult pl;
 	int interp = MARSMD_INTERP_LINEAR;
 	float efield[3] = {0, 0, 0};
+	// > 1: the OpenMP timing mode of the CPU baseline (per-particle loops split over
+	// threads, pair forces into per-thread buffers summed in thread order); the FP64
+	// force twin is not kept in that mode, and the FP32 pair sums round differently
+	int threads = 1;
+	std::vector<std::vector<V4f>> thread_force;
 	bool has_grids() const { return !grids.empty(); }
 };
 
@@ -426,7 +432,58 @@ V4f position_dependent_force(const Sim& s, const V4f& p, int type) {
 	return f;
 }
 
+// The pair terms of sim_pair_forces with one FP32 force buffer per thread, each thread
+// taking a contiguous share of the pairs; the buffers are summed in thread order, so
+// the result does not depend on scheduling.
+void sim_pair_forces_threads(Sim& s, float cutoff, bool compute_energy) {
+	const float cutoff2 = cutoff > 0.0f ? cutoff * cutoff : 0.0f;
+	const int T = s.types.num_types;
+	const int nt = s.threads;
+	s.thread_force.resize(size_t(nt));
+	const size_t P = s.pl.pairs.size();
+#pragma omp parallel num_threads(nt)
+	{
+		const int t = omp_get_thread_num();
+		std::vector<V4f>& f = s.thread_force[size_t(t)];
+		f.assign(s.n, V4f{});
+		const size_t b = P * size_t(t) / size_t(nt), e = P * size_t(t + 1) / size_t(nt);
+		for (size_t k = b; k < e; ++k) {
+			const int x = s.pl.pairs[k].first, y = s.pl.pairs[k].second;
+			const uint32_t tag = s.term_matrix[size_t(s.type[x] * T + s.type[y])];
+			if (!(tag & s.enabled_terms & MARSMD_PAIR_TERM_TABULATED))
+				continue;
+			const PairTerm<float> pt = pair_term<float>(s.box, s.pos[x], s.pos[y],
+														 s.nb_tables[tag >> MARSMD_PAIR_TABLE_SHIFT], cutoff2);
+			if (!pt.valid)
+				continue;
+			f[size_t(x)].x += -pt.force.x;
+			f[size_t(x)].y += -pt.force.y;
+			f[size_t(x)].z += -pt.force.z;
+			f[size_t(y)].x += pt.force.x;
+			f[size_t(y)].y += pt.force.y;
+			f[size_t(y)].z += pt.force.z;
+			if (compute_energy) {
+				f[size_t(x)].t += pt.half_energy;
+				f[size_t(y)].t += pt.half_energy;
+			}
+		}
+	}
+#pragma omp parallel for num_threads(nt)
+	for (size_t i = 0; i < s.n; ++i)
+		for (int t = 0; t < nt; ++t) {
+			const V4f& f = s.thread_force[size_t(t)][i];
+			s.force[i].x += f.x;
+			s.force[i].y += f.y;
+			s.force[i].z += f.z;
+			s.force[i].t += f.t;
+		}
+}
+
 void sim_pair_forces(Sim& s, float cutoff, bool compute_energy) {
+	if (s.threads > 1) {
+		sim_pair_forces_threads(s, cutoff, compute_energy);
+		return;
+	}
 	const float cutoff2 = cutoff > 0.0f ? cutoff * cutoff : 0.0f;
 	const int T = s.types.num_types;
 	for (const auto& pr : s.pl.pairs) {
@@ -672,7 +729,9 @@ void sim_integrate(Sim& s, float dt, float kT, int integrator, uint64_t step) {
 	const uint64_t base_seed = mix_seed(s.base_seed, s.patch_id, step);
 	const uint32_t base_ctr = 0;
 	const marsmd_particle_types& t = s.types;
+	const int nt = s.threads;
 	if (integrator == MARSMD_INTEGRATOR_BROWNIAN) {
+#pragma omp parallel for num_threads(nt) if (nt > 1)
 		for (size_t i = 0; i < s.n; ++i) { // BDIntegrate::operator() (BD.h:44-92)
 			V4f p = s.pos[i];
 			V4f f = s.force[i];
@@ -702,6 +761,7 @@ void sim_integrate(Sim& s, float dt, float kT, int integrator, uint64_t step) {
 	}
 	// Langevin (Patch.cpp:290-342): PMF, deferred closing kick, then B-A-O-A.
 	if (s.has_grids()) {
+#pragma omp parallel for num_threads(nt) if (nt > 1)
 		for (size_t i = 0; i < s.n; ++i) { // ComputePMFKernel (Pmf.h:131-151)
 			const V4f f = position_dependent_force(s, s.pos[i], s.type[i]);
 			s.force[i].x += f.x;
@@ -718,6 +778,7 @@ void sim_integrate(Sim& s, float dt, float kT, int integrator, uint64_t step) {
 	const float SQ = 2.046167337f;
 	const float half_dt = 0.5f * dt;
 	if (s.deferred_kick) { // BAOAB_LastUpdate (BAOAB.h:151-172)
+#pragma omp parallel for num_threads(nt) if (nt > 1)
 		for (size_t i = 0; i < s.n; ++i) {
 			s.mom[i].x += half_dt * s.force[i].x * FCF;
 			s.mom[i].y += half_dt * s.force[i].y * FCF;
@@ -726,6 +787,7 @@ void sim_integrate(Sim& s, float dt, float kT, int integrator, uint64_t step) {
 		s.deferred_kick = false;
 	}
 	s.deferred_kick = true;
+#pragma omp parallel for num_threads(nt) if (nt > 1)
 	for (size_t i = 0; i < s.n; ++i) { // BAOABIntegrate::operator() (BAOAB.h:46-120)
 		V4f p = s.pos[i], m = s.mom[i];
 		const V4f f = s.force[i];
@@ -1251,6 +1313,14 @@ int orc_sim_reorder_particles(void* h) {
 	return 0;
 }
 // One SimManager step sequence per step (SimManager.cpp:271-296, 337-504).
+// Threads for the timing mode of the CPU baseline (1: the serial checker).
+int orc_sim_set_threads(void* h, int32_t threads) {
+	if (threads < 1)
+		return fail("threads must be >= 1");
+	static_cast<Sim*>(h)->threads = threads;
+	return 0;
+}
+
 int orc_sim_run(void* h, const marsmd_run_params* prm, uint64_t first_step, uint64_t num_steps) {
 	Sim& s = *static_cast<Sim*>(h);
 	try {

@@ -69,6 +69,7 @@ def lib():
             "orc_sim_finish_deferred_kick": (I32, [P, F]),
             "orc_sim_reorder_particles": (I32, [P]),
             "orc_sim_run": (I32, [P, C.POINTER(api.RunParams), U64, U64]),
+            "orc_sim_set_threads": (I32, [P, I32]),
             "orc_sim_get_pairs": (C.c_int64, [P, P, U64]),
             "orc_sim_get_topology": (I32, [P, P, P, P]),
         }
@@ -419,6 +420,11 @@ class Sim:
 
     def reorder_particles(self):
         _check(lib().orc_sim_reorder_particles(self._h))
+
+    def set_threads(self, threads: int):
+        """OpenMP threads for timing (1: the serial checker). With more than one the
+        pair forces are summed per thread and the FP64 force twin is not kept."""
+        _check(lib().orc_sim_set_threads(self._h, int(threads)))
 
     def run(self, num_steps, first_step=0, params: Optional[RunParams] = None):
         p = (params or self.spec.params).to_c()

@@ -260,3 +260,18 @@ def test_baoab_equipartition():
     m = sim.copy_particles_to_host()["mom"][:, :3].astype(np.float64)
     want = kT * mass * 2.046167337 ** 2
     assert np.allclose((m ** 2).mean(axis=0), want, rtol=0.08)
+
+
+def test_threaded_timing_mode_matches_the_serial_oracle():
+    # the OpenMP mode used for the multi-core CPU baseline: same pairs, forces summed per
+    # thread (FP32 rounding only), per-particle integrator loops unchanged
+    spec = configs.config0(n=3000, L=170.0, seed=3)
+    serial, threaded = orc.Sim(spec), orc.Sim(spec)
+    threaded.set_threads(4)
+    serial.run(5)
+    threaded.run(5)
+    a, b = serial.copy_particles_to_host(), threaded.copy_particles_to_host()
+    assert np.array_equal(serial.get_pairs(), threaded.get_pairs())
+    assert np.abs(a["pos"][:, :3] - b["pos"][:, :3]).max() < 1e-3
+    fa, fb = a["force"][:, :3].astype(np.float64), b["force"][:, :3].astype(np.float64)
+    assert np.abs(fa - fb).max() <= 1e-5 * np.abs(fa).max()
