@@ -125,6 +125,8 @@ typedef struct marsmd_particle_types {
     int32_t num_types;
     float mass[MARSMD_MAX_TYPES];
     float charge[MARSMD_MAX_TYPES];
+    float radius[MARSMD_MAX_TYPES]; /* softcore pair term                        */
+    float eps[MARSMD_MAX_TYPES];    /* softcore pair term                        */
     float diffusion[MARSMD_MAX_TYPES][3];
     float trans_damping[MARSMD_MAX_TYPES][3];
     int32_t pmf_grid_offset[MARSMD_MAX_TYPES];
@@ -133,6 +135,23 @@ typedef struct marsmd_particle_types {
     int32_t force_grid_id[MARSMD_MAX_TYPES][3];
     float force_grid_scale[MARSMD_MAX_TYPES][3];
 } marsmd_particle_types;
+
+/* Constants of the analytical pair terms: SolventParams (NonBondedInteraction.h:121-127)
+ * for Debye-Hueckel and ONC, and GaussianPotential's members (Pairwise.h:53-57), which
+ * launch_pairwise_nonbonded leaves at their defaults. A NULL pointer wherever one is
+ * taken means these defaults: debye_length 10, dielectric 80, onck_kappa 0.1,
+ * onck_sz 80, onck_z 6.86, gaussian_amp 1, gaussian_sigma 1, gaussian_cutoff_squared 0
+ * (no cutoff of its own). */
+typedef struct marsmd_pair_params {
+    float debye_length;
+    float dielectric;
+    float onck_kappa;
+    float onck_sz;
+    float onck_z;
+    float gaussian_amp;
+    float gaussian_sigma;
+    float gaussian_cutoff_squared;
+} marsmd_pair_params;
 
 /* ParticleView (src/Objects/DeviceParticle.h:18-28). Vector3 fields are float4
  * records; flags and attached_rigid_body_id may be NULL. */
@@ -254,15 +273,26 @@ marsmd_status marsmd_launch_resolve_pair_tables(const int32_t* pairs, const int3
                                                 uint64_t num_types, uint32_t* pair_tag,
                                                 uint64_t num_pairs, marsmd_stream stream);
 /* launch_pairwise_nonbonded (NonBondedInteraction.h:262-296; PairNonbondedComputer
- * Pairwise.h:184-305). tables: device array of marsmd_table. pair_tag may be NULL
- * (every pair takes enabled_terms). Only the TABULATED term is evaluated on this
- * path; other enabled bits are rejected with MARSMD_ERR_VALUE. */
+ * Pairwise.h:184-305): every enabled term (Coulomb, Debye-Hueckel, ONC, Gaussian,
+ * softcore, tabulated) of each pair in one pass. tables: device array of marsmd_table
+ * (may be NULL without the tabulated term). pair_tag may be NULL (every pair takes
+ * enabled_terms). types: host record (charges, softcore radius and eps); params: the
+ * solvent constants (NULL: defaults). */
 marsmd_status marsmd_launch_pairwise_nonbonded(const int32_t* pairs, const marsmd_particle_view* view,
                                                const uint32_t* pair_tag,
-                                               const marsmd_table* tables, const marsmd_box* box,
-                                               int32_t get_energy, uint64_t num_pairs,
+                                               const marsmd_table* tables, const marsmd_particle_types* types,
+                                               const marsmd_box* box, int32_t get_energy, uint64_t num_pairs,
                                                float cutoff_squared, uint32_t enabled_terms,
-                                               marsmd_stream stream);
+                                               const marsmd_pair_params* params, marsmd_stream stream);
+/* launch_analytical_pair_nonbonded (AnalyticalPairKernels.h:21-118): the analytical
+ * terms only, all of their constants taken from params (NULL: defaults). pos4 and
+ * force_energy4 are float4 records; type_ids int32. */
+marsmd_status marsmd_launch_analytical_pair_nonbonded(const int32_t* pairs, const float* pos4, float* force_energy4,
+                                                      const int32_t* type_ids, const uint32_t* pair_tag,
+                                                      const marsmd_particle_types* types, const marsmd_box* box,
+                                                      uint32_t enabled_terms, const marsmd_pair_params* params,
+                                                      int32_t get_energy, uint64_t num_pairs, float cutoff_squared,
+                                                      marsmd_stream stream);
 /* launch_tabulated_bonds (src/Interactions/DeviceBondedInteraction.h:495-518;
  * TabulatedBondComputer BondComputer.h:136-192). bonds: int2 array. */
 marsmd_status marsmd_launch_tabulated_bonds(const int32_t* bonds, const float* pos4,
@@ -365,6 +395,8 @@ typedef struct marsmd_patch_desc {
     const int32_t* restraint_ids;     /* particle per restraint */
     const float* restraint_anchors;   /* 4 floats per restraint (x, y, z, unused) */
     const float* restraint_k;         /* spring constant per restraint */
+    /* constants of the analytical pair terms (NULL: defaults) */
+    const marsmd_pair_params* pair_params;
 } marsmd_patch_desc;
 
 marsmd_status marsmd_patch_create(const marsmd_patch_desc* desc, marsmd_patch** out);
@@ -452,6 +484,7 @@ typedef struct marsmd_batch_desc {
     uint32_t enabled_terms;
     int32_t num_nb_tables;
     const marsmd_table* nb_tables; /* host records; .pot host pointers to knots          */
+    const marsmd_pair_params* pair_params; /* analytical pair terms (NULL: defaults)     */
 } marsmd_batch_desc;
 
 marsmd_status marsmd_batch_create(const marsmd_batch_desc* desc, marsmd_batch** out);

@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void k_b_max_disp(const float4* __restrict__ c
 
 // The fused step over every replica's particles, persistent over XCD-local tiles: the
 // replica of a slot gives its box and its seed mix.
-template<int INTEG, bool ENERGY>
+template<int INTEG, bool ENERGY, bool ANALYTIC>
 __global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restrict__ pos, float4* __restrict__ pos_next,
 														  float4* __restrict__ mom, float4* __restrict__ force,
 														  uint64_t total, const uint32_t* __restrict__ rep_of_slot,
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restr
 		P.step = step;
 		P.seed = mix_seed(base_seed, G.pid, uint64_t(step));
 		const float4 p = pos[t];
-		float4 f = nb_force(uint32_t(t), p, pos, L, T, knots, lds_tag, P.box, P.cutoff2, ENERGY);
+		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, knots, lds_tag, P.box, P.cutoff2, ENERGY);
 		float4 pn;
 		if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
 			pn = bd_update(p, f, types, nullptr, P);
@@ -467,8 +467,7 @@ BatchImpl::BatchImpl(const marsmd_batch_desc& d) : tables(new NbTables{}) {
 	NbTables& T = *tables;
 	T.num_types = types.num_types;
 	T.enabled = d.enabled_terms;
-	if (d.enabled_terms & ~uint32_t(MARSMD_PAIR_TERM_TABULATED) & MARSMD_PAIR_TERM_MASK)
-		raise(MARSMD_ERR_VALUE, "batch: only the TABULATED pair term is on this path");
+	T.coef = make_pair_coeffs(types, d.pair_params);
 	for (int i = 0; i < types.num_types * types.num_types; ++i)
 		T.tag[i] = d.term_matrix ? d.term_matrix[i] : 0u;
 	if (d.num_nb_tables < 0 || d.num_nb_tables > kMaxTables)
@@ -879,9 +878,14 @@ void BatchImpl::launch_steps(const marsmd_run_params& P, uint64_t first, uint64_
 							 hipStream_t s) {
 	const bool bd = P.integrator == MARSMD_INTEGRATOR_BROWNIAN;
 	const bool energy = count == 1 && P.energy_period > 0 && first % P.energy_period == 0;
+	const bool an = (tables->enabled & kAnalyticTerms) != 0;
 	const StepKernel kern =
-		bd ? (energy ? k_b_step<MARSMD_INTEGRATOR_BROWNIAN, true> : k_b_step<MARSMD_INTEGRATOR_BROWNIAN, false>)
-		   : (energy ? k_b_step<MARSMD_INTEGRATOR_LANGEVIN, true> : k_b_step<MARSMD_INTEGRATOR_LANGEVIN, false>);
+		bd ? (energy ? (an ? k_b_step<MARSMD_INTEGRATOR_BROWNIAN, true, true> : k_b_step<MARSMD_INTEGRATOR_BROWNIAN, true, false>)
+					 : (an ? k_b_step<MARSMD_INTEGRATOR_BROWNIAN, false, true>
+						   : k_b_step<MARSMD_INTEGRATOR_BROWNIAN, false, false>))
+		   : (energy ? (an ? k_b_step<MARSMD_INTEGRATOR_LANGEVIN, true, true> : k_b_step<MARSMD_INTEGRATOR_LANGEVIN, true, false>)
+					 : (an ? k_b_step<MARSMD_INTEGRATOR_LANGEVIN, false, true>
+						   : k_b_step<MARSMD_INTEGRATOR_LANGEVIN, false, false>));
 	const unsigned grid = step_grid(reinterpret_cast<const void*>(kern));
 	const unsigned lds = tables->lds ? unsigned(tables->total_knots) * 4u : 0u;
 	StepParams S{};
@@ -974,7 +978,7 @@ void BatchImpl::autotune(const marsmd_run_params& P, uint64_t first_step, uint64
 	// candidates: workgroups per CU from the occupancy down, each with and without
 	// segment replay; results do not depend on them (each particle's step is one lane's
 	// work, and a replay launches the same kernels with the same arguments)
-	(void)step_grid(reinterpret_cast<const void*>(k_b_step<MARSMD_INTEGRATOR_BROWNIAN, false>));
+	(void)step_grid(reinterpret_cast<const void*>(k_b_step<MARSMD_INTEGRATOR_BROWNIAN, false, false>));
 	std::vector<int> per_cu = {occupancy};
 	for (int v : {occupancy - 1, occupancy / 2, 1})
 		if (v >= 1 && std::find(per_cu.begin(), per_cu.end(), v) == per_cu.end())

@@ -236,13 +236,35 @@ marsmd_status marsmd_launch_resolve_pair_tables(const int32_t* pairs, const int3
 }
 marsmd_status marsmd_launch_pairwise_nonbonded(const int32_t* pairs, const marsmd_particle_view* view,
 											   const uint32_t* pair_tag, const marsmd_table* tables,
-											   const marsmd_box* box, int32_t get_energy, uint64_t num_pairs,
-											   float cutoff_squared, uint32_t enabled_terms, marsmd_stream stream) {
+											   const marsmd_particle_types* types, const marsmd_box* box,
+											   int32_t get_energy, uint64_t num_pairs, float cutoff_squared,
+											   uint32_t enabled_terms, const marsmd_pair_params* params,
+											   marsmd_stream stream) {
 	return guard([&] {
 		need(view, "view");
 		need(box, "box");
-		mmd::launch_pairwise_nonbonded(pairs, *view, pair_tag, tables, *box, get_energy != 0, num_pairs,
-									   cutoff_squared, enabled_terms, st(stream));
+		mmd::launch_pairwise_nonbonded(pairs, *view, pair_tag, tables, types, *box, get_energy != 0, num_pairs,
+									   cutoff_squared, enabled_terms, params, st(stream));
+	});
+}
+marsmd_status marsmd_launch_analytical_pair_nonbonded(const int32_t* pairs, const float* pos4, float* force_energy4,
+													  const int32_t* type_ids, const uint32_t* pair_tag,
+													  const marsmd_particle_types* types, const marsmd_box* box,
+													  uint32_t enabled_terms, const marsmd_pair_params* params,
+													  int32_t get_energy, uint64_t num_pairs, float cutoff_squared,
+													  marsmd_stream stream) {
+	return guard([&] {
+		need(types, "types");
+		need(box, "box");
+		if (num_pairs) {
+			need(pairs, "pairs");
+			need(pos4, "pos4");
+			need(force_energy4, "force_energy4");
+			need(type_ids, "type_ids");
+		}
+		mmd::launch_analytical_pair_nonbonded(pairs, pos4, force_energy4, type_ids, pair_tag, *types, *box,
+											  enabled_terms, params, get_energy != 0, num_pairs, cutoff_squared,
+											  st(stream));
 	});
 }
 marsmd_status marsmd_launch_tabulated_bonds(const int32_t* bonds, const float* pos4, float* fe,

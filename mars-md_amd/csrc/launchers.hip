@@ -7,6 +7,8 @@
 // replaces them with the fused, atomic-free per-particle kernel.
 #include "launchers.hpp"
 
+#include "analytic.hpp"
+
 namespace mmd {
 
 namespace {
@@ -29,20 +31,25 @@ __device__ __forceinline__ void atomic_add4(float4* p, float x, float y, float z
 	atomic_add_f32(f + 2, z);
 }
 
-// PairNonbondedComputer::operator() tabulated branch (Pairwise.h:214-304).
+// PairNonbondedComputer::operator() (Pairwise.h:214-304) with ANALYTIC / TABULATED
+// selecting its branches; AnalyticalPairKernel::operator() (AnalyticalPairKernels.h:
+// 44-104) is the ANALYTIC-only instance. The pair's own (x, y) orientation is kept.
+template<bool ANALYTIC, bool TABULATED>
 __global__ void k_pairwise(const int2* __restrict__ pairs, const float4* __restrict__ pos, float4* fe,
-						   const uint32_t* __restrict__ tags, const marsmd_table* __restrict__ tables,
-						   DevBox box, int get_energy, uint64_t np, float cutoff2, uint32_t enabled) {
+						   const int32_t* __restrict__ type_ids, const uint32_t* __restrict__ tags,
+						   const marsmd_table* __restrict__ tables, PairCoeffs C, DevBox box, int get_energy,
+						   uint64_t np, float cutoff2, uint32_t enabled) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= np)
 		return;
 	const uint32_t tag = tags ? tags[i] : enabled;
-	const uint32_t terms = tag & enabled;
-	if (!(terms & MARSMD_PAIR_TERM_TABULATED))
+	const uint32_t terms = tag & enabled & (TABULATED ? MARSMD_PAIR_TERM_MASK : kAnalyticTerms);
+	if (terms == 0u)
 		return;
 	const int2 ij = pairs[i];
 	const float3 r = wrap_diff(box, sub3(pos[ij.y], pos[ij.x]));
-	const float d = sqrtf(len2(r));
+	const float r2 = len2(r);
+	const float d = sqrtf(r2);
 	float3 u = make_float3(0.0f, 0.0f, 0.0f);
 	if (d > 1e-6f) {
 		const float inv = 1.0f / d;
@@ -52,10 +59,15 @@ __global__ void k_pairwise(const int2* __restrict__ pairs, const float4* __restr
 		return;
 	if (cutoff2 > 0.0f && d * d > cutoff2)
 		return;
-	const marsmd_table t = tables[tag >> MARSMD_PAIR_TABLE_SHIFT];
-	const FE v = table_compute(d, t.pot, int(t.size), t.start, t.step_inv, t.is_periodic != 0);
-	const float fm = 0.0f + v.f;
-	const float en = 0.0f + v.e;
+	float fm = 0.0f, en = 0.0f;
+	if (ANALYTIC && (terms & kAnalyticTerms))
+		analytic_terms(terms, d, r2, uint32_t(type_ids[ij.x]), uint32_t(type_ids[ij.y]), C, fm, en);
+	if (TABULATED && (terms & MARSMD_PAIR_TERM_TABULATED)) {
+		const marsmd_table t = tables[tag >> MARSMD_PAIR_TABLE_SHIFT];
+		const FE v = table_compute(d, t.pot, int(t.size), t.start, t.step_inv, t.is_periodic != 0);
+		fm += v.f;
+		en += v.e;
+	}
 	const float fx = fm * u.x, fy = fm * u.y, fz = fm * u.z;
 	atomic_add4(&fe[ij.x], -fx, -fy, -fz);
 	atomic_add4(&fe[ij.y], fx, fy, fz);
@@ -389,16 +401,34 @@ void launch_resolve_pair_tables(const int32_t* pairs, const int32_t* types, cons
 }
 
 void launch_pairwise_nonbonded(const int32_t* pairs, const marsmd_particle_view& v, const uint32_t* tags,
-							   const marsmd_table* tables, const marsmd_box& box, bool energy, uint64_t np,
-							   float cutoff2, uint32_t enabled, hipStream_t s) {
-	if (enabled & ~uint32_t(MARSMD_PAIR_TERM_TABULATED) & MARSMD_PAIR_TERM_MASK)
-		raise(MARSMD_ERR_VALUE, "launch_pairwise_nonbonded: only the TABULATED pair term is on this path");
-	if (np == 0)
+							   const marsmd_table* tables, const marsmd_particle_types* types, const marsmd_box& box,
+							   bool energy, uint64_t np, float cutoff2, uint32_t enabled,
+							   const marsmd_pair_params* params, hipStream_t s) {
+	if (np == 0 || (enabled & MARSMD_PAIR_TERM_MASK) == 0u)
 		return;
-	k_pairwise<<<grid_for(np, 256), 256, 0, s>>>(reinterpret_cast<const int2*>(pairs),
-												  reinterpret_cast<const float4*>(v.pos),
-												  reinterpret_cast<float4*>(v.force_energy), tags, tables,
-												  make_devbox(box), energy ? 1 : 0, np, cutoff2, enabled);
+	const bool an = (enabled & kAnalyticTerms) != 0;
+	if (an && (!types || !v.type_id))
+		raise(MARSMD_ERR_VALUE, "launch_pairwise_nonbonded: the analytical terms need the particle types and type ids");
+	if ((enabled & MARSMD_PAIR_TERM_TABULATED) && !tables)
+		raise(MARSMD_ERR_VALUE, "launch_pairwise_nonbonded: the tabulated term needs the tables");
+	const PairCoeffs C = types ? make_pair_coeffs(*types, params) : PairCoeffs{};
+	auto kern = an ? k_pairwise<true, true> : k_pairwise<false, true>;
+	kern<<<grid_for(np, 256), 256, 0, s>>>(reinterpret_cast<const int2*>(pairs), reinterpret_cast<const float4*>(v.pos),
+										   reinterpret_cast<float4*>(v.force_energy), v.type_id, tags, tables, C,
+										   make_devbox(box), energy ? 1 : 0, np, cutoff2, enabled);
+	MMD_LAUNCH_CHECK();
+}
+
+void launch_analytical_pair_nonbonded(const int32_t* pairs, const float* pos, float* fe, const int32_t* type_ids,
+									  const uint32_t* tags, const marsmd_particle_types& types, const marsmd_box& box,
+									  uint32_t enabled, const marsmd_pair_params* params, bool energy, uint64_t np,
+									  float cutoff2, hipStream_t s) {
+	if (np == 0 || (enabled & kAnalyticTerms) == 0u)
+		return;
+	k_pairwise<true, false><<<grid_for(np, 256), 256, 0, s>>>(
+		reinterpret_cast<const int2*>(pairs), reinterpret_cast<const float4*>(pos), reinterpret_cast<float4*>(fe),
+		type_ids, tags, nullptr, make_pair_coeffs(types, params), make_devbox(box), energy ? 1 : 0, np, cutoff2,
+		enabled);
 	MMD_LAUNCH_CHECK();
 }
 

@@ -8,8 +8,13 @@ namespace mmd {
 void launch_resolve_pair_tables(const int32_t* pairs, const int32_t* types, const uint32_t* matrix, uint64_t nt,
 								uint32_t* tag, uint64_t np, hipStream_t s);
 void launch_pairwise_nonbonded(const int32_t* pairs, const marsmd_particle_view& v, const uint32_t* tags,
-							   const marsmd_table* tables, const marsmd_box& box, bool energy, uint64_t np,
-							   float cutoff2, uint32_t enabled, hipStream_t s);
+							   const marsmd_table* tables, const marsmd_particle_types* types, const marsmd_box& box,
+							   bool energy, uint64_t np, float cutoff2, uint32_t enabled,
+							   const marsmd_pair_params* params, hipStream_t s);
+void launch_analytical_pair_nonbonded(const int32_t* pairs, const float* pos, float* fe, const int32_t* type_ids,
+									  const uint32_t* tags, const marsmd_particle_types& types, const marsmd_box& box,
+									  uint32_t enabled, const marsmd_pair_params* params, bool energy, uint64_t np,
+									  float cutoff2, hipStream_t s);
 void launch_tabulated_bonds(const int32_t* bonds, const float* pos, float* fe, const marsmd_table* tables,
 							const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
 							uint64_t nb, hipStream_t s);

@@ -1,7 +1,9 @@
 // nbforce.hpp — the nonbonded pair term over the wave-tiled neighbour list, shared by
-// the Patch kernels (patch.hip) and force matching (fm.hip).
+// the Patch kernels (patch.hip), the replica batch (batch.hip) and force matching
+// (fm.hip).
 #pragma once
 
+#include "analytic.hpp"
 #include "pairlist.hpp"
 
 namespace mmd {
@@ -20,12 +22,17 @@ struct NbTables {
 	int size[kMaxTables];
 	int offset[kMaxTables];
 	uint32_t tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+	PairCoeffs coef; // read by the analytical terms only
 };
 
 __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
 
 // One pair term on slot t (PairNonbondedComputer, Pairwise.h:184-305) in canonical
 // orientation: r = wrap_diff(pos[higher canon] - pos[lower canon]), the lower takes -F.
+// ANALYTIC adds the analytical terms enabled by the pair's tag; without it only the
+// tabulated term is evaluated (the kernels are instantiated per case, so the
+// tabulated-only path carries none of the analytical code).
+template<bool ANALYTIC>
 __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t tt, const float4& q,
 										const NbTables& T, const float* __restrict__ knots, const uint32_t* tagm,
 										const DevBox& box, float cutoff2, bool energy, float& fx, float& fy,
@@ -39,10 +46,12 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 	const uint32_t tx = t_is_x ? tt : tj;
 	const uint32_t ty = t_is_x ? tj : tt;
 	const uint32_t tag = tagm[tx * uint32_t(T.num_types) + ty];
-	if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
+	const uint32_t terms = tag & T.enabled;
+	if (ANALYTIC ? terms == 0u : !(terms & MARSMD_PAIR_TERM_TABULATED))
 		return;
 	const float3 r = wrap_diff(box, sub3(py, px));
-	const float d = sqrtf(len2(r));
+	const float r2 = len2(r);
+	const float d = sqrtf(r2);
 	float ux = 0.0f, uy = 0.0f, uz = 0.0f;
 	if (d > 1e-6f) {
 		const float inv = 1.0f / d;
@@ -54,9 +63,15 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		return;
 	if (cutoff2 > 0.0f && d * d > cutoff2)
 		return;
-	const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
-	const FE v = table_compute(d, knots + T.offset[ti], T.size[ti], T.start[ti], T.step_inv[ti], false);
-	const float fm = 0.0f + v.f;
+	float fm = 0.0f, en = 0.0f;
+	if (ANALYTIC && (terms & kAnalyticTerms))
+		analytic_terms(terms, d, r2, tx, ty, T.coef, fm, en);
+	if (terms & MARSMD_PAIR_TERM_TABULATED) {
+		const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
+		const FE v = table_compute(d, knots + T.offset[ti], T.size[ti], T.start[ti], T.step_inv[ti], false);
+		fm += v.f;
+		en += v.e;
+	}
 	const float gx = fm * ux, gy = fm * uy, gz = fm * uz;
 	if (t_is_x) {
 		fx += -gx;
@@ -68,13 +83,14 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		fz += gz;
 	}
 	if (energy)
-		e += (0.0f + v.e) * 0.5f;
+		e += en * 0.5f;
 }
 
 // Sum of the nonbonded pair terms on slot t over its neighbour list, in list order.
 // A lane's list is read in groups of 4 indices (one 16-byte load) and pipelined three
 // deep: while group g is evaluated, the positions of group g+1 and the indices of
 // group g+2 are in flight. The padding entries of a row are t itself (skipped as d = 0).
+template<bool ANALYTIC = false>
 __device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const float4* __restrict__ pos,
 										   const NbrList& L, const NbTables& T, const float* __restrict__ knots,
 										   const uint32_t* tagm, const DevBox& box, float cutoff2, bool energy) {
@@ -100,10 +116,10 @@ __device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const fl
 			if (g + 2 < ng)
 				jnn = row[uint64_t(g + 2) * 64u];
 		}
-		nb_pair(p, ct, tt, q0, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair(p, ct, tt, q1, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair(p, ct, tt, q2, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair(p, ct, tt, q3, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, q0, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, q1, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, q2, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, q3, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
 		q0 = n0;
 		q1 = n1;
 		q2 = n2;

@@ -39,6 +39,7 @@ namespace {
 constexpr int kBlock = 256;
 
 
+template<bool ANALYTIC>
 __global__ __launch_bounds__(kBlock) void k_nb_forces(const float4* __restrict__ pos, uint64_t n, NbrList L,
 													  NbTables T, const float* __restrict__ knots_g, DevBox box,
 													  float cutoff2, int energy, float4* __restrict__ force) {
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_forces(const float4* __restrict__
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
-	force[t] = nb_force(uint32_t(t), pos[t], pos, L, T, knots, lds_tag, box, cutoff2, energy != 0);
+	force[t] = nb_force<ANALYTIC>(uint32_t(t), pos[t], pos, L, T, knots, lds_tag, box, cutoff2, energy != 0);
 }
 
 __global__ void k_zero_force(float4* __restrict__ f, uint64_t n) {
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_baoab(float4* __restrict__
 // persistent over XCD-local tiles so the table knots are staged once per workgroup.
 // Bounded to 128 VGPRs (4 waves per SIMD; the tricubic sampler alone would take ~140
 // and leave 3): measured faster despite a few dwords of spill.
-template<int INTEG, bool ENERGY>
+template<int INTEG, bool ENERGY, bool ANALYTIC>
 __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ pos, float4* __restrict__ pos_next,
 												 float4* __restrict__ mom, float4* __restrict__ force, uint64_t n,
 												 NbrList L, NbTables T, const float* __restrict__ knots_g,
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 		if (t >= n)
 			continue;
 		const float4 p = pos[t];
-		float4 f = nb_force(uint32_t(t), p, pos, L, T, knots, lds_tag, S.box, S.cutoff2, ENERGY);
+		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, knots, lds_tag, S.box, S.cutoff2, ENERGY);
 		if (has_bonded) {
 			const float4 b = force[t];
 			f.x += b.x;
@@ -291,8 +292,7 @@ PatchImpl::PatchImpl(const marsmd_patch_desc& d) : tables(new Tables) {
 	NbTables& T = tables->nb;
 	T.num_types = types.num_types;
 	T.enabled = d.enabled_terms;
-	if (d.enabled_terms & ~uint32_t(MARSMD_PAIR_TERM_TABULATED) & MARSMD_PAIR_TERM_MASK)
-		raise(MARSMD_ERR_VALUE, "Patch: only the TABULATED pair term is on this path");
+	T.coef = make_pair_coeffs(types, d.pair_params);
 	const int nt2 = types.num_types * types.num_types;
 	for (int i = 0; i < nt2; ++i)
 		T.tag[i] = d.term_matrix ? d.term_matrix[i] : 0u;
@@ -656,9 +656,10 @@ void PatchImpl::calculate_nonbonded(float pairlist_cutoff, float cutoff, uint64_
 	interpolation = interp;
 	rebuild_policy(pairlist_cutoff, cutoff, step, period, s);
 	const float cutoff2 = cutoff > 0.0f ? cutoff * cutoff : 0.0f;
-	k_nb_forces<<<grid_for(n, kBlock), kBlock, lds_bytes(), s>>>(pos[cur].data(), n, pl->list(), tables->nb,
-																 tables->nb_knots.data(), make_devbox(box), cutoff2,
-																 energy ? 1 : 0, force.data());
+	auto kern = (tables->nb.enabled & kAnalyticTerms) ? k_nb_forces<true> : k_nb_forces<false>;
+	kern<<<grid_for(n, kBlock), kBlock, lds_bytes(), s>>>(pos[cur].data(), n, pl->list(), tables->nb,
+														  tables->nb_knots.data(), make_devbox(box), cutoff2,
+														  energy ? 1 : 0, force.data());
 	MMD_LAUNCH_CHECK();
 }
 
@@ -816,8 +817,15 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 		const int store = (step == first_step + num_steps) ? 1 : 0;
 		float4* pnext = pos[1 - cur].data();
 		const bool bd = P.integrator == MARSMD_INTEGRATOR_BROWNIAN;
-		auto kern = bd ? (energy ? k_step<MARSMD_INTEGRATOR_BROWNIAN, true> : k_step<MARSMD_INTEGRATOR_BROWNIAN, false>)
-					   : (energy ? k_step<MARSMD_INTEGRATOR_LANGEVIN, true> : k_step<MARSMD_INTEGRATOR_LANGEVIN, false>);
+		const bool an = (tables->nb.enabled & kAnalyticTerms) != 0;
+		auto kern = bd ? (energy ? (an ? k_step<MARSMD_INTEGRATOR_BROWNIAN, true, true>
+									   : k_step<MARSMD_INTEGRATOR_BROWNIAN, true, false>)
+								 : (an ? k_step<MARSMD_INTEGRATOR_BROWNIAN, false, true>
+									   : k_step<MARSMD_INTEGRATOR_BROWNIAN, false, false>))
+					   : (energy ? (an ? k_step<MARSMD_INTEGRATOR_LANGEVIN, true, true>
+									   : k_step<MARSMD_INTEGRATOR_LANGEVIN, true, false>)
+								 : (an ? k_step<MARSMD_INTEGRATOR_LANGEVIN, false, true>
+									   : k_step<MARSMD_INTEGRATOR_LANGEVIN, false, false>));
 		const unsigned grid = persistent_grid(reinterpret_cast<const void*>(kern), n);
 		if (timing) {
 			while (timing_events.size() < timing_used + 2) {

@@ -280,13 +280,34 @@ inline Event launch_resolve_pair_tables(const int2* pairs, const int* type_ids, 
 	return Event(stream);
 }
 
+// SolventParams (NonBondedInteraction.h:121-127) with the Gaussian term's constants;
+// the defaults are the reference's.
+struct SolventParams : marsmd_pair_params {
+	SolventParams() : marsmd_pair_params{10.0f, 80.0f, 0.1f, 80.0f, 6.86f, 1.0f, 1.0f, 0.0f} {}
+};
+
 inline Event launch_pairwise_nonbonded(const int2* pairs, const ParticleView& particles, const uint32_t* pair_tag,
-									   const TabulatedPotential* tables, const PeriodicBox& box, bool get_energy,
-									   size_t num_pairs, float cutoff_squared, uint32_t enabled_terms,
+									   const TabulatedPotential* tables, const ParticleTypeView& types,
+									   const PeriodicBox& box, bool get_energy, size_t num_pairs, float cutoff_squared,
+									   uint32_t enabled_terms, const SolventParams& solvent = SolventParams(),
 									   hipStream_t stream = nullptr) {
 	detail::check(marsmd_launch_pairwise_nonbonded(reinterpret_cast<const int32_t*>(pairs), &particles, pair_tag,
-												   tables, &box.c(), get_energy ? 1 : 0, num_pairs, cutoff_squared,
-												   enabled_terms, detail::as_stream(stream)));
+												   tables, &types, &box.c(), get_energy ? 1 : 0, num_pairs,
+												   cutoff_squared, enabled_terms, &solvent,
+												   detail::as_stream(stream)));
+	return Event(stream);
+}
+
+// launch_analytical_pair_nonbonded (AnalyticalPairKernels.h:110-118) over caller arrays.
+inline Event launch_analytical_pair_nonbonded(const int2* pairs, const Vector3* pos, Vector3* force_energy,
+											  const int* type_ids, const uint32_t* pair_tag,
+											  const ParticleTypeView& types, const PeriodicBox& box,
+											  uint32_t enabled_terms, const SolventParams& params, bool get_energy,
+											  size_t num_pairs, float cutoff_squared, hipStream_t stream = nullptr) {
+	detail::check(marsmd_launch_analytical_pair_nonbonded(
+		reinterpret_cast<const int32_t*>(pairs), reinterpret_cast<const float*>(pos),
+		reinterpret_cast<float*>(force_energy), type_ids, pair_tag, &types, &box.c(), enabled_terms, &params,
+		get_energy ? 1 : 0, num_pairs, cutoff_squared, detail::as_stream(stream)));
 	return Event(stream);
 }
 
@@ -362,6 +383,12 @@ struct PatchSetup {
 	std::vector<TabulatedPotential> bond_tables, angle_tables;
 	std::vector<int32_t> exclusions;           // 2 indices per exclusion
 	std::vector<GridView> grids;               // .data: host values
+	std::vector<int32_t> dihedrals, dihedral_table; // 4 indices per dihedral
+	std::vector<TabulatedPotential> dihedral_tables; // periodic, radians
+	std::vector<int32_t> restraint_ids;        // particle per harmonic restraint
+	std::vector<float> restraint_anchors;      // 4 floats per restraint
+	std::vector<float> restraint_k;
+	SolventParams solvent;                     // analytical pair terms
 };
 
 class Patch {
@@ -393,6 +420,16 @@ class Patch {
 		d.exclusions = setup.exclusions.data();
 		d.num_grids = int32_t(setup.grids.size());
 		d.grids = setup.grids.data();
+		d.num_dihedrals = setup.dihedrals.size() / 4;
+		d.dihedrals = setup.dihedrals.data();
+		d.dihedral_table = setup.dihedral_table.empty() ? nullptr : setup.dihedral_table.data();
+		d.num_dihedral_tables = int32_t(setup.dihedral_tables.size());
+		d.dihedral_tables = setup.dihedral_tables.data();
+		d.num_restraints = setup.restraint_ids.size();
+		d.restraint_ids = setup.restraint_ids.data();
+		d.restraint_anchors = setup.restraint_anchors.data();
+		d.restraint_k = setup.restraint_k.data();
+		d.pair_params = &setup.solvent;
 		detail::check(marsmd_patch_create(&d, &p_));
 	}
 	~Patch() {

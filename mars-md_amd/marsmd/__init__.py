@@ -9,6 +9,7 @@ from .core import (
     DeviceTables,
     ExclusionCSR,
     Grid,
+    PairParams,
     ParticleType,
     PeriodicBox,
     TabulatedPotential,
@@ -21,6 +22,7 @@ from .core import (
     launch_BAOAB_LastUpdate,
     launch_BD,
     launch_PMF,
+    launch_analytical_pair_nonbonded,
     launch_pairwise_nonbonded,
     launch_resolve_pair_tables,
     launch_tabulated_angles,
@@ -36,10 +38,10 @@ from .patch import Patch
 from .system import RunParams, SystemSpec
 
 __all__ = [
-    "DeviceTables", "ExclusionCSR", "Grid", "MarsError", "ParticleType", "Patch", "PeriodicBox", "RunParams",
+    "DeviceTables", "ExclusionCSR", "Grid", "MarsError", "PairParams", "ParticleType", "Patch", "PeriodicBox", "RunParams",
     "SystemSpec", "TabulatedPotential", "ZOrderPairlist", "ZOrderSort", "create_pairlist", "device_grid_views",
     "exclusion_csr_from_pairs", "launch_BAOAB", "launch_BAOAB_LastUpdate", "launch_BD", "launch_PMF",
-    "launch_pairwise_nonbonded", "launch_resolve_pair_tables", "launch_tabulated_angles", "launch_tabulated_bonds",
+    "launch_analytical_pair_nonbonded", "launch_pairwise_nonbonded", "launch_resolve_pair_tables", "launch_tabulated_angles", "launch_tabulated_bonds",
     "launch_tabulated_dihedrals", "launch_harmonic_restraints",
     "lib", "make_pair_tag", "make_types", "particle_view", "philox_draw",
 ]

@@ -12,6 +12,11 @@ from pathlib import Path
 MAX_TYPES = 16
 MAX_PMF_TERMS = 32
 
+PAIR_TERM_COULOMB = 1 << 0
+PAIR_TERM_DEBYE_HUCKEL = 1 << 1
+PAIR_TERM_ONCK = 1 << 2
+PAIR_TERM_GAUSSIAN = 1 << 3
+PAIR_TERM_SOFTCORE = 1 << 4
 PAIR_TERM_TABULATED = 1 << 5
 PAIR_TABLE_SHIFT = 8
 INTEGRATOR_BROWNIAN = 0
@@ -73,6 +78,8 @@ class ParticleTypes(C.Structure):
         ("num_types", C.c_int32),
         ("mass", C.c_float * MAX_TYPES),
         ("charge", C.c_float * MAX_TYPES),
+        ("radius", C.c_float * MAX_TYPES),
+        ("eps", C.c_float * MAX_TYPES),
         ("diffusion", (C.c_float * 3) * MAX_TYPES),
         ("trans_damping", (C.c_float * 3) * MAX_TYPES),
         ("pmf_grid_offset", C.c_int32 * MAX_TYPES),
@@ -80,6 +87,19 @@ class ParticleTypes(C.Structure):
         ("pmf_grid_terms", GridTerm * MAX_PMF_TERMS),
         ("force_grid_id", (C.c_int32 * 3) * MAX_TYPES),
         ("force_grid_scale", (C.c_float * 3) * MAX_TYPES),
+    ]
+
+
+class PairParams(C.Structure):
+    _fields_ = [
+        ("debye_length", C.c_float),
+        ("dielectric", C.c_float),
+        ("onck_kappa", C.c_float),
+        ("onck_sz", C.c_float),
+        ("onck_z", C.c_float),
+        ("gaussian_amp", C.c_float),
+        ("gaussian_sigma", C.c_float),
+        ("gaussian_cutoff_squared", C.c_float),
     ]
 
 
@@ -145,6 +165,7 @@ class PatchDesc(C.Structure):
         ("restraint_ids", C.c_void_p),
         ("restraint_anchors", C.c_void_p),
         ("restraint_k", C.c_void_p),
+        ("pair_params", C.c_void_p),
     ]
 
 
@@ -159,6 +180,7 @@ class BatchDesc(C.Structure):
         ("enabled_terms", C.c_uint32),
         ("num_nb_tables", C.c_int32),
         ("nb_tables", C.c_void_p),
+        ("pair_params", C.c_void_p),
     ]
 
 
@@ -230,8 +252,10 @@ PROTOTYPES = {
     "marsmd_pairlist_get_sorter": (_I32, [_P, _PP]),
     "marsmd_pairlist_get_reference_cells": (_I32, [_P, _PP, _PP, _PP, C.POINTER(_U64), C.POINTER(_I32), _P]),
     "marsmd_launch_resolve_pair_tables": (_I32, [_P, _P, _P, _U64, _P, _U64, _P]),
-    "marsmd_launch_pairwise_nonbonded": (_I32, [_P, C.POINTER(ParticleView), _P, _P, C.POINTER(Box), _I32, _U64, _F,
-                                                _U32, _P]),
+    "marsmd_launch_pairwise_nonbonded": (_I32, [_P, C.POINTER(ParticleView), _P, _P, C.POINTER(ParticleTypes),
+                                                C.POINTER(Box), _I32, _U64, _F, _U32, C.POINTER(PairParams), _P]),
+    "marsmd_launch_analytical_pair_nonbonded": (_I32, [_P, _P, _P, _P, _P, C.POINTER(ParticleTypes), C.POINTER(Box),
+                                                       _U32, C.POINTER(PairParams), _I32, _U64, _F, _P]),
     "marsmd_launch_tabulated_bonds": (_I32, [_P, _P, _P, _P, _P, _P, C.POINTER(Box), _I32, _U64, _P]),
     "marsmd_launch_tabulated_angles": (_I32, [_P, _P, _P, _P, _P, _P, C.POINTER(Box), _I32, _U64, _P]),
     "marsmd_launch_tabulated_dihedrals": (_I32, [_P, _P, _P, _P, _P, _P, C.POINTER(Box), _I32, _U64, _P]),

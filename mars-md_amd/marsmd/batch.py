@@ -34,8 +34,8 @@ class Batch:
         first = specs[0]
         for s in specs[1:]:
             if (len(s.types) != len(first.types) or len(s.nb_tables) != len(first.nb_tables)
-                    or not np.array_equal(s.tag_matrix(), first.tag_matrix())):
-                raise ValueError("the replicas of a batch share types, pair tables and pair tags")
+                    or not np.array_equal(s.tag_matrix(), first.tag_matrix()) or s.pair_params != first.pair_params):
+                raise ValueError("the replicas of a batch share types, pair tables, pair tags and pair constants")
         self.specs = list(specs)
         self.counts = np.array([s.n for s in specs], np.uint64)
         self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
@@ -60,6 +60,10 @@ class Batch:
         d.enabled_terms = first.enabled_terms
         d.num_nb_tables = len(first.nb_tables)
         d.nb_tables = C.addressof(recs)
+        if first.pair_params is not None:
+            pp = first.pair_params.to_c()
+            keep.append(pp)
+            d.pair_params = C.addressof(pp)
         h = C.c_void_p()
         api.call("marsmd_batch_create", C.byref(d), C.byref(h))
         self._h = h

@@ -138,6 +138,8 @@ class ParticleType:
     mass: float = 1.0
     charge: float = 0.0
     diffusion: Sequence[float] = (0.0, 0.0, 0.0)
+    radius: float = 0.0  # softcore pair term
+    eps: float = 0.0  # softcore pair term
     trans_damping: Sequence[float] = (0.0, 0.0, 0.0)
     pmf_terms: Sequence[tuple] = ()  # (grid_id, scale, boundary_condition)
     force_grid_id: Sequence[int] = (-1, -1, -1)
@@ -153,6 +155,8 @@ def make_types(types: Sequence[ParticleType]) -> api.ParticleTypes:
     for i, t in enumerate(types):
         out.mass[i] = t.mass
         out.charge[i] = t.charge
+        out.radius[i] = t.radius
+        out.eps[i] = t.eps
         for a in range(3):
             out.diffusion[i][a] = t.diffusion[a]
             out.trans_damping[i][a] = t.trans_damping[a]
@@ -169,6 +173,26 @@ def make_types(types: Sequence[ParticleType]) -> api.ParticleTypes:
             out.pmf_grid_terms[k].boundary_condition = bc
             k += 1
     return out
+
+
+@dataclass
+class PairParams:
+    """Constants of the analytical pair terms: SolventParams
+    (NonBondedInteraction.h:121-127; Debye-Hueckel and ONC) and GaussianPotential's
+    members (Pairwise.h:53-57). The defaults are the reference's."""
+
+    debye_length: float = 10.0
+    dielectric: float = 80.0
+    onck_kappa: float = 0.1
+    onck_sz: float = 80.0
+    onck_z: float = 6.86
+    gaussian_amp: float = 1.0
+    gaussian_sigma: float = 1.0
+    gaussian_cutoff_squared: float = 0.0
+
+    def to_c(self) -> api.PairParams:
+        return api.PairParams(self.debye_length, self.dielectric, self.onck_kappa, self.onck_sz, self.onck_z,
+                              self.gaussian_amp, self.gaussian_sigma, self.gaussian_cutoff_squared)
 
 
 # ---------------------------------------------------------------------------- grids
@@ -405,12 +429,29 @@ def launch_resolve_pair_tables(pairs, type_ids, term_matrix, num_types: int, pai
              _ptr(pair_tag), num_pairs, _stream(stream))
 
 
-def launch_pairwise_nonbonded(pairs, view: api.ParticleView, pair_tag, tables: DeviceTables, box: PeriodicBox,
+def launch_pairwise_nonbonded(pairs, view: api.ParticleView, pair_tag, tables, box: PeriodicBox,
                               get_energy: bool, num_pairs: int, cutoff_squared: float,
-                              enabled_terms: int = api.PAIR_TERM_TABULATED, stream=None):
-    api.call("marsmd_launch_pairwise_nonbonded", _ptr(pairs), C.byref(view), _ptr(pair_tag), tables.ptr,
-             C.byref(box.to_c()),
-             1 if get_energy else 0, num_pairs, float(cutoff_squared), enabled_terms, _stream(stream))
+                              enabled_terms: int = api.PAIR_TERM_TABULATED, stream=None, types=None,
+                              params: Optional[PairParams] = None):
+    """launch_pairwise_nonbonded (NonBondedInteraction.h:262-296): every enabled term.
+    tables: DeviceTables or None; types: a list of ParticleType (charges, softcore
+    radius and eps) when analytical terms are enabled."""
+    t = make_types(types) if types is not None else None
+    api.call("marsmd_launch_pairwise_nonbonded", _ptr(pairs), C.byref(view), _ptr(pair_tag),
+             tables.ptr if tables is not None else None, C.byref(t) if t is not None else None,
+             C.byref(box.to_c()), 1 if get_energy else 0, num_pairs, float(cutoff_squared), enabled_terms,
+             C.byref(params.to_c()) if params is not None else None, _stream(stream))
+
+
+def launch_analytical_pair_nonbonded(pairs, pos, force_energy, type_ids, pair_tag, types, box: PeriodicBox,
+                                     enabled_terms: int, get_energy: bool, num_pairs: int, cutoff_squared: float,
+                                     params: Optional[PairParams] = None, stream=None):
+    """launch_analytical_pair_nonbonded (AnalyticalPairKernels.h:21-118)."""
+    t = make_types(types)
+    api.call("marsmd_launch_analytical_pair_nonbonded", _ptr(pairs), _ptr(pos), _ptr(force_energy), _ptr(type_ids),
+             _ptr(pair_tag), C.byref(t), C.byref(box.to_c()), enabled_terms,
+             C.byref(params.to_c()) if params is not None else None, 1 if get_energy else 0, num_pairs,
+             float(cutoff_squared), _stream(stream))
 
 
 def launch_tabulated_bonds(bonds, pos, force_energy, tables: DeviceTables, table_indices, forms, box: PeriodicBox,

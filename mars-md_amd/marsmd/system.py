@@ -14,7 +14,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _capi as api
-from .core import Grid, ParticleType, PeriodicBox, TabulatedPotential, make_pair_tag, make_types
+from .core import Grid, PairParams, ParticleType, PeriodicBox, TabulatedPotential, make_pair_tag, make_types
 
 
 @dataclass
@@ -74,6 +74,7 @@ class SystemSpec:
     restraint_ids: Optional[np.ndarray] = None  # (R,) particle per harmonic restraint
     restraint_anchors: Optional[np.ndarray] = None  # (R, 3)
     restraint_k: Optional[np.ndarray] = None  # (R,)
+    pair_params: Optional[PairParams] = None  # analytical pair terms (None: the defaults)
     seed: int = 42
     patch_id: int = 0
     name: str = ""
@@ -166,6 +167,10 @@ class SystemSpec:
             anchors[:, :3] = np.asarray(self.restraint_anchors, np.float32)[:, :3]
             d.restraint_anchors = arr(anchors, np.float32)
             d.restraint_k = arr(self.restraint_k, np.float32)
+        if self.pair_params is not None:
+            pp = self.pair_params.to_c()
+            keep.append(pp)
+            d.pair_params = C.addressof(pp)
         if self.grids:
             recs = (api.GridView * len(self.grids))()
             for i, g in enumerate(self.grids):

@@ -219,8 +219,116 @@ inline FE<T> table_compute(T dx, const Table<T>& tab) { // TabulatedPotential.h:
 }
 
 // ---------------------------------------------------------------------- pair force
-// CalcDistance::compute (src/Interactions/Interactions.h:37-53) followed by the
-// tabulated branch of PairNonbondedComputer (Pairwise.h:214-304).
+// ------------------------------------------------------- analytical pair terms
+// Constants of the analytical terms: SolventParams (NonBondedInteraction.h:121-127) and
+// GaussianPotential (Pairwise.h:53-57), per-type charge, softcore radius and eps.
+template<class T>
+struct PairCoef {
+	T debye_length{10}, dielectric{80}, onck_kappa{T(0.1)}, onck_sz{80}, onck_z{T(6.86)};
+	T gaussian_amp{1}, gaussian_sigma{1}, gaussian_cutoff_squared{0};
+	std::vector<T> charge, radius, eps;
+};
+
+constexpr uint32_t kTermCoulomb = 1u, kTermDH = 2u, kTermOnck = 4u, kTermGauss = 8u, kTermSoftcore = 16u,
+				   kTermTabulated = 32u;
+
+inline float sinh_t(float x) { return std::sinh(x); }
+inline double sinh_t(double x) { return std::sinh(x); }
+inline float cosh_t(float x) { return std::cosh(x); }
+inline double cosh_t(double x) { return std::cosh(x); }
+inline float exp_t(float x) { return std::exp(x); }
+inline double exp_t(double x) { return std::exp(x); }
+
+// sinh(v) - v and v cosh(v) - sinh(v): for |v| < 1 the Taylor series to v^9, as
+// math::sinh_minus_x / x_cosh_minus_sinh (Types/Math.h:198-215); the FP64 twin uses
+// the same form.
+template<class T>
+inline T sinh_minus_v(T v) {
+	if (v >= T(1) || v <= T(-1))
+		return sinh_t(v) - v;
+	const T w = v * v;
+	return v * w * (T(1.0 / 6) + w * (T(1.0 / 120) + w * (T(1.0 / 5040) + w * T(1.0 / 362880))));
+}
+template<class T>
+inline T v_cosh_minus_sinh(T v) {
+	if (v >= T(1) || v <= T(-1))
+		return v * cosh_t(v) - sinh_t(v);
+	const T w = v * v;
+	return v * w * (T(1.0 / 3) + w * (T(1.0 / 30) + w * (T(1.0 / 840) + w * T(1.0 / 45360))));
+}
+
+// The analytical branches of PairNonbondedComputer::operator() (Pairwise.h:226-277) in
+// order, with the functors of Columb.h:11-101 (Coulomb, Debye-Hueckel, ONC) and
+// Pairwise.h:47-99 (Gaussian, softcore). d = |r|, d2 = r.r; adds -dU/dr to fm and U to en.
+template<class T>
+inline void analytic_terms(uint32_t terms, T d, T d2, int ti, int tj, const PairCoef<T>& c, T& fm, T& en) {
+	const T C = T(332.0636f); // constants::COULOMB (Constants.h:24)
+	T qi = 0, qj = 0;
+	if (terms & (kTermCoulomb | kTermDH | kTermOnck)) {
+		qi = c.charge[size_t(ti)];
+		qj = c.charge[size_t(tj)];
+	}
+	if (terms & kTermCoulomb) {
+		const T inv_d2 = T(1) / (d * d);
+		const T inv_d = T(1) / d;
+		fm += qi * qj * inv_d2 * C;
+		en += qi * qj * inv_d * C;
+	}
+	if (terms & kTermDH) {
+		const T inv_r = T(1) / d;
+		const T a = qi * qj * C / c.dielectric;
+		const T screen = exp_t(-d / c.debye_length);
+		const T u = a * screen * inv_r;
+		fm += u * (inv_r + T(1) / c.debye_length);
+		en += u;
+	}
+	if (terms & kTermOnck) {
+		const T r = d < T(1e-2f) ? T(1e-2f) : d;
+		const T a = qi * qj * C;
+		const T screen = exp_t(-c.onck_kappa * r);
+		const T v = r / (T(2) * c.onck_z);
+		T eps, deps;
+		if (v < T(1e-5f)) {
+			eps = c.onck_sz * v * v / T(3);
+			deps = c.onck_sz * v / (T(3) * c.onck_z);
+		} else {
+			const T sh = sinh_t(v);
+			eps = c.onck_sz * sinh_minus_v(v) * (sh + v) / (sh * sh);
+			deps = c.onck_sz * v * v_cosh_minus_sinh(v) / (c.onck_z * sh * sh * sh);
+		}
+		en += a * screen / (eps * r);
+		fm += a * screen * (c.onck_kappa / (eps * r) + (deps * r + eps) / (eps * eps * r * r));
+	}
+	if (terms & kTermGauss) {
+		const T dd = d * d;
+		if (!(c.gaussian_cutoff_squared > T(0) && dd > c.gaussian_cutoff_squared)) {
+			const T inv_s2 = T(1) / (c.gaussian_sigma * c.gaussian_sigma);
+			const T u = c.gaussian_amp * exp_t(-dd * inv_s2);
+			fm += T(2) * d * u * inv_s2;
+			en += u;
+		}
+	}
+	if (terms & kTermSoftcore) {
+		const T rad = T(0.5) * (c.radius[size_t(ti)] + c.radius[size_t(tj)]);
+		const T rad2 = rad * rad;
+		const T rad6 = rad2 * rad2 * rad2;
+		const T pe = std::sqrt(c.eps[size_t(ti)] * c.eps[size_t(tj)]);
+		const T d6 = d2 * d2 * d2;
+		T f = T(-12) * pe * (rad6 * rad6 / (d6 * d6 * d2) - rad6 / (d6 * d2));
+		T u = 0;
+		if (d6 < rad6) {
+			const T d12 = d6 * d6, rad12 = rad6 * rad6;
+			u = pe * ((rad12 / d12) - (T(2) * rad6 / d6)) + pe;
+			f = T(-12) * pe * (rad12 / (d12 * d2) - rad6 / (d6 * d2));
+		}
+		fm += -f * d;
+		en += u;
+	}
+}
+
+// CalcDistance::compute (src/Interactions/Interactions.h:37-53) followed by the term
+// branches of PairNonbondedComputer (Pairwise.h:214-304): the analytical terms in
+// `terms` (with types ti, tj of x and y) and the tabulated one when `tab` is given.
 template<class T>
 struct PairTerm {
 	bool valid{false};
@@ -229,11 +337,13 @@ struct PairTerm {
 };
 
 template<class T>
-inline PairTerm<T> pair_term(const Box& box, const V4<T>& px, const V4<T>& py,
-							 const Table<T>& tab, T cutoff_squared) {
+inline PairTerm<T> pair_term(const Box& box, const V4<T>& px, const V4<T>& py, const Table<T>* tab,
+							 T cutoff_squared, uint32_t terms = kTermTabulated, int ti = 0, int tj = 0,
+							 const PairCoef<T>* coef = nullptr) {
 	PairTerm<T> out;
 	const V4<T> r = wrap_diff<T>(box, sub<T>(py, px));
-	const T d = std::sqrt(length2<T>(r));
+	const T d2 = length2<T>(r);
+	const T d = std::sqrt(d2);
 	V4<T> unit{};
 	if (d > T(1e-6)) {
 		const T inv = T(1) / d;
@@ -243,10 +353,15 @@ inline PairTerm<T> pair_term(const Box& box, const V4<T>& px, const V4<T>& py,
 		return out;
 	if (cutoff_squared > T(0) && d * d > cutoff_squared)
 		return out;
-	const FE<T> fe = table_compute<T>(d, tab);
-	// force_magnitude and energy start at 0 and take one tabulated term.
-	const T fm = T(0) + fe.f;
-	const T en = T(0) + fe.e;
+	// force_magnitude and energy start at 0 and take each enabled term in order
+	T fm = 0, en = 0;
+	if (coef && (terms & (kTermCoulomb | kTermDH | kTermOnck | kTermGauss | kTermSoftcore)))
+		analytic_terms<T>(terms, d, d2, ti, tj, *coef, fm, en);
+	if (tab && (terms & kTermTabulated)) {
+		const FE<T> fe = table_compute<T>(d, *tab);
+		fm += fe.f;
+		en += fe.e;
+	}
 	out.valid = true;
 	out.force = V4<T>{fm * unit.x, fm * unit.y, fm * unit.z, T(0)};
 	out.half_energy = en * T(0.5);

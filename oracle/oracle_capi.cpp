@@ -65,6 +65,27 @@ Table<T> make_table(const marsmd_table& t) {
 }
 
 template<class T>
+PairCoef<T> make_coef(const marsmd_particle_types& t, const marsmd_pair_params* p) {
+	PairCoef<T> c;
+	if (p) {
+		c.debye_length = T(p->debye_length);
+		c.dielectric = T(p->dielectric);
+		c.onck_kappa = T(p->onck_kappa);
+		c.onck_sz = T(p->onck_sz);
+		c.onck_z = T(p->onck_z);
+		c.gaussian_amp = T(p->gaussian_amp);
+		c.gaussian_sigma = T(p->gaussian_sigma);
+		c.gaussian_cutoff_squared = T(p->gaussian_cutoff_squared);
+	}
+	for (int k = 0; k < MARSMD_MAX_TYPES; ++k) {
+		c.charge.push_back(T(t.charge[k]));
+		c.radius.push_back(T(t.radius[k]));
+		c.eps.push_back(T(t.eps[k]));
+	}
+	return c;
+}
+
+template<class T>
 Grid<T> make_grid(const marsmd_grid_view& g) {
 	Grid<T> out;
 	out.data = g.data;
@@ -365,6 +386,7 @@ struct Sim {
 	uint32_t enabled_terms = MARSMD_PAIR_TERM_TABULATED;
 	std::vector<Table<float>> nb_tables, bond_tables, angle_tables;
 	std::vector<Table<double>> nb_tables64;
+	PairCoef<float> coef;
 	std::vector<std::pair<int, int>> bonds;
 	std::vector<int32_t> bond_table;
 	std::vector<std::array<int, 3>> angles;
@@ -450,10 +472,13 @@ void sim_pair_forces_threads(Sim& s, float cutoff, bool compute_energy) {
 		for (size_t k = b; k < e; ++k) {
 			const int x = s.pl.pairs[k].first, y = s.pl.pairs[k].second;
 			const uint32_t tag = s.term_matrix[size_t(s.type[x] * T + s.type[y])];
-			if (!(tag & s.enabled_terms & MARSMD_PAIR_TERM_TABULATED))
+			const uint32_t terms = tag & s.enabled_terms;
+			if (!terms)
 				continue;
-			const PairTerm<float> pt = pair_term<float>(s.box, s.pos[x], s.pos[y],
-														 s.nb_tables[tag >> MARSMD_PAIR_TABLE_SHIFT], cutoff2);
+			const PairTerm<float> pt = pair_term<float>(
+				s.box, s.pos[x], s.pos[y],
+				(terms & MARSMD_PAIR_TERM_TABULATED) ? &s.nb_tables[tag >> MARSMD_PAIR_TABLE_SHIFT] : nullptr, cutoff2,
+				terms, s.type[x], s.type[y], &s.coef);
 			if (!pt.valid)
 				continue;
 			f[size_t(x)].x += -pt.force.x;
@@ -490,10 +515,12 @@ void sim_pair_forces(Sim& s, float cutoff, bool compute_energy) {
 		const int x = pr.first, y = pr.second;
 		const uint32_t tag = s.term_matrix[size_t(s.type[x] * T + s.type[y])];
 		const uint32_t terms = tag & s.enabled_terms;
-		if (!(terms & MARSMD_PAIR_TERM_TABULATED))
+		if (!terms)
 			continue;
-		const PairTerm<float> pt = pair_term<float>(s.box, s.pos[x], s.pos[y],
-													 s.nb_tables[tag >> MARSMD_PAIR_TABLE_SHIFT], cutoff2);
+		const PairTerm<float> pt = pair_term<float>(
+			s.box, s.pos[x], s.pos[y],
+			(terms & MARSMD_PAIR_TERM_TABULATED) ? &s.nb_tables[tag >> MARSMD_PAIR_TABLE_SHIFT] : nullptr, cutoff2,
+			terms, s.type[x], s.type[y], &s.coef);
 		if (!pt.valid)
 			continue;
 		s.force[x].x += -pt.force.x;
@@ -985,8 +1012,11 @@ int orc_pair_forces(const int32_t* pairs, uint64_t num_pairs, const float* pos4,
 					const int32_t* types, const uint32_t* term_matrix, int32_t num_types,
 					const marsmd_table* tables, int32_t num_tables, const marsmd_box* box,
 					float cutoff_squared, uint32_t enabled_terms, int32_t mode, uint64_t n,
-					double* out) {
+					double* out, const marsmd_particle_types* ptypes, const marsmd_pair_params* params) {
 	const Box b = make_box(*box);
+	const marsmd_particle_types no_types{};
+	const PairCoef<float> c32 = make_coef<float>(ptypes ? *ptypes : no_types, params);
+	const PairCoef<double> c64 = make_coef<double>(ptypes ? *ptypes : no_types, params);
 	std::vector<Table<float>> t32;
 	std::vector<Table<double>> t64;
 	for (int k = 0; k < num_tables; ++k) {
@@ -1000,12 +1030,14 @@ int orc_pair_forces(const int32_t* pairs, uint64_t num_pairs, const float* pos4,
 		const int x = pairs[2 * k], y = pairs[2 * k + 1];
 		const uint32_t tag = term_matrix ? term_matrix[size_t(types[x] * num_types + types[y])] : enabled_terms;
 		const uint32_t terms = tag & enabled_terms;
-		if (!(terms & MARSMD_PAIR_TERM_TABULATED))
+		if (!terms)
 			continue;
 		const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
+		const bool tab = (terms & MARSMD_PAIR_TERM_TABULATED) != 0;
 		if (mode == 2) {
 			const V4<double> px{p[x].x, p[x].y, p[x].z, 0}, py{p[y].x, p[y].y, p[y].z, 0};
-			const PairTerm<double> pt = pair_term<double>(b, px, py, t64[size_t(ti)], double(cutoff_squared));
+			const PairTerm<double> pt = pair_term<double>(b, px, py, tab ? &t64[size_t(ti)] : nullptr,
+														  double(cutoff_squared), terms, types[x], types[y], &c64);
 			if (!pt.valid)
 				continue;
 			out[4 * x + 0] -= pt.force.x;
@@ -1018,7 +1050,8 @@ int orc_pair_forces(const int32_t* pairs, uint64_t num_pairs, const float* pos4,
 			out[4 * y + 3] += pt.half_energy;
 			continue;
 		}
-		const PairTerm<float> pt = pair_term<float>(b, p[x], p[y], t32[size_t(ti)], cutoff_squared);
+		const PairTerm<float> pt = pair_term<float>(b, p[x], p[y], tab ? &t32[size_t(ti)] : nullptr, cutoff_squared,
+													 terms, types[x], types[y], &c32);
 		if (!pt.valid)
 			continue;
 		if (mode == 1) {
@@ -1170,6 +1203,7 @@ void* orc_sim_create(const marsmd_patch_desc* d) {
 		else
 			s->term_matrix.assign(size_t(T) * size_t(T), 0u);
 		s->enabled_terms = d->enabled_terms;
+		s->coef = make_coef<float>(d->types, d->pair_params);
 		for (int k = 0; k < d->num_nb_tables; ++k) {
 			s->nb_tables.push_back(make_table<float>(d->nb_tables[k]));
 			s->nb_tables64.push_back(make_table<double>(d->nb_tables[k]));

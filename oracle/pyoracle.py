@@ -45,7 +45,8 @@ def lib():
             "orc_build_pairlist": (C.c_int64, [P, U64, C.POINTER(api.Box), F, C.POINTER(api.ExclusionView), U64, I32, P, U64,
                                                P, P, P, P]),
             "orc_exclusion_csr": (C.c_int64, [P, U64, P, P]),
-            "orc_pair_forces": (I32, [P, U64, P, P, P, I32, P, I32, C.POINTER(api.Box), F, U32, I32, U64, P]),
+            "orc_pair_forces": (I32, [P, U64, P, P, P, I32, P, I32, C.POINTER(api.Box), F, U32, I32, U64, P,
+                                      P, P]),
             "orc_table_compute": (None, [C.POINTER(api.Table), F, P]),
             "orc_table_compute64": (None, [C.POINTER(api.Table), C.c_double, P]),
             "orc_bonded_forces": (I32, [P, P, U64, P, P, U64, P, P, I32, P, I32, C.POINTER(api.Box), I32, U64, P]),
@@ -237,8 +238,14 @@ def table_records(tables):
 
 
 def pair_forces(pairs, pos4, types, tag_matrix, tables, box, cutoff_squared, mode=1,
-                enabled_terms=api.PAIR_TERM_TABULATED):
-    """mode 0: FP32 sums, 1: FP32 terms summed in FP64, 2: FP64 twin. -> (N, 4) float64."""
+                enabled_terms=api.PAIR_TERM_TABULATED, particle_types=None, pair_params=None):
+    """mode 0: FP32 sums, 1: FP32 terms summed in FP64, 2: FP64 twin. -> (N, 4) float64.
+    particle_types (list of ParticleType) and pair_params (PairParams, None: defaults)
+    feed the analytical terms."""
+    from marsmd.core import make_types
+
+    pt = make_types(particle_types) if particle_types is not None else None
+    pp = pair_params.to_c() if pair_params is not None else None
     pairs = np.ascontiguousarray(pairs, np.int32)
     pos4 = np.ascontiguousarray(pos4, np.float32)
     types = np.ascontiguousarray(types, np.int32)
@@ -249,7 +256,8 @@ def pair_forces(pairs, pos4, types, tag_matrix, tables, box, cutoff_squared, mod
     nt = 0 if tm is None else int(round(np.sqrt(tm.size)))
     _check(lib().orc_pair_forces(pairs.ctypes.data, len(pairs), pos4.ctypes.data, types.ctypes.data, _p(tm), nt,
                                  C.addressof(recs), len(tables), C.byref(b), cutoff_squared, enabled_terms, mode,
-                                 len(pos4), out.ctypes.data))
+                                 len(pos4), out.ctypes.data, C.byref(pt) if pt is not None else None,
+                                 C.byref(pp) if pp is not None else None))
     return out
 
 

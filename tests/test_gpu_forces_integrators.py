@@ -247,7 +247,8 @@ def _check_step(cuda, patch, spec, step, check_pairs=True):
             ref = orc.build_pairlist(g["pos"], spec.box, P.pairlist_cutoff, off, exc, ne)
             assert np.array_equal(gp, ref["pairs"].astype(np.int64)), f"pair set differs at step {step}"
     fref = orc.pair_forces(gp, g["pos"], g["type"], spec.tag_matrix(), spec.nb_tables, spec.box,
-                           P.cutoff * P.cutoff, mode=1)
+                           P.cutoff * P.cutoff, mode=1, enabled_terms=spec.enabled_terms, particle_types=spec.types,
+                           pair_params=spec.pair_params)
     if topo["bonds"] is not None:
         fref += orc.bonded_forces(g["pos"], spec.box, topo["bonds"], spec.bond_table, spec.bond_tables,
                                   topo["angles"], spec.angle_table, spec.angle_tables, mode=1)
