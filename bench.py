@@ -271,13 +271,17 @@ def main():
     spec = configs.config2(n=args.n, grid_n=args.grid)
     patch = marsmd.Patch(spec)
     stream = torch.cuda.current_stream()
-    patch.run(args.warmup)
+    # warmup: the autotuner's real steps (launch candidates timed on the step kernels),
+    # then the configuration it keeps
+    warmup = max(args.warmup, 60)
+    choice = patch.autotune(warmup)
     torch.cuda.synchronize()
 
     # timed region: `steps` steps in one run() call, bracketed by HIP events on the
-    # stream the library launches on; inside it the library brackets every launch of
-    # the fused step kernel with HIP events on the same stream
-    first = args.warmup
+    # stream the library launches on; inside it the library brackets every step's
+    # kernels (the fused step kernel, and the PMF kernel when split) with HIP events on
+    # the same stream
+    first = warmup
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     patch.set_kernel_timing(True)
@@ -305,7 +309,7 @@ def main():
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warmup,
         "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -318,7 +322,11 @@ def main():
                    "rebuild_period": spec.params.rebuild_period},
         "hbm_roofline_frac_whole_step": value * acc["b_alg_per_particle_step"] / HBM_PEAK,
         "b_alg_per_particle_step": acc["b_alg_per_particle_step"],
-        "roofline": {"bound": "hbm", "kernel": "k_step (fused pair force + PMF + BAOAB)",
+        "launch": {"blocks_per_cu": choice["blocks_per_cu"], "split_pmf": choice["split_pmf"],
+                   "tuned_ms_per_step": choice["ms_per_step"]},
+        "roofline": {"bound": "hbm",
+                     "kernel": ("k_pmf_force + k_step (grid force, then pair force + BAOAB)" if choice["split_pmf"]
+                                else "k_step (fused pair force + PMF + BAOAB)"),
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_unit": "B/launch",
                      "traffic_source": traffic_src,

@@ -464,6 +464,25 @@ marsmd_status marsmd_patch_run(marsmd_patch* p, const marsmd_run_params* params,
 marsmd_status marsmd_patch_set_kernel_timing(marsmd_patch* p, int32_t enable);
 marsmd_status marsmd_patch_get_kernel_timing(marsmd_patch* p, double* total_ms, uint64_t* launches);
 
+/* Launch configuration of marsmd_patch_run's step (no counterpart in the reference).
+ * Results do not depend on it: each particle's step is one lane's work whatever the
+ * grid, and a precomputed position-dependent force is the same function of the same
+ * inputs. With split_pmf, the timing events above bracket both kernels of a step. */
+typedef struct marsmd_patch_launch {
+    int32_t blocks_per_cu; /* persistent step-kernel workgroups per CU; 0: as many as fit */
+    int32_t split_pmf;     /* with grids: the position-dependent force (grid sampling and
+                              field) by a kernel of its own, read by the step kernel */
+} marsmd_patch_launch;
+
+marsmd_status marsmd_patch_set_launch(marsmd_patch* p, const marsmd_patch_launch* launch);
+marsmd_status marsmd_patch_get_launch(marsmd_patch* p, marsmd_patch_launch* launch);
+/* Steps first_step+1 .. first_step+num_steps like marsmd_patch_run while timing every
+ * candidate launch configuration on the step kernels (two passes over the candidates),
+ * then keeps the fastest: *best is set to it and *best_ms_per_step to its time. */
+marsmd_status marsmd_patch_autotune(marsmd_patch* p, const marsmd_run_params* params, uint64_t first_step,
+                                    uint64_t num_steps, marsmd_patch_launch* best, double* best_ms_per_step,
+                                    marsmd_stream stream);
+
 /* ------------------------------------------------------------------ replica batch
  * Many independent replicas of a periodic nonbonded system, each one the reference's
  * Patch (Patch.h:48-645) driven by SimManager::run's per-step sequence

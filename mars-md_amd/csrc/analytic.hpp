@@ -62,16 +62,18 @@ __device__ __forceinline__ float v_cosh_minus_sinh(float v) {
 	return v * w * (1.0f / 3.0f + w * (1.0f / 30.0f + w * (1.0f / 840.0f + w * (1.0f / 45360.0f))));
 }
 
-// The analytical terms enabled in `terms` for a pair of types (ti, tj) at distance d
-// (d2 = the squared length of the wrapped difference): force magnitude (-dU/dr) and
-// energy added to fm and en in the reference's order.
-__device__ __forceinline__ void analytic_terms(uint32_t terms, float d, float d2, uint32_t ti, uint32_t tj,
-											   const PairCoeffs& C, float& fm, float& en) {
+// The analytical terms enabled in `terms` for a pair at distance d (d2 = the squared
+// length of the wrapped difference) whose particles carry charges ci, cj and softcore
+// radii and eps (ri, rj, ei, ej): force magnitude (-dU/dr) and energy added to fm and
+// en in the reference's order.
+__device__ __forceinline__ void analytic_terms(uint32_t terms, float d, float d2, float ci, float cj, float ri,
+											   float rj, float ei, float ej, const marsmd_pair_params& prm, float& fm,
+											   float& en) {
 	constexpr uint32_t charged = MARSMD_PAIR_TERM_COULOMB | MARSMD_PAIR_TERM_DEBYE_HUCKEL | MARSMD_PAIR_TERM_ONCK;
 	float qi = 0.0f, qj = 0.0f;
 	if (terms & charged) {
-		qi = C.charge[ti];
-		qj = C.charge[tj];
+		qi = ci;
+		qj = cj;
 	}
 	if (terms & MARSMD_PAIR_TERM_COULOMB) { // ColumbPotential::compute
 		const float inv_d2 = 1.0f / (d * d);
@@ -81,17 +83,17 @@ __device__ __forceinline__ void analytic_terms(uint32_t terms, float d, float d2
 	}
 	if (terms & MARSMD_PAIR_TERM_DEBYE_HUCKEL) { // U = A exp(-r/l) / r, A = C qi qj / eps
 		const float inv_r = 1.0f / d;
-		const float a = qi * qj * kCoulomb / C.prm.dielectric;
-		const float screen = expf(-d / C.prm.debye_length);
+		const float a = qi * qj * kCoulomb / prm.dielectric;
+		const float screen = expf(-d / prm.debye_length);
 		const float u = a * screen * inv_r;
-		fm += u * (inv_r + 1.0f / C.prm.debye_length);
+		fm += u * (inv_r + 1.0f / prm.debye_length);
 		en += u;
 	}
 	if (terms & MARSMD_PAIR_TERM_ONCK) { // U = C qi qj exp(-k r) / (eps(r) r)
 		const float r = d < 1e-2f ? 1e-2f : d;
 		const float a = qi * qj * kCoulomb;
-		const float screen = expf(-C.prm.onck_kappa * r);
-		const float sz = C.prm.onck_sz, z = C.prm.onck_z;
+		const float screen = expf(-prm.onck_kappa * r);
+		const float sz = prm.onck_sz, z = prm.onck_z;
 		const float v = r / (2.0f * z);
 		float eps, deps;
 		if (v < 1e-5f) {
@@ -103,22 +105,22 @@ __device__ __forceinline__ void analytic_terms(uint32_t terms, float d, float d2
 			deps = sz * v * v_cosh_minus_sinh(v) / (z * s * s * s);
 		}
 		en += a * screen / (eps * r);
-		fm += a * screen * (C.prm.onck_kappa / (eps * r) + (deps * r + eps) / (eps * eps * r * r));
+		fm += a * screen * (prm.onck_kappa / (eps * r) + (deps * r + eps) / (eps * eps * r * r));
 	}
 	if (terms & MARSMD_PAIR_TERM_GAUSSIAN) { // U = amp exp(-r^2 / sigma^2)
 		const float dd = d * d;
-		if (!(C.prm.gaussian_cutoff_squared > 0.0f && dd > C.prm.gaussian_cutoff_squared)) {
-			const float inv_s2 = 1.0f / (C.prm.gaussian_sigma * C.prm.gaussian_sigma);
-			const float u = C.prm.gaussian_amp * expf(-dd * inv_s2);
+		if (!(prm.gaussian_cutoff_squared > 0.0f && dd > prm.gaussian_cutoff_squared)) {
+			const float inv_s2 = 1.0f / (prm.gaussian_sigma * prm.gaussian_sigma);
+			const float u = prm.gaussian_amp * expf(-dd * inv_s2);
 			fm += 2.0f * d * u * inv_s2;
 			en += u;
 		}
 	}
 	if (terms & MARSMD_PAIR_TERM_SOFTCORE) { // softcoreForce returns dU/dr / r
-		const float rad = 0.5f * (C.radius[ti] + C.radius[tj]);
+		const float rad = 0.5f * (ri + rj);
 		const float rad2 = rad * rad;
 		const float rad6 = rad2 * rad2 * rad2;
-		const float pe = sqrtf(C.eps[ti] * C.eps[tj]);
+		const float pe = sqrtf(ei * ej);
 		const float d6 = d2 * d2 * d2;
 		float f = -12.0f * pe * (rad6 * rad6 / (d6 * d6 * d2) - rad6 / (d6 * d2));
 		float u = 0.0f;

@@ -234,12 +234,15 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restr
 														  uint64_t total, const uint32_t* __restrict__ rep_of_slot,
 														  const RepGeom* __restrict__ geom, NbrList L, NbTables T,
 														  const float* __restrict__ knots_g,
-														  marsmd_particle_types types, StepParams S,
+														  marsmd_particle_types types_arg, StepParams S,
 														  uint64_t base_seed, int store_force,
 														  const uint32_t* __restrict__ step_base) {
 	extern __shared__ float lds_knots[];
-	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
-	const float* knots = stage_tables(T, knots_g, lds_knots, lds_tag);
+	__shared__ NbLds lds_nb;
+	__shared__ marsmd_particle_types lds_types;
+	stage_types(types_arg, lds_types);
+	const float* knots = stage_tables(T, knots_g, lds_knots, lds_nb);
+	const marsmd_particle_types& types = lds_types;
 	// a replayed segment carries the step's offset in S.step and its first step in memory
 	const uint32_t step = S.step + (step_base ? step_base[0] : 0u);
 	const TileRange R = xcd_tiles(uint32_t((total + kBatchBlock - 1) / kBatchBlock));
@@ -253,10 +256,12 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restr
 		P.step = step;
 		P.seed = mix_seed(base_seed, G.pid, uint64_t(step));
 		const float4 p = pos[t];
-		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, knots, lds_tag, P.box, P.cutoff2, ENERGY);
+		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, knots, lds_nb, P.box, P.cutoff2, ENERGY);
 		float4 pn;
 		if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
-			pn = bd_update(p, f, types, nullptr, P);
+			const float4 pmf = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, nullptr, P.ef,
+														P.scheme);
+			pn = bd_update(p, f, types, pmf, P);
 		} else {
 			float4 mo = mom[t];
 			pn = baoab_update(p, mo, f, types, P);

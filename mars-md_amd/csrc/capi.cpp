@@ -469,6 +469,37 @@ marsmd_status marsmd_patch_get_kernel_timing(marsmd_patch* p, double* total_ms, 
 	});
 }
 
+marsmd_status marsmd_patch_set_launch(marsmd_patch* p, const marsmd_patch_launch* launch) {
+	return guard([&] {
+		need(p, "patch");
+		need(launch, "launch");
+		p->impl.set_launch(launch->blocks_per_cu, launch->split_pmf != 0);
+	});
+}
+
+marsmd_status marsmd_patch_get_launch(marsmd_patch* p, marsmd_patch_launch* launch) {
+	return guard([&] {
+		need(p, "patch");
+		need(launch, "launch");
+		launch->blocks_per_cu = p->impl.blocks_per_cu;
+		launch->split_pmf = p->impl.split_pmf ? 1 : 0;
+	});
+}
+
+marsmd_status marsmd_patch_autotune(marsmd_patch* p, const marsmd_run_params* params, uint64_t first_step,
+									uint64_t num_steps, marsmd_patch_launch* best, double* best_ms_per_step,
+									marsmd_stream stream) {
+	return guard([&] {
+		need(p, "patch");
+		need(params, "params");
+		need(best, "best");
+		need(best_ms_per_step, "best_ms_per_step");
+		p->impl.autotune(*params, first_step, num_steps, st(stream), best_ms_per_step);
+		best->blocks_per_cu = p->impl.blocks_per_cu;
+		best->split_pmf = p->impl.split_pmf ? 1 : 0;
+	});
+}
+
 // ------------------------------------------------------------------ replica batch
 struct marsmd_batch {
 	mmd::BatchImpl impl;

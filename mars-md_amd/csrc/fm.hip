@@ -31,12 +31,12 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const float4* __restrict
 														uint64_t n, NbTables T, const float* __restrict__ knots_g,
 														DevBox box, float cutoff2, float4* __restrict__ out) {
 	extern __shared__ float lds_knots[];
-	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
-	const float* knots = stage_tables(T, knots_g, lds_knots, lds_tag);
+	__shared__ NbLds lds_nb;
+	const float* knots = stage_tables(T, knots_g, lds_knots, lds_nb);
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
-	const float4 f = nb_force(uint32_t(t), spos[t], spos, L, T, knots, lds_tag, box, cutoff2, true);
+	const float4 f = nb_force(uint32_t(t), spos[t], spos, L, T, knots, lds_nb, box, cutoff2, true);
 	out[sorted_idx[t]] = f;
 }
 
@@ -48,10 +48,8 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restric
 														 float cutoff2, const float4* __restrict__ grad,
 														 double* __restrict__ partial) {
 	extern __shared__ double lds_grad[];
-	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
-	const int nt2 = T.num_types * T.num_types;
-	for (int i = threadIdx.x; i < nt2; i += blockDim.x)
-		lds_tag[i] = T.tag[i];
+	__shared__ NbLds lds_nb;
+	stage_params(T, lds_nb);
 	double* acc = LDS ? lds_grad : partial;
 	if (LDS)
 		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
@@ -72,7 +70,7 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restric
 			const uint32_t cj = meta_canon(mj);
 			if (!(ct < cj))
 				continue; // the pair is taken from its lower canonical end
-			const uint32_t tag = lds_tag[tt * uint32_t(T.num_types) + meta_type(mj)];
+			const uint32_t tag = lds_nb.tag[tt * uint32_t(T.num_types) + meta_type(mj)];
 			if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
 				continue;
 			const float3 r = wrap_diff(box, sub3(q, p));
@@ -83,18 +81,18 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restric
 				continue;
 			const float inv = 1.0f / d;
 			const float ux = inv * r.x, uy = inv * r.y, uz = inv * r.z;
-			const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
-			const float w = (d - T.start[ti]) * T.step_inv[ti];
+			const float4 tp = lds_nb.tab[tag >> MARSMD_PAIR_TABLE_SHIFT]; // {start, step_inv, size, offset}
+			const float w = (d - tp.x) * tp.y;
 			int h = int(floorf(w));
-			if (h >= T.size[ti] - 1)
+			if (h >= __float_as_int(tp.z) - 1)
 				continue;
 			if (h < 0)
 				h = 0;
 			const float4 gy = grad[cj];
 			const double dfm = (double(gy.x) - double(gx.x)) * double(ux) + (double(gy.y) - double(gx.y)) * double(uy) +
 							   (double(gy.z) - double(gx.z)) * double(uz);
-			const double g = dfm * double(T.step_inv[ti]);
-			const int k0 = T.offset[ti] + h;
+			const double g = dfm * double(tp.y);
+			const int k0 = __float_as_int(tp.w) + h;
 			if (LDS) {
 				atomicAdd(&acc[k0], g);
 				atomicAdd(&acc[k0 + 1], -g);

@@ -60,8 +60,11 @@ __global__ void k_pairwise(const int2* __restrict__ pairs, const float4* __restr
 	if (cutoff2 > 0.0f && d * d > cutoff2)
 		return;
 	float fm = 0.0f, en = 0.0f;
-	if (ANALYTIC && (terms & kAnalyticTerms))
-		analytic_terms(terms, d, r2, uint32_t(type_ids[ij.x]), uint32_t(type_ids[ij.y]), C, fm, en);
+	if (ANALYTIC && (terms & kAnalyticTerms)) {
+		const int ti = type_ids[ij.x], tj = type_ids[ij.y];
+		analytic_terms(terms, d, r2, C.charge[ti], C.charge[tj], C.radius[ti], C.radius[tj], C.eps[ti], C.eps[tj],
+					   C.prm, fm, en);
+	}
 	if (TABULATED && (terms & MARSMD_PAIR_TERM_TABULATED)) {
 		const marsmd_table t = tables[tag >> MARSMD_PAIR_TABLE_SHIFT];
 		const FE v = table_compute(d, t.pot, int(t.size), t.start, t.step_inv, t.is_periodic != 0);

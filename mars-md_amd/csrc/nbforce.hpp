@@ -25,6 +25,15 @@ struct NbTables {
 	PairCoeffs coef; // read by the analytical terms only
 };
 
+// What a pair term reads besides positions and knots, staged in LDS once per
+// workgroup: parameters indexed per lane (table index, particle type) would otherwise
+// be vector loads from the kernel-argument block on every pair.
+struct NbLds {
+	uint32_t tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+	float4 tab[kMaxTables];        // {start, step_inv, size, offset} (size, offset as int bits)
+	float4 coef[MARSMD_MAX_TYPES]; // {charge, softcore radius, softcore eps, 0}
+};
+
 __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
 
 // One pair term on slot t (PairNonbondedComputer, Pairwise.h:184-305) in canonical
@@ -34,7 +43,7 @@ __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
 // tabulated-only path carries none of the analytical code).
 template<bool ANALYTIC>
 __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t tt, const float4& q,
-										const NbTables& T, const float* __restrict__ knots, const uint32_t* tagm,
+										const NbTables& T, const float* __restrict__ knots, const NbLds& S,
 										const DevBox& box, float cutoff2, bool energy, float& fx, float& fy,
 										float& fz, float& e) {
 	const uint32_t mj = fb(q.w);
@@ -45,7 +54,7 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 	const float4 py = t_is_x ? q : p;
 	const uint32_t tx = t_is_x ? tt : tj;
 	const uint32_t ty = t_is_x ? tj : tt;
-	const uint32_t tag = tagm[tx * uint32_t(T.num_types) + ty];
+	const uint32_t tag = S.tag[tx * uint32_t(T.num_types) + ty];
 	const uint32_t terms = tag & T.enabled;
 	if (ANALYTIC ? terms == 0u : !(terms & MARSMD_PAIR_TERM_TABULATED))
 		return;
@@ -64,11 +73,13 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 	if (cutoff2 > 0.0f && d * d > cutoff2)
 		return;
 	float fm = 0.0f, en = 0.0f;
-	if (ANALYTIC && (terms & kAnalyticTerms))
-		analytic_terms(terms, d, r2, tx, ty, T.coef, fm, en);
+	if (ANALYTIC && (terms & kAnalyticTerms)) {
+		const float4 cx = S.coef[tx], cy = S.coef[ty];
+		analytic_terms(terms, d, r2, cx.x, cy.x, cx.y, cy.y, cx.z, cy.z, T.coef.prm, fm, en);
+	}
 	if (terms & MARSMD_PAIR_TERM_TABULATED) {
-		const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
-		const FE v = table_compute(d, knots + T.offset[ti], T.size[ti], T.start[ti], T.step_inv[ti], false);
+		const float4 tp = S.tab[tag >> MARSMD_PAIR_TABLE_SHIFT];
+		const FE v = table_compute(d, knots + __float_as_int(tp.w), __float_as_int(tp.z), tp.x, tp.y, false);
 		fm += v.f;
 		en += v.e;
 	}
@@ -93,7 +104,7 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 template<bool ANALYTIC = false>
 __device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const float4* __restrict__ pos,
 										   const NbrList& L, const NbTables& T, const float* __restrict__ knots,
-										   const uint32_t* tagm, const DevBox& box, float cutoff2, bool energy) {
+										   const NbLds& S, const DevBox& box, float cutoff2, bool energy) {
 	const uint32_t mt = fb(p.w);
 	const uint32_t ct = meta_canon(mt);
 	const uint32_t tt = meta_type(mt);
@@ -116,10 +127,10 @@ __device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const fl
 			if (g + 2 < ng)
 				jnn = row[uint64_t(g + 2) * 64u];
 		}
-		nb_pair<ANALYTIC>(p, ct, tt, q0, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, q1, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, q2, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, q3, T, knots, tagm, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, q0, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, q1, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, q2, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, q3, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
 		q0 = n0;
 		q1 = n1;
 		q2 = n2;
@@ -129,12 +140,33 @@ __device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const fl
 	return make_float4(fx, fy, fz, e);
 }
 
-// Stage table knots and the tag matrix in LDS.
-__device__ __forceinline__ const float* stage_tables(const NbTables& T, const float* __restrict__ knots_g,
-													 float* lds_knots, uint32_t* lds_tag) {
+// Stage the tag matrix, the table parameters and the per-type coefficients in LDS (the
+// caller synchronises).
+__device__ __forceinline__ void stage_params(const NbTables& T, NbLds& S) {
 	const int nt2 = T.num_types * T.num_types;
 	for (int i = threadIdx.x; i < nt2; i += blockDim.x)
-		lds_tag[i] = T.tag[i];
+		S.tag[i] = T.tag[i];
+	for (int i = threadIdx.x; i < T.num_tables; i += blockDim.x)
+		S.tab[i] = make_float4(T.start[i], T.step_inv[i], __int_as_float(T.size[i]), __int_as_float(T.offset[i]));
+	for (int i = threadIdx.x; i < T.num_types; i += blockDim.x)
+		S.coef[i] = make_float4(T.coef.charge[i], T.coef.radius[i], T.coef.eps[i], 0.0f);
+}
+
+// Copy the particle-type record into LDS: the integrators and the PMF index it per
+// lane (mass, damping, diffusion, charge, grid terms), which from the kernel-argument
+// block would be vector loads (the caller synchronises).
+__device__ __forceinline__ void stage_types(const marsmd_particle_types& src, marsmd_particle_types& dst) {
+	static_assert(sizeof(marsmd_particle_types) % 4 == 0, "4-byte fields");
+	const uint32_t* a = reinterpret_cast<const uint32_t*>(&src);
+	uint32_t* b = reinterpret_cast<uint32_t*>(&dst);
+	for (int i = threadIdx.x; i < int(sizeof(marsmd_particle_types) / 4); i += blockDim.x)
+		b[i] = a[i];
+}
+
+// Stage table knots (when they fit) and the pair parameters in LDS.
+__device__ __forceinline__ const float* stage_tables(const NbTables& T, const float* __restrict__ knots_g,
+													 float* lds_knots, NbLds& S) {
+	stage_params(T, S);
 	if (T.lds) {
 		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
 			lds_knots[i] = knots_g[i];

@@ -44,12 +44,12 @@ __global__ __launch_bounds__(kBlock) void k_nb_forces(const float4* __restrict__
 													  NbTables T, const float* __restrict__ knots_g, DevBox box,
 													  float cutoff2, int energy, float4* __restrict__ force) {
 	extern __shared__ float lds_knots[];
-	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
-	const float* knots = stage_tables(T, knots_g, lds_knots, lds_tag);
+	__shared__ NbLds lds_nb;
+	const float* knots = stage_tables(T, knots_g, lds_knots, lds_nb);
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
-	force[t] = nb_force<ANALYTIC>(uint32_t(t), pos[t], pos, L, T, knots, lds_tag, box, cutoff2, energy != 0);
+	force[t] = nb_force<ANALYTIC>(uint32_t(t), pos[t], pos, L, T, knots, lds_nb, box, cutoff2, energy != 0);
 }
 
 __global__ void k_zero_force(float4* __restrict__ f, uint64_t n) {
@@ -65,9 +65,32 @@ __global__ __launch_bounds__(kBlock) void k_integrate_bd(float4* __restrict__ po
 	if (t >= n)
 		return;
 	float4 f = force[t];
-	const float4 p = bd_update(pos[t], f, types, S.has_grids ? grids : nullptr, S);
+	const float4 p0 = pos[t];
+	const float4 pmf = position_dependent_force(p0.x, p0.y, p0.z, int(meta_type(fb(p0.w))), types,
+												S.has_grids ? grids : nullptr, S.ef, S.scheme);
+	const float4 p = bd_update(p0, f, types, pmf, S);
 	pos[t] = p;
 	force[t].w = f.w;
+}
+
+// The position-dependent force of every particle (compute_position_dependent_force,
+// Pmf.h:68-124: grid terms and the electric field) for a step kernel that reads it
+// instead of sampling the grids itself: the same function of the same inputs, so the
+// step's results are the same bits.
+__global__ __launch_bounds__(kBlock) void k_pmf_force(const float4* __restrict__ pos, uint64_t n,
+													  marsmd_particle_types types_arg,
+													  const marsmd_grid_view* __restrict__ grids, StepParams S,
+													  float4* __restrict__ out) {
+	__shared__ marsmd_particle_types lds_types;
+	stage_types(types_arg, lds_types);
+	__syncthreads();
+	const marsmd_particle_types& types = lds_types;
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const float4 p = pos[t];
+	out[t] = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, S.has_grids ? grids : nullptr,
+									  S.ef, S.scheme);
 }
 
 __global__ __launch_bounds__(kBlock) void k_integrate_pmf(const float4* __restrict__ pos, float4* __restrict__ force,
@@ -117,23 +140,26 @@ __global__ __launch_bounds__(kBlock) void k_integrate_baoab(float4* __restrict__
 // persistent over XCD-local tiles so the table knots are staged once per workgroup.
 // Bounded to 128 VGPRs (4 waves per SIMD; the tricubic sampler alone would take ~140
 // and leave 3): measured faster despite a few dwords of spill.
-template<int INTEG, bool ENERGY, bool ANALYTIC>
+template<int INTEG, bool ENERGY, bool ANALYTIC, bool PMF_IN>
 __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ pos, float4* __restrict__ pos_next,
 												 float4* __restrict__ mom, float4* __restrict__ force, uint64_t n,
 												 NbrList L, NbTables T, const float* __restrict__ knots_g,
-												 marsmd_particle_types types,
+												 marsmd_particle_types types_arg,
 												 const marsmd_grid_view* __restrict__ grids, StepParams S,
-												 int has_bonded, int store_force) {
+												 int has_bonded, int store_force, const float4* __restrict__ pmf_in) {
 	extern __shared__ float lds_knots[];
-	__shared__ uint32_t lds_tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
-	const float* knots = stage_tables(T, knots_g, lds_knots, lds_tag);
+	__shared__ NbLds lds_nb;
+	__shared__ marsmd_particle_types lds_types;
+	stage_types(types_arg, lds_types);
+	const float* knots = stage_tables(T, knots_g, lds_knots, lds_nb);
+	const marsmd_particle_types& types = lds_types;
 	const TileRange R = xcd_tiles(uint32_t((n + kBlock - 1) / kBlock));
 	for (uint32_t tile = R.tile; tile < R.end; tile += R.stride) {
 		const uint64_t t = uint64_t(tile) * kBlock + threadIdx.x;
 		if (t >= n)
 			continue;
 		const float4 p = pos[t];
-		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, knots, lds_tag, S.box, S.cutoff2, ENERGY);
+		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, knots, lds_nb, S.box, S.cutoff2, ENERGY);
 		if (has_bonded) {
 			const float4 b = force[t];
 			f.x += b.x;
@@ -143,11 +169,15 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 		}
 		float4 pn;
 		if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
-			pn = bd_update(p, f, types, S.has_grids ? grids : nullptr, S);
+			const float4 pmf = PMF_IN ? pmf_in[t]
+									  : position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types,
+																 S.has_grids ? grids : nullptr, S.ef, S.scheme);
+			pn = bd_update(p, f, types, pmf, S);
 		} else {
 			if (S.has_grids) {
-				const float4 g = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, grids,
-														  S.ef, S.scheme);
+				const float4 g = PMF_IN ? pmf_in[t]
+										: position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types,
+																   grids, S.ef, S.scheme);
 				f.x += g.x;
 				f.y += g.y;
 				f.z += g.z;
@@ -161,6 +191,26 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 		if (ENERGY || store_force)
 			force[t] = f;
 	}
+}
+
+using StepFn = decltype(&k_step<MARSMD_INTEGRATOR_BROWNIAN, false, false, false>);
+template<int I, bool E, bool A>
+StepFn pick_step_pmf(bool pmf_in) {
+	return pmf_in ? k_step<I, E, A, true> : k_step<I, E, A, false>;
+}
+template<int I, bool E>
+StepFn pick_step_an(bool an, bool pmf_in) {
+	return an ? pick_step_pmf<I, E, true>(pmf_in) : pick_step_pmf<I, E, false>(pmf_in);
+}
+template<int I>
+StepFn pick_step_e(bool energy, bool an, bool pmf_in) {
+	return energy ? pick_step_an<I, true>(an, pmf_in) : pick_step_an<I, false>(an, pmf_in);
+}
+// The fused step kernel for an integrator, energy step, analytical terms and whether
+// the position-dependent force comes precomputed.
+StepFn pick_step(bool bd, bool energy, bool an, bool pmf_in) {
+	return bd ? pick_step_e<MARSMD_INTEGRATOR_BROWNIAN>(energy, an, pmf_in)
+			  : pick_step_e<MARSMD_INTEGRATOR_LANGEVIN>(energy, an, pmf_in);
 }
 
 // Permute slot-indexed state into the new sorted order:
@@ -640,7 +690,8 @@ unsigned PatchImpl::persistent_grid(const void* kernel, uint64_t nn) {
 		MMD_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
 		num_cus = std::max(num_cus, 1);
 	}
-	uint64_t g = uint64_t(it->second.second) * uint64_t(num_cus);
+	const int per_cu = blocks_per_cu > 0 ? std::min(blocks_per_cu, it->second.second) : it->second.second;
+	uint64_t g = uint64_t(per_cu) * uint64_t(num_cus);
 	g = std::min<uint64_t>(g, (tiles + 7) / 8 * 8);
 	g = std::max<uint64_t>((g + 7) / 8 * 8, 8);
 	return unsigned(g);
@@ -818,15 +869,12 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 		float4* pnext = pos[1 - cur].data();
 		const bool bd = P.integrator == MARSMD_INTEGRATOR_BROWNIAN;
 		const bool an = (tables->nb.enabled & kAnalyticTerms) != 0;
-		auto kern = bd ? (energy ? (an ? k_step<MARSMD_INTEGRATOR_BROWNIAN, true, true>
-									   : k_step<MARSMD_INTEGRATOR_BROWNIAN, true, false>)
-								 : (an ? k_step<MARSMD_INTEGRATOR_BROWNIAN, false, true>
-									   : k_step<MARSMD_INTEGRATOR_BROWNIAN, false, false>))
-					   : (energy ? (an ? k_step<MARSMD_INTEGRATOR_LANGEVIN, true, true>
-									   : k_step<MARSMD_INTEGRATOR_LANGEVIN, true, false>)
-								 : (an ? k_step<MARSMD_INTEGRATOR_LANGEVIN, false, true>
-									   : k_step<MARSMD_INTEGRATOR_LANGEVIN, false, false>));
+		// with grids, the position-dependent force may come from its own kernel
+		const bool pmf_in = split_pmf && num_grids > 0;
+		const StepFn kern = pick_step(bd, energy, an, pmf_in);
 		const unsigned grid = persistent_grid(reinterpret_cast<const void*>(kern), n);
+		if (pmf_in)
+			pmf_buf.resize(n);
 		if (timing) {
 			while (timing_events.size() < timing_used + 2) {
 				hipEvent_t e;
@@ -835,9 +883,13 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 			}
 			MMD_HIP(hipEventRecord(timing_events[timing_used], s));
 		}
+		if (pmf_in) {
+			k_pmf_force<<<g, kBlock, 0, s>>>(pos[cur].data(), n, types, grids_dev.data(), S, pmf_buf.data());
+			MMD_LAUNCH_CHECK();
+		}
 		kern<<<grid, kBlock, lds_bytes(), s>>>(pos[cur].data(), pnext, mom.data(), force.data(), n, pl->list(),
 											   tables->nb, tables->nb_knots.data(), types, grids_dev.data(), S,
-											   has_bonded, store);
+											   has_bonded, store, pmf_in ? pmf_buf.data() : nullptr);
 		if (timing) {
 			MMD_HIP(hipEventRecord(timing_events[timing_used + 1], s));
 			timing_used += 2;
@@ -847,6 +899,72 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 		MMD_LAUNCH_CHECK();
 		cur = 1 - cur;
 	}
+}
+
+void PatchImpl::set_launch(int bpc, bool split) {
+	if (bpc < 0)
+		raise(MARSMD_ERR_VALUE, "Patch: blocks_per_cu must be >= 0");
+	blocks_per_cu = bpc;
+	split_pmf = split;
+}
+
+void PatchImpl::autotune(const marsmd_run_params& P, uint64_t first_step, uint64_t num_steps, hipStream_t s,
+						 double* best_ms) {
+	// candidates: step-kernel workgroups per CU from the occupancy down, with the
+	// position-dependent force sampled inside the step kernel or by its own kernel (when
+	// there are grids); every candidate computes the same bits
+	const bool bd = P.integrator == MARSMD_INTEGRATOR_BROWNIAN;
+	const bool an = (tables->nb.enabled & kAnalyticTerms) != 0;
+	struct Cand {
+		int bpc;
+		bool split;
+		double ms = 1e300;
+	};
+	std::vector<Cand> cands;
+	for (bool sp : {false, true}) {
+		if (sp && num_grids == 0)
+			continue;
+		// each step kernel's own occupancy (the split one holds fewer registers), then less
+		const StepFn probe = pick_step(bd, false, an, sp);
+		const int saved = blocks_per_cu;
+		blocks_per_cu = 0;
+		(void)persistent_grid(reinterpret_cast<const void*>(probe), std::max<uint64_t>(n, 1));
+		blocks_per_cu = saved;
+		const int occ = occupancy[reinterpret_cast<const void*>(probe)].second;
+		std::vector<int> per_cu = {occ};
+		for (int v : {occ - 1, occ / 2})
+			if (v >= 1 && std::find(per_cu.begin(), per_cu.end(), v) == per_cu.end())
+				per_cu.push_back(v);
+		for (int v : per_cu)
+			cands.push_back(Cand{v, sp});
+	}
+	const bool was_timing = timing;
+	const uint64_t passes = 2;
+	const uint64_t chunk = num_steps / (passes * cands.size());
+	uint64_t step = first_step;
+	if (chunk >= 1) {
+		for (uint64_t pass = 0; pass < passes; ++pass)
+			for (Cand& c : cands) {
+				set_launch(c.bpc, c.split);
+				set_kernel_timing(true);
+				run(P, step, chunk, s);
+				step += chunk;
+				double ms = 0.0;
+				uint64_t k = 0;
+				kernel_timing(&ms, &k);
+				if (k)
+					c.ms = std::min(c.ms, ms / double(k));
+			}
+	}
+	const Cand* best = &cands.front();
+	for (const Cand& c : cands)
+		if (c.ms < best->ms)
+			best = &c;
+	set_launch(best->bpc, best->split);
+	set_kernel_timing(was_timing);
+	if (step < first_step + num_steps)
+		run(P, step, first_step + num_steps - step, s);
+	*best_ms = best->ms < 1e300 ? best->ms : 0.0;
 }
 
 } // namespace mmd

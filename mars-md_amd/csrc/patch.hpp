@@ -51,6 +51,11 @@ struct PatchImpl {
 	bool timing = false;
 	std::vector<hipEvent_t> timing_events; // pairs: start, stop
 	size_t timing_used = 0;
+	// launch configuration: persistent step-kernel workgroups per CU (0: as many as
+	// fit) and whether a kernel of its own precomputes the position-dependent force
+	int blocks_per_cu = 0;
+	bool split_pmf = false;
+	DevArray<float4> pmf_buf;
 
 	explicit PatchImpl(const marsmd_patch_desc& d);
 	~PatchImpl();
@@ -68,6 +73,11 @@ struct PatchImpl {
 	void run(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s);
 	void set_kernel_timing(bool enable);
 	void kernel_timing(double* total_ms, uint64_t* launches);
+	void set_launch(int blocks_per_cu, bool split_pmf);
+	// Steps num_steps real steps while timing every candidate launch configuration on
+	// the step kernels, and keeps the fastest (its ms per step in *best_ms).
+	void autotune(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s,
+				  double* best_ms);
 
   private:
 	void build(float pairlist_cutoff, hipStream_t s);

@@ -30,12 +30,13 @@ __device__ __forceinline__ float4 wrap_pos(const DevBox& box, float4 p) {
 	return p;
 }
 
-// BDIntegrate with the PMF fused (BD.h:44-92); f is ForceEnergy (xyz force, w energy).
-__device__ __forceinline__ float4 bd_update(float4 p, float4& f, const marsmd_particle_types& types,
-										   const marsmd_grid_view* grids, const StepParams& S) {
+// BDIntegrate (BD.h:44-92) given the particle's position-dependent force `pmf`
+// (compute_position_dependent_force, Pmf.h:68-124: grid terms and the electric field);
+// f is ForceEnergy (xyz force, w energy).
+__device__ __forceinline__ float4 bd_update(float4 p, float4& f, const marsmd_particle_types& types, const float4 pmf,
+										   const StepParams& S) {
 	const uint32_t m = fb(p.w);
 	const int ty = int(meta_type(m));
-	const float4 pmf = position_dependent_force(p.x, p.y, p.z, ty, types, grids, S.ef, S.scheme);
 	const float Fx = f.x + pmf.x, Fy = f.y + pmf.y, Fz = f.z + pmf.z;
 	f.w += pmf.w;
 	const float* D = types.diffusion[ty];

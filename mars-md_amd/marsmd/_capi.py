@@ -188,6 +188,10 @@ class BatchLaunch(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int32), ("use_graph", C.c_int32)]
 
 
+class PatchLaunch(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int32), ("split_pmf", C.c_int32)]
+
+
 class FMDesc(C.Structure):
     _fields_ = [
         ("box", Box),
@@ -282,6 +286,10 @@ PROTOTYPES = {
     "marsmd_patch_run": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, _P]),
     "marsmd_patch_set_kernel_timing": (_I32, [_P, _I32]),
     "marsmd_patch_get_kernel_timing": (_I32, [_P, C.POINTER(C.c_double), C.POINTER(_U64)]),
+    "marsmd_patch_set_launch": (_I32, [_P, C.POINTER(PatchLaunch)]),
+    "marsmd_patch_get_launch": (_I32, [_P, C.POINTER(PatchLaunch)]),
+    "marsmd_patch_autotune": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, C.POINTER(PatchLaunch),
+                                     C.POINTER(C.c_double), _P]),
     "marsmd_batch_create": (_I32, [C.POINTER(BatchDesc), C.POINTER(_P)]),
     "marsmd_batch_destroy": (_I32, [_P]),
     "marsmd_batch_set_base_seed": (_I32, [_P, _U64]),

@@ -32,6 +32,9 @@ class Batch:
         if not specs:
             raise ValueError("a batch needs at least one replica")
         first = specs[0]
+        if any(s.seed != first.seed for s in specs):
+            raise ValueError("the replicas of a batch share the base seed (their patch ids tell their random "
+                             "streams apart)")
         for s in specs[1:]:
             if (len(s.types) != len(first.types) or len(s.nb_tables) != len(first.nb_tables)
                     or not np.array_equal(s.tag_matrix(), first.tag_matrix()) or s.pair_params != first.pair_params):

@@ -93,6 +93,29 @@ class Patch:
         api.call("marsmd_patch_get_kernel_timing", self._h, C.byref(ms), C.byref(k))
         return ms.value, k.value
 
+    def set_launch(self, blocks_per_cu: int = 0, split_pmf: bool = False):
+        """Launch configuration of run()'s step: persistent workgroups per CU (0: as many
+        as fit) and, with grids, the position-dependent force from a kernel of its own.
+        Results are the same bits whatever the configuration."""
+        lc = api.PatchLaunch(int(blocks_per_cu), 1 if split_pmf else 0)
+        api.call("marsmd_patch_set_launch", self._h, C.byref(lc))
+
+    def get_launch(self) -> dict:
+        lc = api.PatchLaunch()
+        api.call("marsmd_patch_get_launch", self._h, C.byref(lc))
+        return {"blocks_per_cu": lc.blocks_per_cu, "split_pmf": bool(lc.split_pmf)}
+
+    def autotune(self, num_steps: int, first_step: int = 0, params: Optional[RunParams] = None, stream=None) -> dict:
+        """Steps first_step+1 .. first_step+num_steps (real steps, as run) while timing
+        every launch candidate on the step kernels, and keeps the fastest. Returns it
+        with its ms per step."""
+        p = (params or self.spec.params).to_c()
+        lc = api.PatchLaunch()
+        ms = C.c_double(0.0)
+        api.call("marsmd_patch_autotune", self._h, C.byref(p), first_step, num_steps, C.byref(lc), C.byref(ms),
+                 _stream(stream))
+        return {"blocks_per_cu": lc.blocks_per_cu, "split_pmf": bool(lc.split_pmf), "ms_per_step": ms.value}
+
     def maybe_reorder(self, step: int, params: Optional[RunParams] = None, stream=None) -> bool:
         """ParticleReorderManager::maybe_reorder (ReorderManager.cpp:8-25) with the
         reorderPeriod fallback of SimManager::execute_force_calculation (:343-349)."""

@@ -8,6 +8,7 @@ from pathlib import Path
 import pytest
 
 from marsmd import _capi as api
+from marsmd import configs
 
 ROOT = Path(__file__).resolve().parent.parent
 HEADER = ROOT / "include" / "marsmd.h"
@@ -56,6 +57,8 @@ STRUCTS = {
     "marsmd_fm_desc": api.FMDesc,
     "marsmd_batch_desc": api.BatchDesc,
     "marsmd_batch_launch": api.BatchLaunch,
+    "marsmd_patch_launch": api.PatchLaunch,
+    "marsmd_pair_params": api.PairParams,
 }
 
 
@@ -95,3 +98,13 @@ def test_cpp_interface_compiles(tmp_path):
                         "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", f"-I{ROOT / 'mars-md_amd' / 'host'}",
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_batch_rejects_replicas_with_different_seeds():
+    # checked on the host before the library is called
+    from marsmd.batch import Batch
+
+    a = configs.config0(n=100, L=60.0, seed=1)
+    b = configs.config0(n=120, L=60.0, seed=2)
+    with pytest.raises(ValueError, match="base seed"):
+        Batch([a, b])

@@ -107,6 +107,7 @@ def test_fused_run_and_batch_with_analytical_terms(cuda):
     specs = [analytic_spec(n=3000, L=155.0, seed=21), analytic_spec(n=2200, L=140.0, seed=22)]
     for r, s in enumerate(specs):
         s.patch_id = r
+        s.seed = 42  # the batch's base seed is shared; the patch id tells the streams apart
     steps = 25
     b = Batch(specs)
     b.run(steps)

@@ -78,6 +78,7 @@ def test_incomplete_reference_grid_after_pair_buffer_growth(cuda):
     seam = configs.config0(n=n, L=L, seed=5)
     seam.positions = seam_pair_positions(n, L)[:, :3].copy()
     small = _specs()[0]
+    seam.seed = small.seed  # one base seed per batch
     params = configs.RunParams(cutoff=rc, pairlist_cutoff=rc, rebuild_period=20, timestep=0.0, kT=configs.KT)
     b = Batch([seam, small])
     b.run(1, params=params)
