@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restr
 	__shared__ NbLds lds_nb;
 	__shared__ marsmd_particle_types lds_types;
 	stage_types(types_arg, lds_types);
-	const float* knots = stage_tables(T, knots_g, lds_knots, lds_nb);
+	stage_tables(T, knots_g, lds_knots, lds_nb);
 	const marsmd_particle_types& types = lds_types;
 	// a replayed segment carries the step's offset in S.step and its first step in memory
 	const uint32_t step = S.step + (step_base ? step_base[0] : 0u);
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restr
 		P.step = step;
 		P.seed = mix_seed(base_seed, G.pid, uint64_t(step));
 		const float4 p = pos[t];
-		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, knots, lds_nb, P.box, P.cutoff2, ENERGY);
+		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, lds_knots, knots_g, lds_nb, P.box, P.cutoff2, ENERGY);
 		float4 pn;
 		if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
 			const float4 pmf = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, nullptr, P.ef,

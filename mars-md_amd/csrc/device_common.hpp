@@ -299,18 +299,30 @@ struct __attribute__((aligned(4))) GridRun4 {
 	float v[4];
 };
 
+// Grid values live in global memory; the view's pointer comes from memory, so it is
+// cast to the global address space (global loads instead of flat ones, whose waits
+// would also cover LDS traffic).
+// (The host pass of the compile only parses these device functions.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MMD_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define MMD_GLOBAL_AS
+#endif
+typedef MMD_GLOBAL_AS const float gfloat;
+__device__ __forceinline__ gfloat* grid_data(const marsmd_grid_view& g) { return (gfloat*)g.data; }
+
 // K consecutive z taps starting at jz0 of the row at `row`: a single load when the run
 // lies inside the grid, else tap by tap through the boundary mapping.
 template<int K>
-__device__ __forceinline__ void load_run(const float* __restrict__ row, int jz0, int nz, int bc, float out[K]) {
+__device__ __forceinline__ void load_run(gfloat* __restrict__ row, int jz0, int nz, int bc, float out[K]) {
 	if (jz0 >= 0 && jz0 + K <= nz) {
 		if constexpr (K == 4) {
-			const GridRun4 r = *reinterpret_cast<const GridRun4*>(row + jz0);
+			const GridRun4 r = *(MMD_GLOBAL_AS const GridRun4*)(row + jz0);
 #pragma unroll
 			for (int q = 0; q < 4; ++q)
 				out[q] = r.v[q];
 		} else {
-			const GridRun2 r = *reinterpret_cast<const GridRun2*>(row + jz0);
+			const GridRun2 r = *(MMD_GLOBAL_AS const GridRun2*)(row + jz0);
 #pragma unroll
 			for (int q = 0; q < 2; ++q)
 				out[q] = r.v[q];
@@ -344,7 +356,7 @@ __device__ inline GridSample sample_linear(const marsmd_grid_view& g, float px, 
 				r[ix][iy][0] = 0.0f;
 				r[ix][iy][1] = 0.0f;
 			} else {
-				load_run<2>(g.data + ox + oy, k0, nz, bc, r[ix][iy]);
+				load_run<2>(grid_data(g) + ox + oy, k0, nz, bc, r[ix][iy]);
 			}
 		}
 	}
@@ -412,7 +424,7 @@ __device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, f
 				for (int q = 0; q < 4; ++q)
 					r[ix][q] = 0.0f;
 			} else {
-				load_run<4>(g.data + ox[ix] + oy, hz - 1, nz, bc, r[ix]);
+				load_run<4>(grid_data(g) + ox[ix] + oy, hz - 1, nz, bc, r[ix]);
 			}
 		}
 #pragma unroll

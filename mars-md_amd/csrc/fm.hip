@@ -32,11 +32,11 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const float4* __restrict
 														DevBox box, float cutoff2, float4* __restrict__ out) {
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
-	const float* knots = stage_tables(T, knots_g, lds_knots, lds_nb);
+	stage_tables(T, knots_g, lds_knots, lds_nb);
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
-	const float4 f = nb_force(uint32_t(t), spos[t], spos, L, T, knots, lds_nb, box, cutoff2, true);
+	const float4 f = nb_force(uint32_t(t), spos[t], spos, L, T, lds_knots, knots_g, lds_nb, box, cutoff2, true);
 	out[sorted_idx[t]] = f;
 }
 

@@ -40,7 +40,9 @@ __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
 // orientation: r = wrap_diff(pos[higher canon] - pos[lower canon]), the lower takes -F.
 // ANALYTIC adds the analytical terms enabled by the pair's tag; without it only the
 // tabulated term is evaluated (the kernels are instantiated per case, so the
-// tabulated-only path carries none of the analytical code).
+// tabulated-only path carries none of the analytical code). `knots` points into LDS or
+// global memory, as the caller's instantiation says: a pointer that might be either
+// would be read with flat loads, whose waits also drain the position gathers in flight.
 template<bool ANALYTIC>
 __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t tt, const float4& q,
 										const NbTables& T, const float* __restrict__ knots, const NbLds& S,
@@ -102,7 +104,7 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 // deep: while group g is evaluated, the positions of group g+1 and the indices of
 // group g+2 are in flight. The padding entries of a row are t itself (skipped as d = 0).
 template<bool ANALYTIC = false>
-__device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const float4* __restrict__ pos,
+__device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, const float4* __restrict__ pos,
 										   const NbrList& L, const NbTables& T, const float* __restrict__ knots,
 										   const NbLds& S, const DevBox& box, float cutoff2, bool energy) {
 	const uint32_t mt = fb(p.w);
@@ -164,15 +166,26 @@ __device__ __forceinline__ void stage_types(const marsmd_particle_types& src, ma
 }
 
 // Stage table knots (when they fit) and the pair parameters in LDS.
-__device__ __forceinline__ const float* stage_tables(const NbTables& T, const float* __restrict__ knots_g,
-													 float* lds_knots, NbLds& S) {
+__device__ __forceinline__ void stage_tables(const NbTables& T, const float* __restrict__ knots_g, float* lds_knots,
+											 NbLds& S) {
 	stage_params(T, S);
 	if (T.lds) {
 		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
 			lds_knots[i] = knots_g[i];
 	}
 	__syncthreads();
-	return T.lds ? lds_knots : knots_g;
+}
+
+// nb_force_list with the knots read from LDS when staged there, else from global
+// memory: each branch sees one address space.
+template<bool ANALYTIC = false>
+__device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const float4* __restrict__ pos,
+										   const NbrList& L, const NbTables& T, const float* lds_knots,
+										   const float* __restrict__ knots_g, const NbLds& S, const DevBox& box,
+										   float cutoff2, bool energy) {
+	if (T.lds)
+		return nb_force_list<ANALYTIC>(t, p, pos, L, T, lds_knots, S, box, cutoff2, energy);
+	return nb_force_list<ANALYTIC>(t, p, pos, L, T, knots_g, S, box, cutoff2, energy);
 }
 
 } // namespace mmd

@@ -45,11 +45,11 @@ __global__ __launch_bounds__(kBlock) void k_nb_forces(const float4* __restrict__
 													  float cutoff2, int energy, float4* __restrict__ force) {
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
-	const float* knots = stage_tables(T, knots_g, lds_knots, lds_nb);
+	stage_tables(T, knots_g, lds_knots, lds_nb);
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
-	force[t] = nb_force<ANALYTIC>(uint32_t(t), pos[t], pos, L, T, knots, lds_nb, box, cutoff2, energy != 0);
+	force[t] = nb_force<ANALYTIC>(uint32_t(t), pos[t], pos, L, T, lds_knots, knots_g, lds_nb, box, cutoff2, energy != 0);
 }
 
 __global__ void k_zero_force(float4* __restrict__ f, uint64_t n) {
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 	__shared__ NbLds lds_nb;
 	__shared__ marsmd_particle_types lds_types;
 	stage_types(types_arg, lds_types);
-	const float* knots = stage_tables(T, knots_g, lds_knots, lds_nb);
+	stage_tables(T, knots_g, lds_knots, lds_nb);
 	const marsmd_particle_types& types = lds_types;
 	const TileRange R = xcd_tiles(uint32_t((n + kBlock - 1) / kBlock));
 	for (uint32_t tile = R.tile; tile < R.end; tile += R.stride) {
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 		if (t >= n)
 			continue;
 		const float4 p = pos[t];
-		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, knots, lds_nb, S.box, S.cutoff2, ENERGY);
+		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, lds_knots, knots_g, lds_nb, S.box, S.cutoff2, ENERGY);
 		if (has_bonded) {
 			const float4 b = force[t];
 			f.x += b.x;
