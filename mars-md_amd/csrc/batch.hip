@@ -797,7 +797,7 @@ bool BatchImpl::needs_rebuild(float skin, hipStream_t s) {
 }
 
 unsigned BatchImpl::step_grid(const void* kernel) {
-	const unsigned lds = tables->lds ? unsigned(tables->total_knots) * 4u : 0u;
+	const unsigned lds = tables->lds ? unsigned(std::max(tables->total_knots, 2)) * 4u : 0u;
 	if (occupancy == 0) {
 		MMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, kernel, kBatchBlock, lds));
 		occupancy = std::max(occupancy, 1);
@@ -892,7 +892,7 @@ void BatchImpl::launch_steps(const marsmd_run_params& P, uint64_t first, uint64_
 					 : (an ? k_b_step<MARSMD_INTEGRATOR_LANGEVIN, false, true>
 						   : k_b_step<MARSMD_INTEGRATOR_LANGEVIN, false, false>));
 	const unsigned grid = step_grid(reinterpret_cast<const void*>(kern));
-	const unsigned lds = tables->lds ? unsigned(tables->total_knots) * 4u : 0u;
+	const unsigned lds = tables->lds ? unsigned(std::max(tables->total_knots, 2)) * 4u : 0u;
 	StepParams S{};
 	S.cutoff2 = P.cutoff > 0.0f ? P.cutoff * P.cutoff : 0.0f;
 	S.dt = P.timestep;

@@ -48,8 +48,9 @@ inline DevBox make_devbox(const marsmd_box& b) {
 
 // PeriodicBox::wrap_diff_scalar (src/System/PeriodicBox.h:453-461).
 __device__ __forceinline__ float wrap_diff_scalar(float x, float l, float inv_l) {
-	const int image = int(floorf(x * inv_l));
-	x -= float(image) * l;
+	// float(int(floor(y))) == floor(y) for the |y| < 2^31 of any in-box difference
+	const float image = floorf(x * inv_l);
+	x -= image * l;
 	if (x >= 0.5f * l)
 		x -= l;
 	return x;

@@ -202,7 +202,7 @@ FMImpl::~FMImpl() {
 }
 
 void FMImpl::forces(const float* knots, float* out4, hipStream_t s) {
-	const unsigned lds = tables->lds ? unsigned(tables->total_knots) * 4u : 0u;
+	const unsigned lds = tables->lds ? unsigned(std::max(tables->total_knots, 2)) * 4u : 0u;
 	for (uint64_t k = 0; k < S; ++k) {
 		Snap& sn = *snaps[k];
 		const NbrList L{sn.counts.data(), sn.tile_base.data(), sn.nbr.data()};

@@ -58,7 +58,42 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 	const uint32_t ty = t_is_x ? tj : tt;
 	const uint32_t tag = S.tag[tx * uint32_t(T.num_types) + ty];
 	const uint32_t terms = tag & T.enabled;
-	if (ANALYTIC ? terms == 0u : !(terms & MARSMD_PAIR_TERM_TABULATED))
+	if constexpr (!ANALYTIC) {
+		// Tabulated term only, without branches: every quantity is computed for every
+		// pair (table reads clamped into the table) and a pair that the reference would
+		// skip adds +0, which leaves the sums' bits unchanged (they start at +0 and can
+		// never become -0). Straight-line code lets the four pairs of a group overlap
+		// their LDS reads.
+		const bool tab = (terms & MARSMD_PAIR_TERM_TABULATED) != 0u;
+		const float3 r = wrap_diff(box, sub3(py, px));
+		const float d = sqrtf(len2(r));
+		const float inv = 1.0f / d;
+		const bool far = d > 1e-6f;
+		const float ux = far ? inv * r.x : 0.0f, uy = far ? inv * r.y : 0.0f, uz = far ? inv * r.z : 0.0f;
+		const bool in = tab && !(d < 1e-6f) && !(cutoff2 > 0.0f && d * d > cutoff2);
+		// TabulatedPotential::compute (TabulatedPotential.h:19-52), non-periodic
+		const float4 tp = S.tab[tab ? (tag >> MARSMD_PAIR_TABLE_SHIFT) : 0u];
+		const int size = __float_as_int(tp.z);
+		const float* pot = knots + __float_as_int(tp.w);
+		float w = (d - tp.x) * tp.y;
+		const int home = int(floorf(w));
+		w = w - float(home);
+		const int hc = max(min(home, size - 2), 0);
+		const float U0 = pot[hc];
+		const float U1 = pot[hc + 1];
+		const float dU = U1 - U0;
+		const bool beyond = home >= size - 1;
+		const float fm = 0.0f + (beyond ? 0.0f : -dU * tp.y);
+		const float en = 0.0f + (beyond ? U1 : dU * w + U0);
+		const float gx = fm * ux, gy = fm * uy, gz = fm * uz;
+		fx += in ? (t_is_x ? -gx : gx) : 0.0f;
+		fy += in ? (t_is_x ? -gy : gy) : 0.0f;
+		fz += in ? (t_is_x ? -gz : gz) : 0.0f;
+		if (energy)
+			e += in ? en * 0.5f : 0.0f;
+		return;
+	}
+	if (terms == 0u)
 		return;
 	const float3 r = wrap_diff(box, sub3(py, px));
 	const float r2 = len2(r);
@@ -150,6 +185,8 @@ __device__ __forceinline__ void stage_params(const NbTables& T, NbLds& S) {
 		S.tag[i] = T.tag[i];
 	for (int i = threadIdx.x; i < T.num_tables; i += blockDim.x)
 		S.tab[i] = make_float4(T.start[i], T.step_inv[i], __int_as_float(T.size[i]), __int_as_float(T.offset[i]));
+	if (T.num_tables == 0 && threadIdx.x == 0) // pairs without the tabulated term read table 0
+		S.tab[0] = make_float4(0.0f, 0.0f, __int_as_float(2), __int_as_float(0));
 	for (int i = threadIdx.x; i < T.num_types; i += blockDim.x)
 		S.coef[i] = make_float4(T.coef.charge[i], T.coef.radius[i], T.coef.eps[i], 0.0f);
 }

@@ -671,7 +671,10 @@ bool PatchImpl::rebuild_policy(float pairlist_cutoff, float cutoff, uint64_t ste
 	return rebuild;
 }
 
-unsigned PatchImpl::lds_bytes() const { return tables->nb.lds ? unsigned(tables->nb.total_knots) * 4u : 0u; }
+// (at least two knots: the step kernels read a clamped pair of knots for every pair)
+unsigned PatchImpl::lds_bytes() const {
+	return tables->nb.lds ? unsigned(std::max(tables->nb.total_knots, 2)) * 4u : 0u;
+}
 
 // Workgroups for a persistent launch over the 256-slot tiles: as many as are resident
 // at once (occupancy x CUs), a multiple of 8 (one share per XCD), at most the tiles.
