@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_b_walk(const RepGeom* __restrict__ geom
 												uint32_t* __restrict__ counts, uint32_t* __restrict__ nbr) {
 	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
 	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
-	__shared__ uint64_t seg_k[kHomeWaves][kSegSort];
+	__shared__ float4 home_pos[kHomeWaves][64];
 	const int wv = threadIdx.x >> 6;
 	const TileRange TR = xcd_tiles((num_items + kHomeWaves - 1) / kHomeWaves);
 	for (uint32_t gidx = TR.tile; gidx < TR.end; gidx += TR.stride) {
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_b_walk(const RepGeom* __restrict__ geom
 			P.dom_lo[a] = G.lo[a];
 			P.dom_hi[a] = G.hi[a];
 		}
-		walk_home_cell<true>(P, item - item_off[r], seg_b[wv], seg_pre[wv], seg_k[wv], counts, nbr);
+		walk_home_cell<true>(P, item - item_off[r], seg_b[wv], seg_pre[wv], home_pos[wv], counts, nbr);
 	}
 }
 

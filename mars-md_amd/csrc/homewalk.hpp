@@ -39,13 +39,12 @@ __device__ __forceinline__ bool pair_predicate(const DevBox& box, float rc2, con
 // of the home cell's box (both widened by the quantisation margin, so the test is
 // conservative across the periodic seam), stores their particle ranges as segments
 // in LDS, then streams the candidates 64 at a time (one coalesced load per lane) and
-// tests each against the wave's home particles, the candidate broadcast with
-// readlane. Predicate and orientation are k_walk's (the reference's), so the pair set
-// is the same; a particle's list is in candidate order. Entries past `cap` are not
-// written (the host sees the count and walks again with a larger cap).
+// tests them against the wave's home particles, which are broadcast from LDS.
+// Predicate and orientation are k_walk's (the reference's), so the pair set is the
+// same; a particle's list is in candidate order. Entries past `cap` are not written
+// (the host sees the count and walks again with a larger cap).
 constexpr int kHomeWaves = 4;
 constexpr int kSegMax = 448;
-constexpr int kSegSort = 512; // power of two >= kSegMax
 
 struct HomeParams {
 	const float4* spos;
@@ -119,22 +118,33 @@ __device__ __forceinline__ void wave_lds_sync() {
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Tests the candidates of segments [0, nseg) (ntot particles) against this lane's home
-// particle t (when valid) and records the hits.
+// Tests the candidates of segments [0, nseg) (ntot particles) against the wave's home
+// particles (lane l holds home particle t = c0 + l when valid, with its count) and
+// records the hits. The candidates are spread over the lanes, 64 at a time, and the
+// home particles are visited one by one from LDS: each (home particle, candidate) pair
+// is evaluated by exactly one lane, so a wave whose home cell holds fewer than 64
+// particles does not idle lanes on every test. Hits of home particle i are ranked by
+// lane (candidate order) and appended to its row after its earlier hits, so every row
+// lists its neighbours in candidate order.
 template<bool FILL>
 __device__ __forceinline__ void home_test_segments(const HomeParams& P, const uint32_t* seg_b, uint32_t* seg_pre,
 												   uint32_t nseg, uint32_t ntot, bool valid, uint32_t t,
-												   const float4& p, uint32_t& cnt, uint32_t* __restrict__ nbr) {
-	const int lane = threadIdx.x & 63;
+												   const float4& p, uint32_t& cnt, uint32_t* __restrict__ nbr,
+												   float4* home_pos) {
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
 	if (lane == 0)
 		seg_pre[nseg] = ntot;
+	home_pos[lane] = p;
 	wave_lds_sync();
-	const uint64_t row = (uint64_t(t) >> 6) * 64u * P.cap;
+	const uint32_t c0 = t - lane;                       // first home slot of this chunk
+	const uint32_t nh = uint32_t(__popcll(__ballot(valid))); // valid lanes are a prefix
 	for (uint32_t q0 = 0; q0 < ntot; q0 += 64) {
-		const uint32_t qi = q0 + uint32_t(lane);
+		const uint32_t qi = q0 + lane;
+		const bool live = qi < ntot;
 		uint32_t j = 0xFFFFFFFFu;
 		float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		if (qi < ntot) {
+		if (live) {
 			uint32_t lo = 0, hi = nseg; // seg_pre[lo] <= qi < seg_pre[hi]
 			while (hi - lo > 1) {
 				const uint32_t mid = (lo + hi) >> 1;
@@ -146,82 +156,36 @@ __device__ __forceinline__ void home_test_segments(const HomeParams& P, const ui
 			j = seg_b[lo] + (qi - seg_pre[lo]);
 			q = P.spos[j];
 		}
-		const uint32_t m = min(64u, ntot - q0);
-		for (uint32_t u = 0; u < m; ++u) {
-			const uint32_t ju = bcast_u32(j, u);
-			const float4 qu = make_float4(bcast_f32(q.x, u), bcast_f32(q.y, u), bcast_f32(q.z, u), bcast_f32(q.w, u));
-			if (valid && ju != t) {
-				const bool t_lo = ju > t; // operands selected, one predicate evaluation
-				const float4 plo = t_lo ? p : qu;
-				const float4 phi = t_lo ? qu : p;
-				bool in = pair_predicate(P.box, P.rc2, P.ex, plo, phi);
+		for (uint32_t i = 0; i < nh; ++i) {
+			const uint32_t ti = c0 + i;
+			const float4 pi = home_pos[i];
+			bool in = false;
+			if (live && j != ti) {
+				const bool t_lo = j > ti; // operands selected, one predicate evaluation
+				const float4 plo = t_lo ? pi : q;
+				const float4 phi = t_lo ? q : pi;
+				in = pair_predicate(P.box, P.rc2, P.ex, plo, phi);
 				if (in && P.ref_filter)
 					in = in_reference_stencil(P, plo, phi);
-				if (in) {
-					if (FILL && cnt < P.cap)
-						nbr[nbr_slot_index(row, t, cnt)] = ju;
-					++cnt;
+			}
+			const uint64_t hits = __ballot(in);
+			if (hits) {
+				const uint32_t base = bcast_u32(cnt, i);
+				if (FILL && in) {
+					const uint32_t k = base + uint32_t(__popcll(hits & lt_mask));
+					if (k < P.cap)
+						nbr[nbr_slot_index((uint64_t(ti) >> 6) * 64u * P.cap, ti, k)] = j;
 				}
+				if (lane == i)
+					cnt += uint32_t(__popcll(hits));
 			}
 		}
 	}
-	wave_lds_sync();
-}
-
-// Orders segments [0, nseg) (ntot candidates) by first slot. The candidates are then
-// tested in slot order, so each lane's row lists its neighbours in slot (Morton) order
-// and, at one row entry, the lanes of a wave read positions that lie close together.
-// Bitonic sort of {first slot, length} keys in LDS, then the prefix is rebuilt.
-__device__ __forceinline__ void sort_segments(uint32_t* seg_b, uint32_t* seg_pre, uint64_t* key, uint32_t nseg,
-											  uint32_t ntot) {
-	const uint32_t lane = threadIdx.x & 63u;
-	if (nseg < 2)
-		return;
-	uint32_t np2 = 2;
-	while (np2 < nseg)
-		np2 <<= 1;
-	for (uint32_t i = lane; i < np2; i += 64) {
-		uint64_t k = ~uint64_t(0);
-		if (i < nseg) {
-			const uint32_t end = i + 1 < nseg ? seg_pre[i + 1] : ntot;
-			k = (uint64_t(seg_b[i]) << 32) | uint64_t(end - seg_pre[i]);
-		}
-		key[i] = k;
-	}
-	wave_lds_sync();
-	for (uint32_t size = 2; size <= np2; size <<= 1)
-		for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-			for (uint32_t i = lane; i < np2; i += 64) {
-				const uint32_t x = i ^ stride;
-				if (x > i) {
-					const uint64_t a = key[i], b = key[x];
-					if ((a > b) == ((i & size) == 0)) {
-						key[i] = b;
-						key[x] = a;
-					}
-				}
-			}
-			wave_lds_sync();
-		}
-	const uint32_t per = (nseg + 63u) >> 6;
-	const uint32_t i0 = lane * per;
-	uint32_t local = 0;
-	for (uint32_t u = 0; u < per; ++u)
-		if (i0 + u < nseg)
-			local += uint32_t(key[i0 + u]);
-	uint32_t run = wave_inclusive_scan(local) - local;
-	for (uint32_t u = 0; u < per; ++u)
-		if (i0 + u < nseg) {
-			const uint64_t k = key[i0 + u];
-			seg_b[i0 + u] = uint32_t(k >> 32);
-			seg_pre[i0 + u] = run;
-			run += uint32_t(k);
-		}
 	wave_lds_sync();
 }
 
 template<bool FILL>
-__device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg_b, uint32_t* seg_pre, uint64_t* seg_k,
+__device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg_b, uint32_t* seg_pre, float4* home_pos,
 							   uint32_t* __restrict__ counts, uint32_t* __restrict__ nbr) {
 	const int lane = threadIdx.x & 63;
 	const uint32_t hb = P.home_begin[home], he = P.home_end[home];
@@ -287,8 +251,7 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 			const uint64_t mask = __ballot(c > 0);
 			const uint32_t nnew = uint32_t(__popcll(mask));
 			if (nseg + nnew > uint32_t(kSegMax)) {
-				sort_segments(seg_b, seg_pre, seg_k, nseg, ntot);
-				home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr);
+				home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr, home_pos);
 				nseg = 0;
 				ntot = 0;
 			}
@@ -302,8 +265,7 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 			nseg += nnew;
 			ntot += sum;
 		}
-		sort_segments(seg_b, seg_pre, seg_k, nseg, ntot);
-		home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr);
+		home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr, home_pos);
 		if (valid) {
 			counts[t] = cnt;
 			if (FILL && cnt <= P.cap) {
@@ -320,13 +282,13 @@ __global__ __launch_bounds__(256) void k_walk_home(HomeParams P, uint32_t* __res
 												   uint32_t* __restrict__ nbr) {
 	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
 	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
-	__shared__ uint64_t seg_k[kHomeWaves][kSegSort];
+	__shared__ float4 home_pos[kHomeWaves][64];
 	const int wv = threadIdx.x >> 6;
 	const TileRange R = xcd_tiles((P.num_home + kHomeWaves - 1) / kHomeWaves);
 	for (uint32_t g = R.tile; g < R.end; g += R.stride) {
 		const uint32_t home = g * kHomeWaves + uint32_t(wv);
 		if (home < P.num_home)
-			walk_home_cell<FILL>(P, home, seg_b[wv], seg_pre[wv], seg_k[wv], counts, nbr);
+			walk_home_cell<FILL>(P, home, seg_b[wv], seg_pre[wv], home_pos[wv], counts, nbr);
 	}
 }
 
