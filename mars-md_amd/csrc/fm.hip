@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restric
 			const uint32_t cj = meta_canon(mj);
 			if (!(ct < cj))
 				continue; // the pair is taken from its lower canonical end
-			const uint32_t tag = lds_nb.tag[tt * uint32_t(T.num_types) + meta_type(mj)];
+			const uint32_t tag = lds_nb.tag[(tt << 4) | meta_type(mj)];
 			if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
 				continue;
 			const float3 r = wrap_diff(box, sub3(q, p));

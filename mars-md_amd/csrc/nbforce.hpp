@@ -29,7 +29,7 @@ struct NbTables {
 // workgroup: parameters indexed per lane (table index, particle type) would otherwise
 // be vector loads from the kernel-argument block on every pair.
 struct NbLds {
-	uint32_t tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+	uint32_t tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES]; // [x type << 4 | y type]
 	float4 tab[kMaxTables];        // {start, step_inv, size, offset} (size, offset as int bits)
 	float4 coef[MARSMD_MAX_TYPES]; // {charge, softcore radius, softcore eps, 0}
 };
@@ -56,21 +56,25 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 	const float4 py = t_is_x ? q : p;
 	const uint32_t tx = t_is_x ? tt : tj;
 	const uint32_t ty = t_is_x ? tj : tt;
-	const uint32_t tag = S.tag[tx * uint32_t(T.num_types) + ty];
+	const uint32_t tag = S.tag[(tx << 4) | ty];
 	const uint32_t terms = tag & T.enabled;
 	if constexpr (!ANALYTIC) {
 		// Tabulated term only, without branches: every quantity is computed for every
 		// pair (table reads clamped into the table) and a pair that the reference would
-		// skip adds +0, which leaves the sums' bits unchanged (they start at +0 and can
+		// skip adds +-0, which leaves the sums' bits unchanged (they start at +0 and can
 		// never become -0). Straight-line code lets the four pairs of a group overlap
 		// their LDS reads.
 		const bool tab = (terms & MARSMD_PAIR_TERM_TABULATED) != 0u;
-		const float3 r = wrap_diff(box, sub3(py, px));
-		const float d = sqrtf(len2(r));
-		const float inv = 1.0f / d;
-		const bool far = d > 1e-6f;
-		const float ux = far ? inv * r.x : 0.0f, uy = far ? inv * r.y : 0.0f, uz = far ? inv * r.z : 0.0f;
-		const bool in = tab && !(d < 1e-6f) && !(cutoff2 > 0.0f && d * d > cutoff2);
+		// py - px is q - p or its negation, exactly (rounding is sign-symmetric)
+		const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+		const float3 r = wrap_diff(box, t_is_x ? make_float3(dx, dy, dz) : make_float3(-dx, -dy, -dz));
+		const float d = sqrtf(len2(r)); // correctly rounded: d picks the table bin
+		// the unit vector only scales the force: the hardware reciprocal (1 ulp) in place
+		// of the correctly rounded division moves the force by ~1e-7 relative
+		// (floored at 1e-6, so the zero-distance pads give a finite product with +0)
+		const float inv = __builtin_amdgcn_rcpf(fmaxf(d, 1e-6f));
+		// the pairs the reference evaluates; at d == 1e-6 exactly its unit vector is 0
+		const bool keep = tab && !(d < 1e-6f) && !(cutoff2 > 0.0f && d * d > cutoff2);
 		// TabulatedPotential::compute (TabulatedPotential.h:19-52), non-periodic
 		const float4 tp = S.tab[tab ? (tag >> MARSMD_PAIR_TABLE_SHIFT) : 0u];
 		const int size = __float_as_int(tp.z);
@@ -85,12 +89,13 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		const bool beyond = home >= size - 1;
 		const float fm = 0.0f + (beyond ? 0.0f : -dU * tp.y);
 		const float en = 0.0f + (beyond ? U1 : dU * w + U0);
-		const float gx = fm * ux, gy = fm * uy, gz = fm * uz;
-		fx += in ? (t_is_x ? -gx : gx) : 0.0f;
-		fy += in ? (t_is_x ? -gy : gy) : 0.0f;
-		fz += in ? (t_is_x ? -gz : gz) : 0.0f;
+		// the lower canonical end takes -F: -(fm u) = (-fm) u exactly
+		const float sfm = (keep && d > 1e-6f) ? (t_is_x ? -fm : fm) : 0.0f;
+		fx += sfm * (inv * r.x);
+		fy += sfm * (inv * r.y);
+		fz += sfm * (inv * r.z);
 		if (energy)
-			e += in ? en * 0.5f : 0.0f;
+			e += keep ? en * 0.5f : 0.0f;
 		return;
 	}
 	if (terms == 0u)
@@ -150,29 +155,42 @@ __device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, con
 	if (ng == 0)
 		return make_float4(fx, fy, fz, e);
 	const uint4* __restrict__ row = reinterpret_cast<const uint4*>(L.nbr + L.tile_base[t >> 6]) + (t & 63u);
-	const uint4 j0 = row[0];
-	uint4 jn = ng > 1 ? row[64] : j0;
-	float4 q0 = pos[j0.x], q1 = pos[j0.y], q2 = pos[j0.z], q3 = pos[j0.w];
-	for (uint32_t g = 0; g < ng; ++g) {
-		float4 n0 = q0, n1 = q1, n2 = q2, n3 = q3;
-		uint4 jnn = jn;
+	// two register sets (a, b) take turns: while one group is evaluated the positions
+	// of the next are in flight in the other set, and the indices of the one after
+	const uint4 ja = row[0];
+	float4 a0 = pos[ja.x], a1 = pos[ja.y], a2 = pos[ja.z], a3 = pos[ja.w];
+	float4 b0, b1, b2, b3;
+	uint4 jb = ng > 1 ? row[64] : ja;
+	for (uint32_t g = 0;; g += 2) {
+		uint4 jc = jb;
 		if (g + 1 < ng) {
-			n0 = pos[jn.x];
-			n1 = pos[jn.y];
-			n2 = pos[jn.z];
-			n3 = pos[jn.w];
+			b0 = pos[jb.x];
+			b1 = pos[jb.y];
+			b2 = pos[jb.z];
+			b3 = pos[jb.w];
 			if (g + 2 < ng)
-				jnn = row[uint64_t(g + 2) * 64u];
+				jc = row[uint64_t(g + 2) * 64u];
 		}
-		nb_pair<ANALYTIC>(p, ct, tt, q0, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, q1, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, q2, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, q3, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		q0 = n0;
-		q1 = n1;
-		q2 = n2;
-		q3 = n3;
-		jn = jnn;
+		nb_pair<ANALYTIC>(p, ct, tt, a0, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, a1, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, a2, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, a3, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		if (g + 1 >= ng)
+			break;
+		if (g + 2 < ng) {
+			a0 = pos[jc.x];
+			a1 = pos[jc.y];
+			a2 = pos[jc.z];
+			a3 = pos[jc.w];
+			if (g + 3 < ng)
+				jb = row[uint64_t(g + 3) * 64u];
+		}
+		nb_pair<ANALYTIC>(p, ct, tt, b0, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, b1, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, b2, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, b3, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		if (g + 2 >= ng)
+			break;
 	}
 	return make_float4(fx, fy, fz, e);
 }
@@ -182,7 +200,7 @@ __device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, con
 __device__ __forceinline__ void stage_params(const NbTables& T, NbLds& S) {
 	const int nt2 = T.num_types * T.num_types;
 	for (int i = threadIdx.x; i < nt2; i += blockDim.x)
-		S.tag[i] = T.tag[i];
+		S.tag[((i / T.num_types) << 4) | (i % T.num_types)] = T.tag[i];
 	for (int i = threadIdx.x; i < T.num_tables; i += blockDim.x)
 		S.tab[i] = make_float4(T.start[i], T.step_inv[i], __int_as_float(T.size[i]), __int_as_float(T.offset[i]));
 	if (T.num_tables == 0 && threadIdx.x == 0) // pairs without the tabulated term read table 0
