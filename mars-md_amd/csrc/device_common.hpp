@@ -384,17 +384,31 @@ __device__ inline GridSample sample_linear(const marsmd_grid_view& g, float px, 
 	return s;
 }
 
-__device__ __forceinline__ float cr_value(float v0, float v1, float v2, float v3, float t) {
-	const float a3 = 0.5f * (-v0 + 3.0f * v1 - 3.0f * v2 + v3);
-	const float a2 = 0.5f * (2.0f * v0 - 5.0f * v1 + 4.0f * v2 - v3);
-	const float a1 = 0.5f * (-v0 + v2);
-	return ((a3 * t + a2) * t + a1) * t + v1;
+// Catmull-Rom interpolation along one axis as weights of the four taps: the cubic
+// ((a3 t + a2) t + a1) t + v1 with a3 = (-v0 + 3 v1 - 3 v2 + v3) / 2,
+// a2 = (2 v0 - 5 v1 + 4 v2 - v3) / 2, a1 = (v2 - v0) / 2 (BaseGridDevice.h:405-475) is
+// sum_i w_i(t) v_i with the weights below, and its t-derivative sum_i w_i'(t) v_i. The
+// weights are formed once per axis and each pass is then four multiply-adds per output,
+// where the reference's form re-derives the coefficients for every tap line; the sums
+// agree with it to rounding.
+struct CrWeights {
+	float w[4], d[4];
+};
+__device__ __forceinline__ CrWeights cr_weights(float t) {
+	const float t2 = t * t, t3 = t2 * t;
+	CrWeights c;
+	c.w[0] = 0.5f * (2.0f * t2 - t3 - t);
+	c.w[1] = 0.5f * (3.0f * t3 - 5.0f * t2 + 2.0f);
+	c.w[2] = 0.5f * (4.0f * t2 - 3.0f * t3 + t);
+	c.w[3] = 0.5f * (t3 - t2);
+	c.d[0] = 0.5f * (4.0f * t - 3.0f * t2 - 1.0f);
+	c.d[1] = 0.5f * (9.0f * t2 - 10.0f * t);
+	c.d[2] = 0.5f * (8.0f * t - 9.0f * t2 + 1.0f);
+	c.d[3] = 0.5f * (3.0f * t2 - 2.0f * t);
+	return c;
 }
-__device__ __forceinline__ float cr_deriv(float v0, float v1, float v2, float v3, float t) {
-	const float a3 = 0.5f * (-v0 + 3.0f * v1 - 3.0f * v2 + v3);
-	const float a2 = 0.5f * (2.0f * v0 - 5.0f * v1 + 4.0f * v2 - v3);
-	const float a1 = 0.5f * (-v0 + v2);
-	return (3.0f * a3 * t + 2.0f * a2) * t + a1;
+__device__ __forceinline__ float dot4(const float w[4], float v0, float v1, float v2, float v3) {
+	return fmaf(w[3], v3, fmaf(w[2], v2, fmaf(w[1], v1, w[0] * v0)));
 }
 
 // sample_grid_cubic (BaseGridDevice.h:405-475): the x pass runs over the 4 x 4 (y, z)
@@ -407,6 +421,7 @@ __device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, f
 	mat_transform(g.basis_inv, v, gp);
 	const int hx = int(floorf(gp[0])), hy = int(floorf(gp[1])), hz = int(floorf(gp[2]));
 	const float wx = gp[0] - float(hx), wy = gp[1] - float(hy), wz = gp[2] - float(hz);
+	const CrWeights Wx = cr_weights(wx), Wy = cr_weights(wy), Wz = cr_weights(wz);
 	const int nx = int(g.dims[0]), ny = int(g.dims[1]), nz = int(g.dims[2]);
 	const int64_t sx = int64_t(ny) * int64_t(nz);
 	int64_t ox[4];
@@ -430,22 +445,21 @@ __device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, f
 		}
 #pragma unroll
 		for (int iz = 0; iz < 4; ++iz) {
-			d1[iy][iz] = cr_deriv(r[0][iz], r[1][iz], r[2][iz], r[3][iz], wx);
-			v1[iy][iz] = cr_value(r[0][iz], r[1][iz], r[2][iz], r[3][iz], wx);
+			d1[iy][iz] = dot4(Wx.d, r[0][iz], r[1][iz], r[2][iz], r[3][iz]);
+			v1[iy][iz] = dot4(Wx.w, r[0][iz], r[1][iz], r[2][iz], r[3][iz]);
 		}
 	}
 	float d2x[4], d2y[4], v2[4];
 #pragma unroll
 	for (int iz = 0; iz < 4; ++iz) {
-		d2x[iz] = cr_value(d1[0][iz], d1[1][iz], d1[2][iz], d1[3][iz], wy);
-		d2y[iz] = cr_deriv(v1[0][iz], v1[1][iz], v1[2][iz], v1[3][iz], wy);
-		v2[iz] = cr_value(v1[0][iz], v1[1][iz], v1[2][iz], v1[3][iz], wy);
+		d2x[iz] = dot4(Wy.w, d1[0][iz], d1[1][iz], d1[2][iz], d1[3][iz]);
+		d2y[iz] = dot4(Wy.d, v1[0][iz], v1[1][iz], v1[2][iz], v1[3][iz]);
+		v2[iz] = dot4(Wy.w, v1[0][iz], v1[1][iz], v1[2][iz], v1[3][iz]);
 	}
-	const float gg[3] = {cr_value(d2x[0], d2x[1], d2x[2], d2x[3], wz),
-						 cr_value(d2y[0], d2y[1], d2y[2], d2y[3], wz),
-						 cr_deriv(v2[0], v2[1], v2[2], v2[3], wz)};
+	const float gg[3] = {dot4(Wz.w, d2x[0], d2x[1], d2x[2], d2x[3]), dot4(Wz.w, d2y[0], d2y[1], d2y[2], d2y[3]),
+						 dot4(Wz.d, v2[0], v2[1], v2[2], v2[3])};
 	GridSample s;
-	s.value = cr_value(v2[0], v2[1], v2[2], v2[3], wz);
+	s.value = dot4(Wz.w, v2[0], v2[1], v2[2], v2[3]);
 	mat_transpose_transform(g.basis_inv, gg, s.grad);
 	return s;
 }

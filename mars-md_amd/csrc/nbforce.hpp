@@ -32,6 +32,9 @@ struct NbLds {
 	uint32_t tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES]; // [x type << 4 | y type]
 	float4 tab[kMaxTables];        // {start, step_inv, size, offset} (size, offset as int bits)
 	float4 coef[MARSMD_MAX_TYPES]; // {charge, softcore radius, softcore eps, 0}
+	// per type pair: its enabled table's tab record, size 0 when the pair has no
+	// enabled tabulated term (one read per pair on the tabulated-only path)
+	float4 pairtab[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
 };
 
 __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
@@ -56,15 +59,15 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 	const float4 py = t_is_x ? q : p;
 	const uint32_t tx = t_is_x ? tt : tj;
 	const uint32_t ty = t_is_x ? tj : tt;
-	const uint32_t tag = S.tag[(tx << 4) | ty];
-	const uint32_t terms = tag & T.enabled;
 	if constexpr (!ANALYTIC) {
 		// Tabulated term only, without branches: every quantity is computed for every
 		// pair (table reads clamped into the table) and a pair that the reference would
 		// skip adds +-0, which leaves the sums' bits unchanged (they start at +0 and can
 		// never become -0). Straight-line code lets the four pairs of a group overlap
 		// their LDS reads.
-		const bool tab = (terms & MARSMD_PAIR_TERM_TABULATED) != 0u;
+		const float4 tp = S.pairtab[(tx << 4) | ty];
+		const int size = __float_as_int(tp.z);
+		const bool tab = size != 0;
 		// py - px is q - p or its negation, exactly (rounding is sign-symmetric)
 		const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
 		const float3 r = wrap_diff(box, t_is_x ? make_float3(dx, dy, dz) : make_float3(-dx, -dy, -dz));
@@ -75,9 +78,8 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		const float inv = __builtin_amdgcn_rcpf(fmaxf(d, 1e-6f));
 		// the pairs the reference evaluates; at d == 1e-6 exactly its unit vector is 0
 		const bool keep = tab && !(d < 1e-6f) && !(cutoff2 > 0.0f && d * d > cutoff2);
-		// TabulatedPotential::compute (TabulatedPotential.h:19-52), non-periodic
-		const float4 tp = S.tab[tab ? (tag >> MARSMD_PAIR_TABLE_SHIFT) : 0u];
-		const int size = __float_as_int(tp.z);
+		// TabulatedPotential::compute (TabulatedPotential.h:19-52), non-periodic; a pair
+		// without the term reads knots 0 and 1 (size 0 clamps the bin to 0) and adds 0
 		const float* pot = knots + __float_as_int(tp.w);
 		float w = (d - tp.x) * tp.y;
 		const int home = int(floorf(w));
@@ -98,6 +100,8 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 			e += keep ? en * 0.5f : 0.0f;
 		return;
 	}
+	const uint32_t tag = S.tag[(tx << 4) | ty];
+	const uint32_t terms = tag & T.enabled;
 	if (terms == 0u)
 		return;
 	const float3 r = wrap_diff(box, sub3(py, px));
@@ -199,8 +203,16 @@ __device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, con
 // caller synchronises).
 __device__ __forceinline__ void stage_params(const NbTables& T, NbLds& S) {
 	const int nt2 = T.num_types * T.num_types;
-	for (int i = threadIdx.x; i < nt2; i += blockDim.x)
-		S.tag[((i / T.num_types) << 4) | (i % T.num_types)] = T.tag[i];
+	for (int i = threadIdx.x; i < nt2; i += blockDim.x) {
+		const uint32_t tag = T.tag[i];
+		const int k = ((i / T.num_types) << 4) | (i % T.num_types);
+		S.tag[k] = tag;
+		const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
+		S.pairtab[k] = (tag & T.enabled & MARSMD_PAIR_TERM_TABULATED) && ti < T.num_tables
+						   ? make_float4(T.start[ti], T.step_inv[ti], __int_as_float(T.size[ti]),
+										 __int_as_float(T.offset[ti]))
+						   : make_float4(0.0f, 0.0f, __int_as_float(0), __int_as_float(0));
+	}
 	for (int i = threadIdx.x; i < T.num_tables; i += blockDim.x)
 		S.tab[i] = make_float4(T.start[i], T.step_inv[i], __int_as_float(T.size[i]), __int_as_float(T.offset[i]));
 	if (T.num_tables == 0 && threadIdx.x == 0) // pairs without the tabulated term read table 0
