@@ -32,6 +32,22 @@ __device__ __forceinline__ bool pair_predicate(const DevBox& box, float rc2, con
 	}
 	return true;
 }
+
+// pair_predicate for a home particle pi (slot ti) and a candidate q (slot j != ti),
+// oriented by slot: the difference phi - plo is q - pi or its exact negation (rounding
+// is sign-symmetric), so the orientation costs three selects; the metadata of the two
+// ends is only looked at when exclusions or rigid bodies are given.
+__device__ __forceinline__ bool pair_predicate_home(const DevBox& box, float rc2, const marsmd_exclusion_view& ex,
+													const float4& pi, uint32_t ti, const float4& q, uint32_t j) {
+	const bool t_lo = j > ti;
+	const float dx = q.x - pi.x, dy = q.y - pi.y, dz = q.z - pi.z;
+	const float3 dr = wrap_diff(box, t_lo ? make_float3(dx, dy, dz) : make_float3(-dx, -dy, -dz));
+	if (!(len2(dr) <= rc2))
+		return false;
+	if (!ex.rigid_body_id && !ex.offsets)
+		return true;
+	return pair_predicate(box, rc2, ex, t_lo ? pi : q, t_lo ? q : pi);
+}
 // ------------------------------------------------------------ home-cell walk
 // One wave per home cell: a Morton cell of the home level h, whose particles are
 // contiguous in sorted order. The wave enumerates the fine cells (level m = h + s)
@@ -156,17 +172,15 @@ __device__ __forceinline__ void home_test_segments(const HomeParams& P, const ui
 			j = seg_b[lo] + (qi - seg_pre[lo]);
 			q = P.spos[j];
 		}
+#pragma unroll 2
 		for (uint32_t i = 0; i < nh; ++i) {
 			const uint32_t ti = c0 + i;
 			const float4 pi = home_pos[i];
 			bool in = false;
 			if (live && j != ti) {
-				const bool t_lo = j > ti; // operands selected, one predicate evaluation
-				const float4 plo = t_lo ? pi : q;
-				const float4 phi = t_lo ? q : pi;
-				in = pair_predicate(P.box, P.rc2, P.ex, plo, phi);
+				in = pair_predicate_home(P.box, P.rc2, P.ex, pi, ti, q, j);
 				if (in && P.ref_filter)
-					in = in_reference_stencil(P, plo, phi);
+					in = in_reference_stencil(P, pi, q); // symmetric in its two ends
 			}
 			const uint64_t hits = __ballot(in);
 			if (hits) {

@@ -54,6 +54,21 @@ __global__ void k_encode(const float4* __restrict__ pos, uint64_t n, float3 bmin
 		atomicOr(outside, 1u);
 }
 
+// The stable sort of nondecreasing keys: codes = keys, sorted and inverse maps the
+// identity; a key below its predecessor sets bit 1 of `flag`.
+__global__ void k_presorted(const uint32_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ codes,
+							uint32_t* __restrict__ sidx, uint32_t* __restrict__ inv, uint32_t* __restrict__ flag) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const uint32_t k = keys[t];
+	codes[t] = k;
+	sidx[t] = uint32_t(t);
+	inv[t] = uint32_t(t);
+	if (t > 0 && keys[t - 1] > k)
+		atomicOr(flag, 2u);
+}
+
 __global__ void k_inverse(const uint32_t* __restrict__ sidx, uint64_t n, uint32_t* __restrict__ inv) {
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
@@ -515,7 +530,7 @@ void ZSort::ensure(uint64_t nn) {
 }
 
 void ZSort::sort(const float4* pos, uint64_t nn, const float bmin[3], const float bmax[3],
-				 const float* domain_dev, bool canon_from_meta, hipStream_t s) {
+				 const float* domain_dev, bool canon_from_meta, hipStream_t s, bool presorted) {
 	(void)domain_dev;
 	ensure(nn);
 	n = nn;
@@ -526,6 +541,12 @@ void ZSort::sort(const float4* pos, uint64_t nn, const float bmin[3], const floa
 											   make_float3(bmax[0], bmax[1], bmax[2]), canon_from_meta,
 											   keys_unsorted.data(), vals_unsorted.data(), outside_flag.data());
 	MMD_LAUNCH_CHECK();
+	if (presorted) {
+		k_presorted<<<grid_for(nn, 256), 256, 0, s>>>(keys_unsorted.data(), nn, codes.data(), sorted_idx.data(),
+													  inverse.data(), outside_flag.data());
+		MMD_LAUNCH_CHECK();
+		return;
+	}
 	radix_sort_pairs(keys_unsorted.data(), vals_unsorted.data(), codes.data(), sorted_idx.data(), nn, 30,
 					 scratch.data(), scratch.count, s);
 	k_inverse<<<grid_for(nn, 256), 256, 0, s>>>(sorted_idx.data(), nn, inverse.data());
@@ -760,6 +781,8 @@ bool PairlistImpl::walk_home(hipStream_t s, bool& outside) {
 		MMD_HIP(hipMemcpyAsync(&host[0], sorter.outside_flag.data(), 4, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipMemcpyAsync(&host[1], dev_scalars.data() + 2, 16, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipStreamSynchronize(s));
+		if (host[0] & 2u)
+			raise(MARSMD_ERR_RUNTIME, "pairlist: keys after a reorder were not in order");
 		if (host[0] != 0) {
 			outside = true;
 			return true;
@@ -827,7 +850,11 @@ void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool 
 		domain_min[a] = lo[a];
 		domain_max[a] = hi[a];
 	}
-	sorter.sort(pos, nn, lo, hi, nullptr, canon_from_meta, s);
+	// after a reorder over the periodic box the canonical labels are the ranks of these
+	// very keys, so the sort is the identity
+	const bool presorted = presorted_next && all_periodic && canon_from_meta;
+	presorted_next = false;
+	sorter.sort(pos, nn, lo, hi, nullptr, canon_from_meta, s, presorted);
 	k_finalize<<<grid_for(nn, 256), 256, 0, s>>>(sorter.sorted_idx.data(), nn, pos, slot_of_canon, canon_from_meta,
 												 types, sorter.inverse.data(), spos.data());
 	MMD_LAUNCH_CHECK();
@@ -868,6 +895,8 @@ void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool 
 		MMD_HIP(hipMemcpyAsync(&host[1], sorter.outside_flag.data(), 4, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipMemcpyAsync(&host[2], dev_scalars.data() + 2, 16, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipStreamSynchronize(s));
+		if (host[1] & 2u)
+			raise(MARSMD_ERR_RUNTIME, "pairlist: keys after a reorder were not in order");
 		if (host[1] != 0 && !use_ref) {
 			// A coordinate outside the Morton domain was clamped: cell geometry no
 			// longer bounds positions, so walk the reference grid exactly.

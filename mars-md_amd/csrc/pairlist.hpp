@@ -20,15 +20,19 @@ struct ZSort {
 	DevArray<uint32_t> keys_unsorted, vals_unsorted; // canonical-order keys and iota
 	DevArray<uint32_t> codes, sorted_idx, inverse;    // results
 	DevArray<uint8_t> scratch;
-	DevArray<uint32_t> outside_flag; // set when a coordinate was clamped by the encode
+	DevArray<uint32_t> outside_flag; // bit 0: a coordinate was clamped by the encode
 	explicit ZSort(uint64_t max_n);
 	void ensure(uint64_t n);
 	// Encode positions (slot order) into canonical-order keys and sort them.
 	// canon_from_meta: the canonical index of slot s is meta_canon(pos[s].w);
 	// otherwise it is s. domain_dev (6 floats: min xyz, max xyz) overrides bmin/bmax
 	// when non-null.
+	// presorted: the canonical-order keys are known to be nondecreasing (the particles
+	// were just relabelled by a sort of the same keys), so the stable sort is the
+	// identity and is replaced by a copy that checks the order (bit 1 of outside_flag
+	// when violated).
 	void sort(const float4* pos, uint64_t n, const float bmin[3], const float bmax[3],
-			  const float* domain_dev, bool canon_from_meta, hipStream_t s);
+			  const float* domain_dev, bool canon_from_meta, hipStream_t s, bool presorted = false);
 	// ReorderDataKernel<T> (ZOrderKernels.h:48-63).
 	void reorder_data(const void* in, void* out, uint64_t n, uint32_t elem, hipStream_t s) const;
 	// RemapIndicesKernel (ZOrderKernels.h:87-101).
@@ -54,6 +58,7 @@ __host__ __device__ inline uint64_t nbr_slot_index(uint64_t tile_base, uint32_t 
 struct PairlistImpl {
 	uint64_t max_particles = 0;
 	uint64_t ref_max_pairs = 0;     // Pairlist::max_pairs_ bookkeeping (grows x1.25)
+	bool presorted_next = false;    // set by a reorder: the next build's sort is the identity
 	uint64_t max_pairs_ceiling = ~uint64_t(0);
 	marsmd_box box{};
 	marsmd_exclusion_view excl{};

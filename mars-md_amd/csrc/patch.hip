@@ -586,6 +586,7 @@ void PatchImpl::copy_from_host(const float* pos4, const float* mom4, const int32
 	built = false;
 	force_rebuild = false;
 	deferred_kick = false;
+	pl->presorted_next = false;
 	rebuild_exclusions(nullptr);
 	MMD_HIP(hipDeviceSynchronize());
 }
@@ -742,6 +743,8 @@ void PatchImpl::calculate_bonded(bool energy, hipStream_t s) {
 }
 
 void PatchImpl::integrate(float dt, float kT, int integrator, uint64_t step, hipStream_t s) {
+	// positions move: a build after this one is no longer presorted by a reorder
+	pl->presorted_next = false;
 	if (n == 0)
 		return;
 	StepParams S{};
@@ -821,6 +824,7 @@ void PatchImpl::reorder(hipStream_t s) {
 		MMD_LAUNCH_CHECK();
 		rebuild_exclusions(s);
 	}
+	pl->presorted_next = true;
 	force_rebuild = true;
 }
 
