@@ -49,6 +49,9 @@ struct RepGeom {
 	uint64_t pid;
 	// the reference grid's stencil, when it misses seam pairs (see HomeParams)
 	int ref_filter, ref_shift, ref_cells, ref_r[3], ref_full[3];
+	// the home-frame prefilter of the walk (see HomeParams)
+	int pre;
+	float pre_lo2, pre_hi2;
 };
 
 // keys / vals in canonical position: global canonical g = off_r + canon.
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(256) void k_b_walk(const RepGeom* __restrict__ geom
 												uint32_t* __restrict__ counts, uint32_t* __restrict__ nbr) {
 	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
 	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
-	__shared__ float4 home_pos[kHomeWaves][64];
+	__shared__ float4 home_pos[kHomeWaves][kHomeLds];
 	const int wv = threadIdx.x >> 6;
 	const TileRange TR = xcd_tiles((num_items + kHomeWaves - 1) / kHomeWaves);
 	for (uint32_t gidx = TR.tile; gidx < TR.end; gidx += TR.stride) {
@@ -192,6 +195,9 @@ __global__ __launch_bounds__(256) void k_b_walk(const RepGeom* __restrict__ geom
 			P.dom_lo[a] = G.lo[a];
 			P.dom_hi[a] = G.hi[a];
 		}
+		P.pre = G.pre;
+		P.pre_lo2 = G.pre_lo2;
+		P.pre_hi2 = G.pre_hi2;
 		walk_home_cell<true>(P, item - item_off[r], seg_b[wv], seg_pre[wv], home_pos[wv], counts, nbr);
 	}
 }
@@ -662,6 +668,7 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 				complete = false;
 		}
 		g.ref_filter = complete ? 0 : 1;
+		g.pre = home_prefilter_band(boxes[r], g.lo, g.hi, h, rc, &g.pre_lo2, &g.pre_hi2) ? 1 : 0;
 		g.ref_shift = 10 - ref.m;
 		g.ref_cells = rcells;
 		g.fine_off = uint32_t(fine_total);

@@ -5,6 +5,10 @@
 
 #include "pairlist.hpp"
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 namespace mmd {
 
 namespace {
@@ -61,6 +65,8 @@ __device__ __forceinline__ bool pair_predicate_home(const DevBox& box, float rc2
 // (the host sees the count and walks again with a larger cap).
 constexpr int kHomeWaves = 4;
 constexpr int kSegMax = 448;
+// LDS per wave for the home particles: 64 positions, then 64 candidate slots
+constexpr int kHomeLds = 64 + 16;
 
 struct HomeParams {
 	const float4* spos;
@@ -88,7 +94,94 @@ struct HomeParams {
 	int ref_r[3];
 	int ref_full[3];
 	float dom_lo[3], dom_hi[3];
+	// Home-frame prefilter (pre != 0): each candidate is moved by whole box lengths next
+	// to the wave's first home particle, and its plain squared distance d2' to a home
+	// particle decides the pair when it is below pre_lo2 (in) or above pre_hi2 (out);
+	// between the two, which are rc^2 -+ a bound on |d2' - d2| over pairs near the
+	// cutoff, the reference's predicate decides. Set only where the box is wide enough
+	// for that move to give the minimum image of every pair within the cutoff.
+	int pre;
+	float pre_lo2, pre_hi2;
+	// The fine cells the walk enumerates, when given: per entry the cell offset from the
+	// home cell's first fine cell on each axis (the absolute cell on an axis enumerated
+	// whole), in enumeration order, for the cells that pass the gap test; null: every
+	// cell of the reach is enumerated and gap-tested per home cell.
+	const int4* stencil;
+	uint32_t stencil_n;
 };
+
+// The fine cells of the walk's enumeration that pass its gap test, as offsets (see
+// HomeParams::stencil), in its order (x fastest); empty when there are more than
+// max_cells of them. The test is the walk's own: per axis not enumerated whole, the
+// number of cells between the cell and the home cell's span times the cell width, less
+// twice the margin, floored at 0, squared and summed, against rc^2.
+inline std::vector<int4> home_stencil(int h, int s, const int reach[3], const int full[3], const float cw[3],
+									  const float margin[3], float rc2, size_t max_cells) {
+	const int fine_cells = 1 << (h + s), span = 1 << s;
+	int lo[3], nax[3];
+	for (int a = 0; a < 3; ++a) {
+		lo[a] = full[a] ? 0 : -reach[a];
+		nax[a] = full[a] ? fine_cells : span + 2 * reach[a];
+	}
+	std::vector<int4> out;
+	for (int z = 0; z < nax[2]; ++z)
+		for (int y = 0; y < nax[1]; ++y)
+			for (int x = 0; x < nax[0]; ++x) {
+				const int o[3] = {lo[0] + x, lo[1] + y, lo[2] + z};
+				float gap2 = 0.0f;
+				for (int a = 0; a < 3; ++a) {
+					if (full[a])
+						continue;
+					const int apart = o[a] < 0 ? -o[a] - 1 : (o[a] >= span ? o[a] - span : 0);
+					const float g = std::fmax(0.0f, float(apart) * cw[a] - 2.0f * margin[a]);
+					gap2 += g * g;
+				}
+				if (gap2 <= rc2) {
+					if (out.size() == max_cells)
+						return {};
+					out.push_back(make_int4(o[0], o[1], o[2], 0));
+				}
+			}
+	// Morton order of the offsets: 64 consecutive candidates then come from a compact
+	// cluster of cells, which a home particle's cutoff sphere often misses as a whole
+	auto key = [](const int4& c) {
+		uint64_t k = 0;
+		const uint32_t v[3] = {uint32_t(c.x + 1024), uint32_t(c.y + 1024), uint32_t(c.z + 1024)};
+		for (int b = 10; b >= 0; --b)
+			for (int a = 0; a < 3; ++a)
+				k = (k << 1) | ((v[a] >> b) & 1u);
+		return k;
+	};
+	std::stable_sort(out.begin(), out.end(), [&](const int4& a, const int4& b) { return key(a) < key(b); });
+	return out;
+}
+
+// The prefilter's band for a domain [lo, hi) whose home cells (level h) have width
+// extent / 2^h: false when a periodic axis is too short for the home frame to give
+// minimum images (half the box must exceed a home cell plus the cutoff, with slack).
+// The bound on |d2' - d2|: the two differences of an axis differ by a few roundings at
+// the coordinates' magnitude C (|x| <= C), i.e. a few ulp(C), over |d| <= rc + 1, plus
+// the roundings of the squares and sums at rc^2.
+inline bool home_prefilter_band(const marsmd_box& box, const float lo[3], const float hi[3], int h, float rc,
+								float* lo2, float* hi2) {
+	float C = 0.0f;
+	for (int a = 0; a < 3; ++a) {
+		const float w = (hi[a] - lo[a]) / float(1 << h);
+		C = std::fmax(C, std::fmax(std::fabs(lo[a]), std::fabs(hi[a])));
+		if (box.periodic[a] && !(0.5f * box.size[a] > w + rc + 0.01f * box.size[a] + 1.0f))
+			return false;
+		C = std::fmax(C, box.size[a]);
+	}
+	const float rc2 = rc * rc;
+	const float ulp_c = std::nextafter(C, INFINITY) - C;
+	const float ulp_r = std::nextafter(rc2 + 1.0f, INFINITY) - (rc2 + 1.0f);
+	const float eps = 64.0f * ulp_c * (rc + 1.0f) + 16.0f * ulp_r;
+	if (!(eps < 0.01f * rc2))
+		return false;
+	*lo2 = rc2 - eps;
+	*hi2 = rc2 + eps;
+	return true;
+}
 
 // Whether the reference grid's stencil (BuildCellNeighborsKernel, ZOrderNeighbor.h:
 // 55-139) links the cells of a and b. The relation is symmetric, so it is the pair
@@ -134,27 +227,105 @@ __device__ __forceinline__ void wave_lds_sync() {
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The hits of the wave's nh home particles (visited one by one from LDS, the next one
+// read ahead) among one block of up to 64 candidates (lane: slot j, position q) by the
+// reference's predicate: the mask over the block's lanes of home particle i, in lane i.
+__device__ __forceinline__ uint64_t exact_hit_masks(const HomeParams& P, const float4* home_pos, uint32_t nh,
+													 uint32_t c0s, bool live, uint32_t j, const float4& q) {
+	const uint32_t lane = threadIdx.x & 63u;
+	uint64_t mine = 0;
+	float4 pn = home_pos[0];
+	for (uint32_t i = 0; i < nh; ++i) {
+		const float4 pi = pn;
+		pn = home_pos[(i + 1u) & 63u];
+		const uint32_t ti = c0s + i;
+		bool in = live && j != ti && pair_predicate_home(P.box, P.rc2, P.ex, pi, ti, q, j);
+		if (in && P.ref_filter)
+			in = in_reference_stencil(P, pi, q); // symmetric in its two ends
+		const uint64_t hits = __ballot(in);
+		mine = lane == i ? hits : mine;
+	}
+	return mine;
+}
+
+// The same masks from the home-frame prefilter (the candidate moved into the home frame
+// as qx, qy, qz; +inf on lanes without a candidate): a pair with d2' <= pre_lo2 is in,
+// one with d2' > pre_hi2 out. A home particle's own slot among the candidates is not
+// excluded here (the caller skips it). When any pair of the block falls between the two
+// bounds the block is decided again by exact_hit_masks.
+__device__ __forceinline__ uint64_t home_frame_hit_masks(const HomeParams& P, const float4* home_pos, uint32_t nh,
+														 uint32_t c0s, bool live, uint32_t j, const float4& q,
+														 float qx, float qy, float qz) {
+	const uint32_t lane = threadIdx.x & 63u;
+	uint64_t mine = 0, band = 0;
+	float4 pn = home_pos[0];
+	for (uint32_t i = 0; i < nh; ++i) {
+		const float4 pi = pn;
+		pn = home_pos[(i + 1u) & 63u];
+		const float dx = qx - pi.x, dy = qy - pi.y, dz = qz - pi.z;
+		const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+		const uint64_t hits = __ballot(d2 <= P.pre_lo2);
+		band |= __ballot(d2 <= P.pre_hi2) & ~hits;
+		mine = lane == i ? hits : mine;
+	}
+	if (band)
+		mine = exact_hit_masks(P, home_pos, nh, c0s, live, j, q);
+	return mine;
+}
+
+// One block of candidates against the wave's home particles: the hit masks (lane i:
+// home particle i's hits over the block's lanes), then every home lane appends its hits
+// to its row in lane (candidate) order, reading their slots from `cand` (LDS).
+template<bool FAST, bool FILL>
+__device__ __forceinline__ void test_candidate_block(const HomeParams& P, const float4* home_pos, uint32_t* cand,
+													 uint32_t nh, uint32_t c0, bool live, uint32_t j, const float4& q,
+													 float qx, float qy, float qz, uint32_t& cnt,
+													 uint32_t* __restrict__ nbr) {
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t c0s = uint32_t(__builtin_amdgcn_readfirstlane(int(c0))); // wave-uniform
+	cand[lane] = j;
+	wave_lds_sync();
+	const uint64_t mine = FAST ? home_frame_hit_masks(P, home_pos, nh, c0s, live, j, q, qx, qy, qz)
+							   : exact_hit_masks(P, home_pos, nh, c0s, live, j, q);
+	if (lane < nh) {
+		const uint32_t t = c0s + lane;
+		uint32_t* __restrict__ row = nbr + (uint64_t(t) >> 6) * 64u * P.cap + (t & 63u) * 4u;
+		uint64_t m = mine;
+		while (m) {
+			const uint32_t b = uint32_t(__builtin_ctzll(m));
+			m &= m - 1;
+			const uint32_t jb = cand[b];
+			if (FAST && jb == t) // never a pair with itself
+				continue;
+			if (FILL && cnt < P.cap) // entry cnt of slot t (nbr_slot_index)
+				row[(cnt >> 2) * 256u + (cnt & 3u)] = jb;
+			++cnt;
+		}
+	}
+	wave_lds_sync();
+}
+
 // Tests the candidates of segments [0, nseg) (ntot particles) against the wave's home
 // particles (lane l holds home particle t = c0 + l when valid, with its count) and
 // records the hits. The candidates are spread over the lanes, 64 at a time, and the
 // home particles are visited one by one from LDS: each (home particle, candidate) pair
 // is evaluated by exactly one lane, so a wave whose home cell holds fewer than 64
-// particles does not idle lanes on every test. Hits of home particle i are ranked by
-// lane (candidate order) and appended to its row after its earlier hits, so every row
-// lists its neighbours in candidate order.
+// particles does not idle lanes on every test. Every row lists its neighbours in
+// candidate order.
 template<bool FILL>
 __device__ __forceinline__ void home_test_segments(const HomeParams& P, const uint32_t* seg_b, uint32_t* seg_pre,
 												   uint32_t nseg, uint32_t ntot, bool valid, uint32_t t,
 												   const float4& p, uint32_t& cnt, uint32_t* __restrict__ nbr,
 												   float4* home_pos) {
 	const uint32_t lane = threadIdx.x & 63u;
-	const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
 	if (lane == 0)
 		seg_pre[nseg] = ntot;
 	home_pos[lane] = p;
 	wave_lds_sync();
+	uint32_t* cand = reinterpret_cast<uint32_t*>(home_pos + 64); // the block's candidate slots
 	const uint32_t c0 = t - lane;                       // first home slot of this chunk
 	const uint32_t nh = uint32_t(__popcll(__ballot(valid))); // valid lanes are a prefix
+	const float4 p0 = home_pos[0];                       // the home frame's anchor
 	for (uint32_t q0 = 0; q0 < ntot; q0 += 64) {
 		const uint32_t qi = q0 + lane;
 		const bool live = qi < ntot;
@@ -172,27 +343,18 @@ __device__ __forceinline__ void home_test_segments(const HomeParams& P, const ui
 			j = seg_b[lo] + (qi - seg_pre[lo]);
 			q = P.spos[j];
 		}
-#pragma unroll 2
-		for (uint32_t i = 0; i < nh; ++i) {
-			const uint32_t ti = c0 + i;
-			const float4 pi = home_pos[i];
-			bool in = false;
-			if (live && j != ti) {
-				in = pair_predicate_home(P.box, P.rc2, P.ex, pi, ti, q, j);
-				if (in && P.ref_filter)
-					in = in_reference_stencil(P, pi, q); // symmetric in its two ends
+		if (P.pre && !P.ref_filter && !P.ex.rigid_body_id && !P.ex.offsets) {
+			// the candidate in the home frame (whole box lengths, so the image is exact);
+			// lanes without a candidate are infinitely far
+			float qx = INFINITY, qy = INFINITY, qz = INFINITY;
+			if (live) {
+				qx = P.box.periodic[0] ? q.x - P.box.L[0] * rintf((q.x - p0.x) * P.box.inv_L[0]) : q.x;
+				qy = P.box.periodic[1] ? q.y - P.box.L[1] * rintf((q.y - p0.y) * P.box.inv_L[1]) : q.y;
+				qz = P.box.periodic[2] ? q.z - P.box.L[2] * rintf((q.z - p0.z) * P.box.inv_L[2]) : q.z;
 			}
-			const uint64_t hits = __ballot(in);
-			if (hits) {
-				const uint32_t base = bcast_u32(cnt, i);
-				if (FILL && in) {
-					const uint32_t k = base + uint32_t(__popcll(hits & lt_mask));
-					if (k < P.cap)
-						nbr[nbr_slot_index((uint64_t(ti) >> 6) * 64u * P.cap, ti, k)] = j;
-				}
-				if (lane == i)
-					cnt += uint32_t(__popcll(hits));
-			}
+			test_candidate_block<true, FILL>(P, home_pos, cand, nh, c0, live, j, q, qx, qy, qz, cnt, nbr);
+		} else {
+			test_candidate_block<false, FILL>(P, home_pos, cand, nh, c0, live, j, q, q.x, q.y, q.z, cnt, nbr);
 		}
 	}
 	wave_lds_sync();
@@ -225,7 +387,7 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 			nax[a] = hgh - l + 1;
 		}
 	}
-	const int total = nax[0] * nax[1] * nax[2];
+	const int total = P.stencil ? int(P.stencil_n) : nax[0] * nax[1] * nax[2];
 	const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
 	for (uint32_t c0 = hb; c0 < he; c0 += 64) {
 		const uint32_t t = c0 + uint32_t(lane);
@@ -236,7 +398,25 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 		for (int cb = 0; cb < total; cb += 64) {
 			const int k = cb + lane;
 			uint32_t b = 0, e = 0;
-			if (k < total) {
+			if (P.stencil && k < total) {
+				const int4 o = P.stencil[k];
+				const int off[3] = {o.x, o.y, o.z};
+				bool keep = true;
+				uint32_t code = 0;
+#pragma unroll
+				for (int a = 0; a < 3; ++a) {
+					int u = P.full[a] ? off[a] : hc[a] * span + off[a];
+					if (P.box.periodic[a])
+						u = (u + fine_cells) & (fine_cells - 1);
+					else if (u < 0 || u >= fine_cells)
+						keep = false;
+					code |= split_by3(uint32_t(u) & 1023u) << (2 - a);
+				}
+				if (keep) {
+					b = P.fine_begin[code];
+					e = P.fine_end[code];
+				}
+			} else if (k < total) {
 				const int iv[3] = {k % nax[0], (k / nax[0]) % nax[1], k / (nax[0] * nax[1])};
 				float gap2 = 0.0f;
 				bool keep = true;
@@ -296,7 +476,7 @@ __global__ __launch_bounds__(256) void k_walk_home(HomeParams P, uint32_t* __res
 												   uint32_t* __restrict__ nbr) {
 	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
 	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
-	__shared__ float4 home_pos[kHomeWaves][64];
+	__shared__ float4 home_pos[kHomeWaves][kHomeLds];
 	const int wv = threadIdx.x >> 6;
 	const TileRange R = xcd_tiles((P.num_home + kHomeWaves - 1) / kHomeWaves);
 	for (uint32_t g = R.tile; g < R.end; g += R.stride) {

@@ -732,6 +732,15 @@ bool PairlistImpl::walk_home(hipStream_t s, bool& outside) {
 	P.rc2 = cutoff_squared;
 	P.box = make_devbox(box);
 	P.ex = excl;
+	P.pre = home_prefilter_band(box, domain_min, domain_max, h, cutoff, &P.pre_lo2, &P.pre_hi2) ? 1 : 0;
+	stencil_host = home_stencil(h, P.s, P.reach, P.full, P.cw, P.margin, P.rc2, 8192);
+	if (!stencil_host.empty()) {
+		stencil_dev.resize(stencil_host.size());
+		MMD_HIP(hipMemcpyAsync(stencil_dev.data(), stencil_host.data(), sizeof(int4) * stencil_host.size(),
+							   hipMemcpyHostToDevice, s));
+		P.stencil = stencil_dev.data();
+		P.stencil_n = uint32_t(stencil_host.size());
+	}
 	if (!ref_complete) {
 		// keep exactly the pairs the reference grid's stencil reaches
 		const int cells = 1 << ref_grid.m;

@@ -3,6 +3,8 @@
 
 #include "internal.hpp"
 
+#include <vector>
+
 namespace mmd {
 
 // Reference cell grid of ZOrderPairlist::select_cell_grid (ZOrderPairlist.cpp:24-81).
@@ -75,6 +77,8 @@ struct PairlistImpl {
 	// home-cell walk: home-level cell ranges and the per-slot capacity of its rows
 	DevArray<uint32_t> home_begin, home_end;
 	uint32_t home_cap = 0;
+	std::vector<int4> stencil_host; // the home walk's fine-cell stencil
+	DevArray<int4> stencil_dev;
 	int home_level = -1; // level of the last home-cell walk (-1: count / scan / fill path)
 	uint64_t nbr_entries = 0; // = 2 * num_pairs
 	uint64_t nbr_storage = 0; // padded entries held by nbr

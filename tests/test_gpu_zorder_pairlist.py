@@ -117,6 +117,27 @@ def test_pair_set_positions_on_box_faces(cuda):
     assert np.array_equal(gp, ref["pairs"].astype(np.int64))
 
 
+def test_pairs_at_the_cutoff_bitexact(cuda):
+    # pairs planted within a few 1e-7 .. 1e-4 relative of the cutoff, many of them across
+    # the periodic seam: the walk decides these with the reference's exact predicate
+    rs = np.random.default_rng(31)
+    n_bg, n_pairs, L, rc = 40_000, 12_000, 800.0, 35.0
+    a = rs.uniform(0, L, size=(n_pairs, 3))
+    a[: n_pairs // 3, 0] = rs.uniform(0, 20.0, size=n_pairs // 3)  # next to the x = 0 face
+    u = rs.normal(size=(n_pairs, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    rel = rs.choice([-1.0, 1.0], n_pairs) * 10.0 ** rs.uniform(-7, -4, n_pairs)
+    b = np.mod(a + u * (rc * (1.0 + rel))[:, None], L)
+    pos = np.zeros((n_bg + 2 * n_pairs, 4), np.float32)
+    pos[:n_bg, :3] = rs.uniform(0, L, size=(n_bg, 3))
+    pos[n_bg::2, :3] = a
+    pos[n_bg + 1::2, :3] = b
+    box = PeriodicBox((L, L, L))
+    _, gp = _gpu_pairs(cuda, pos, box, rc)
+    ref = orc.build_pairlist(pos, box, rc)
+    assert np.array_equal(gp, ref["pairs"].astype(np.int64))
+
+
 def test_known_answers_and_exclusions(cuda):
     pos = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [2, 0, 0, 0]], np.float32)
     tb = PeriodicBox((64.0, 64.0, 64.0), periodic=(False, False, False), origin=(-4.0, -4.0, -4.0))
