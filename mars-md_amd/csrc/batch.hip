@@ -52,6 +52,8 @@ struct RepGeom {
 	// the home-frame prefilter of the walk (see HomeParams)
 	int pre;
 	float pre_lo2, pre_hi2;
+	// the replica's fine-cell stencil in the batch's table (count 0: none)
+	uint32_t stencil_off, stencil_n;
 };
 
 // keys / vals in canonical position: global canonical g = off_r + canon.
@@ -147,7 +149,8 @@ __global__ __launch_bounds__(256) void k_b_walk(const RepGeom* __restrict__ geom
 												const uint32_t* __restrict__ home_end,
 												const uint32_t* __restrict__ fine_begin,
 												const uint32_t* __restrict__ fine_end, float rc2, uint32_t cap,
-												uint32_t* __restrict__ counts, uint32_t* __restrict__ nbr) {
+												const int4* __restrict__ stencils, uint32_t* __restrict__ counts,
+												uint32_t* __restrict__ nbr) {
 	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
 	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
 	__shared__ float4 home_pos[kHomeWaves][kHomeLds];
@@ -198,6 +201,8 @@ __global__ __launch_bounds__(256) void k_b_walk(const RepGeom* __restrict__ geom
 		P.pre = G.pre;
 		P.pre_lo2 = G.pre_lo2;
 		P.pre_hi2 = G.pre_hi2;
+		P.stencil = G.stencil_n ? stencils + G.stencil_off : nullptr;
+		P.stencil_n = G.stencil_n;
 		walk_home_cell<true>(P, item - item_off[r], seg_b[wv], seg_pre[wv], home_pos[wv], counts, nbr);
 	}
 }
@@ -669,6 +674,20 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 		}
 		g.ref_filter = complete ? 0 : 1;
 		g.pre = home_prefilter_band(boxes[r], g.lo, g.hi, h, rc, &g.pre_lo2, &g.pre_hi2) ? 1 : 0;
+		// the walk's stencil, shared by the replicas of one geometry
+		const std::vector<float> key = {float(h),		   float(g.mf - h),	   float(g.reach[0]),
+										float(g.reach[1]), float(g.reach[2]),  float(g.full[0]),
+										float(g.full[1]),  float(g.full[2]),   g.cw[0],
+										g.cw[1],		   g.cw[2],			   g.margin[0],
+										g.margin[1],	   g.margin[2],		   rc2};
+		auto it = stencil_index.find(key);
+		if (it == stencil_index.end()) {
+			const std::vector<int4> st = home_stencil(h, g.mf - h, g.reach, g.full, g.cw, g.margin, rc2, 8192);
+			it = stencil_index.emplace(key, std::make_pair(uint32_t(stencils_host.size()), uint32_t(st.size()))).first;
+			stencils_host.insert(stencils_host.end(), st.begin(), st.end());
+		}
+		g.stencil_off = it->second.first;
+		g.stencil_n = it->second.second;
 		g.ref_shift = 10 - ref.m;
 		g.ref_cells = rcells;
 		g.fine_off = uint32_t(fine_total);
@@ -682,6 +701,12 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 	num_fine = fine_total;
 	num_home = home_total;
 	num_items = items[R];
+	if (stencils_host.size() > stencils_uploaded) { // new geometries (a synchronous upload, rare)
+		stencils.resize(stencils_host.size());
+		MMD_HIP(hipMemcpy(stencils.data(), stencils_host.data(), sizeof(int4) * stencils_host.size(),
+						  hipMemcpyHostToDevice));
+		stencils_uploaded = stencils_host.size();
+	}
 	geom.resize(R * sizeof(RepGeom));
 	MMD_HIP(hipMemcpyAsync(geom.data(), geom_host.data(), R * sizeof(RepGeom), hipMemcpyHostToDevice, s));
 	item_off.resize(R + 1);
@@ -756,7 +781,7 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 		MMD_LAUNCH_CHECK();
 		k_b_walk<<<grid, 256, 0, s>>>(gd, item_off.data(), uint32_t(R), uint32_t(num_items), pos[cur].data(),
 									  home_begin.data(), home_end.data(), fine_begin.data(), fine_end.data(), rc2,
-									  cap, counts.data(), nbr.data());
+									  cap, stencils.data(), counts.data(), nbr.data());
 		MMD_LAUNCH_CHECK();
 		fill_u32(scalars.data() + 2, 0u, 4, s);
 		k_b_count_stats<<<std::min<unsigned>(grid_for(total, 256), 240), 256, 0, s>>>(

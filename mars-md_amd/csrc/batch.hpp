@@ -1,6 +1,7 @@
 // batch.hpp — host class behind marsmd_batch (see batch.hip).
 #pragma once
 
+#include <map>
 #include <vector>
 
 #include "pairlist.hpp"
@@ -34,6 +35,12 @@ struct BatchImpl {
 	uint64_t num_fine = 0, num_home = 0, num_items = 0, nbr_entries = 0;
 	uint32_t cap = 0, max_neighbors = 0;
 	std::vector<uint8_t> geom_host;     // host copies behind the asynchronous uploads
+	// fine-cell stencils of the home walk, one per distinct replica geometry (key: the
+	// walk parameters it depends on; value: offset and count in stencils)
+	std::map<std::vector<float>, std::pair<uint32_t, uint32_t>> stencil_index;
+	std::vector<int4> stencils_host;
+	DevArray<int4> stencils;
+	size_t stencils_uploaded = 0;
 	std::vector<uint32_t> items_host;
 	std::vector<uint64_t> ref_max_pairs; // per replica: Pairlist::max_pairs_ bookkeeping
 	std::vector<uint64_t> rep_pairs_host;

@@ -87,6 +87,19 @@ struct DevArray {
 	}
 };
 
+// Pinned host words for device read-backs: a copy into them is asynchronous, so a
+// read-back costs one stream synchronisation (a copy into pageable memory waits itself).
+struct PinnedWords {
+	uint32_t* ptr = nullptr;
+	explicit PinnedWords(size_t n) { MMD_HIP(hipHostMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(uint32_t))); }
+	PinnedWords(const PinnedWords&) = delete;
+	PinnedWords& operator=(const PinnedWords&) = delete;
+	~PinnedWords() {
+		if (ptr)
+			(void)hipHostFree(ptr);
+	}
+};
+
 // ----------------------------------------------------------- sort.hip
 uint64_t radix_sort_scratch_bytes(uint64_t n);
 // Stable LSD radix sort of (key, value) over bits [0, key_bits). keys_out/vals_out

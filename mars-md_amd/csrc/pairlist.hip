@@ -372,9 +372,14 @@ __global__ __launch_bounds__(256) void k_tile_rows(const uint32_t* __restrict__ 
 
 // Largest count and total entries: grid-stride partials reduced in the workgroup, then
 // one atomic pair per workgroup (launched with a few hundred workgroups).
+// The sorter's flag word is copied to max_count[1], so that one read-back of four words
+// returns the three.
 __global__ __launch_bounds__(256) void k_count_stats(const uint32_t* __restrict__ counts, uint64_t n,
 													 unsigned long long* __restrict__ total,
-													 uint32_t* __restrict__ max_count) {
+													 uint32_t* __restrict__ max_count,
+													 const uint32_t* __restrict__ flag) {
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		max_count[1] = flag[0];
 	__shared__ uint32_t wmax[4];
 	__shared__ unsigned long long wsum[4];
 	uint32_t m = 0;
@@ -675,11 +680,13 @@ void PairlistImpl::walk(bool fill, bool use_ref, hipStream_t s) {
 	MMD_LAUNCH_CHECK();
 }
 
-// Largest count and total entries of `counts` into dev_scalars [2] and [4..5].
+// Largest count, the sorter's flag word and total entries of `counts` into dev_scalars
+// [2], [3] and [4..5].
 void PairlistImpl::count_stats(uint64_t nn, hipStream_t s) {
 	fill_u32(dev_scalars.data() + 2, 0u, 4, s);
 	k_count_stats<<<std::min<unsigned>(grid_for(nn, 256), 240), 256, 0, s>>>(
-		counts.data(), nn, reinterpret_cast<unsigned long long*>(dev_scalars.data() + 4), dev_scalars.data() + 2);
+		counts.data(), nn, reinterpret_cast<unsigned long long*>(dev_scalars.data() + 4), dev_scalars.data() + 2,
+		sorter.outside_flag.data());
 	MMD_LAUNCH_CHECK();
 }
 
@@ -733,11 +740,21 @@ bool PairlistImpl::walk_home(hipStream_t s, bool& outside) {
 	P.box = make_devbox(box);
 	P.ex = excl;
 	P.pre = home_prefilter_band(box, domain_min, domain_max, h, cutoff, &P.pre_lo2, &P.pre_hi2) ? 1 : 0;
-	stencil_host = home_stencil(h, P.s, P.reach, P.full, P.cw, P.margin, P.rc2, 8192);
+	// the stencil depends on the geometry only: made and uploaded when that changes
+	const std::vector<float> key = {float(h),	   float(P.s),		   float(P.reach[0]), float(P.reach[1]),
+									float(P.reach[2]), float(P.full[0]),   float(P.full[1]),  float(P.full[2]),
+									P.cw[0],		   P.cw[1],			   P.cw[2],			  P.margin[0],
+									P.margin[1],	   P.margin[2],		   P.rc2};
+	if (key != stencil_key) {
+		stencil_key = key;
+		stencil_host = home_stencil(h, P.s, P.reach, P.full, P.cw, P.margin, P.rc2, 8192);
+		if (!stencil_host.empty()) {
+			stencil_dev.resize(stencil_host.size());
+			MMD_HIP(hipMemcpy(stencil_dev.data(), stencil_host.data(), sizeof(int4) * stencil_host.size(),
+							  hipMemcpyHostToDevice));
+		}
+	}
 	if (!stencil_host.empty()) {
-		stencil_dev.resize(stencil_host.size());
-		MMD_HIP(hipMemcpyAsync(stencil_dev.data(), stencil_host.data(), sizeof(int4) * stencil_host.size(),
-							   hipMemcpyHostToDevice, s));
 		P.stencil = stencil_dev.data();
 		P.stencil_n = uint32_t(stencil_host.size());
 	}
@@ -786,18 +803,17 @@ bool PairlistImpl::walk_home(hipStream_t s, bool& outside) {
 		k_walk_home<true><<<grid, 256, 0, s>>>(P, counts.data(), nbr.data());
 		MMD_LAUNCH_CHECK();
 		count_stats(n, s);
-		uint32_t host[5] = {0, 0, 0, 0, 0};
-		MMD_HIP(hipMemcpyAsync(&host[0], sorter.outside_flag.data(), 4, hipMemcpyDeviceToHost, s));
-		MMD_HIP(hipMemcpyAsync(&host[1], dev_scalars.data() + 2, 16, hipMemcpyDeviceToHost, s));
+		uint32_t* host = readback.ptr; // max count, flag word, total entries (2 words)
+		MMD_HIP(hipMemcpyAsync(host, dev_scalars.data() + 2, 16, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipStreamSynchronize(s));
-		if (host[0] & 2u)
+		if (host[1] & 2u)
 			raise(MARSMD_ERR_RUNTIME, "pairlist: keys after a reorder were not in order");
-		if (host[0] != 0) {
+		if (host[1] != 0) {
 			outside = true;
 			return true;
 		}
-		max_neighbors = host[1];
-		std::memcpy(&nbr_entries, &host[3], 8);
+		max_neighbors = host[0];
+		std::memcpy(&nbr_entries, &host[2], 8);
 		if (max_neighbors <= home_cap) {
 			nbr_storage = storage;
 			home_level = h;
@@ -899,11 +915,11 @@ void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool 
 		MMD_LAUNCH_CHECK();
 		count_stats(nn, s);
 		exclusive_scan_u32(tile_rows.data(), tile_base.data(), tiles, scan_scratch.data(), s);
-		uint32_t host[6] = {0, 0, 0, 0, 0, 0};
+		uint32_t* host = readback.ptr; // storage, then max count, flag word, total entries
 		MMD_HIP(hipMemcpyAsync(&host[0], tile_base.data() + tiles, 4, hipMemcpyDeviceToHost, s));
-		MMD_HIP(hipMemcpyAsync(&host[1], sorter.outside_flag.data(), 4, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipMemcpyAsync(&host[2], dev_scalars.data() + 2, 16, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipStreamSynchronize(s));
+		host[1] = host[3]; // the flag word, where the checks below read it
 		if (host[1] & 2u)
 			raise(MARSMD_ERR_RUNTIME, "pairlist: keys after a reorder were not in order");
 		if (host[1] != 0 && !use_ref) {

@@ -79,6 +79,8 @@ struct PairlistImpl {
 	uint32_t home_cap = 0;
 	std::vector<int4> stencil_host; // the home walk's fine-cell stencil
 	DevArray<int4> stencil_dev;
+	std::vector<float> stencil_key; // the parameters stencil_dev was made for
+	PinnedWords readback{8};
 	int home_level = -1; // level of the last home-cell walk (-1: count / scan / fill path)
 	uint64_t nbr_entries = 0; // = 2 * num_pairs
 	uint64_t nbr_storage = 0; // padded entries held by nbr

@@ -136,67 +136,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate_baoab(float4* __restrict__
 	mom[t] = mo;
 }
 
-// Rows of a block of kBalSlots slots are walked in order of length: the lanes of a
-// wave run for as many groups of 4 entries as the longest of their 64 rows, and slots
-// that are neighbours in Morton order have unrelated lengths (a wave of consecutive
-// slots walks ~1.5x the mean length of its rows, one of ranked slots ~1.05x).
-constexpr uint32_t kBalShift = 8;
-constexpr uint32_t kBalSlots = 1u << kBalShift;
-
-// Rank the slots of each block by group count (ties by slot; bitonic sort of
-// count << 10 | offset in LDS) and copy each slot's row to its rank in a list of
-// uniform tile capacity (`cap_groups` groups of 4 entries per lane). The entries are
-// copied as they are (pads included), so a rank's row holds its slot's row.
-__global__ __launch_bounds__(kBalSlots) void k_balance(NbrList L, uint64_t n, uint32_t cap_groups,
-												 uint32_t* __restrict__ slot_of_rank, uint32_t* __restrict__ bcount,
-												 uint4* __restrict__ bnbr) {
-	__shared__ uint32_t key[kBalSlots];
-	const uint64_t b0 = uint64_t(blockIdx.x) * kBalSlots;
-	const uint32_t nb = uint32_t(min<uint64_t>(kBalSlots, n - b0));
-	for (uint32_t i = threadIdx.x; i < kBalSlots; i += blockDim.x)
-		key[i] = i < nb ? (((L.count[b0 + i] + 3u) >> 2) << kBalShift) | i : 0xFFFFFFFFu;
-	__syncthreads();
-	for (uint32_t k = 2; k <= kBalSlots; k <<= 1) {
-		for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-			for (uint32_t i = threadIdx.x; i < kBalSlots; i += blockDim.x) {
-				const uint32_t l = i ^ j;
-				if (l > i) {
-					const uint32_t a = key[i], c = key[l];
-					if ((a > c) == ((i & k) == 0)) {
-						key[i] = c;
-						key[l] = a;
-					}
-				}
-			}
-			__syncthreads();
-		}
-	}
-	for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
-		const uint64_t r = b0 + i;
-		const uint64_t t = b0 + (key[i] & (kBalSlots - 1u));
-		const uint32_t c = L.count[t];
-		slot_of_rank[r] = uint32_t(t);
-		bcount[r] = c;
-		const uint4* __restrict__ src = reinterpret_cast<const uint4*>(L.nbr + L.tile_base[t >> 6]) + (t & 63u);
-		uint4* __restrict__ dst = bnbr + (r >> 6) * 64u * cap_groups + (r & 63u);
-		const uint32_t ng = (c + 3u) >> 2;
-		for (uint32_t g = 0; g < ng; ++g)
-			dst[uint64_t(g) * 64u] = src[uint64_t(g) * 64u];
-	}
-}
-
-__global__ void k_tile_stride(uint32_t* __restrict__ tile_base, uint64_t tiles, uint32_t stride) {
-	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (t <= tiles)
-		tile_base[t] = uint32_t(t * stride);
-}
-
 // The fused step: nonbonded + bonded (from `force` when has_bonded) + PMF + integrator,
 // persistent over XCD-local tiles so the table knots are staged once per workgroup.
-// With ranked rows (slot_of_rank, ranks within the 256 slots of a tile) a thread walks
-// the row of rank r and integrates its slot t, while the position-dependent force is
-// sampled per slot in slot order, where the 64 particles of a wave are neighbours in
-// space and so share grid lines, and handed over through LDS.
 // Bounded to 128 VGPRs (4 waves per SIMD; the tricubic sampler alone would take ~140
 // and leave 3): measured faster despite a few dwords of spill.
 template<int INTEG, bool ENERGY, bool ANALYTIC, bool PMF_IN>
@@ -205,69 +146,50 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 												 NbrList L, NbTables T, const float* __restrict__ knots_g,
 												 marsmd_particle_types types_arg,
 												 const marsmd_grid_view* __restrict__ grids, StepParams S,
-												 int has_bonded, int store_force, const float4* __restrict__ pmf_in,
-												 const uint32_t* __restrict__ slot_of_rank) {
-	static_assert(kBalSlots == uint32_t(kBlock), "ranks stay inside the tile");
+												 int has_bonded, int store_force, const float4* __restrict__ pmf_in) {
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
 	__shared__ marsmd_particle_types lds_types;
-	__shared__ float4 lds_pmf[2][kBlock]; // alternate tiles: one barrier per tile
 	stage_types(types_arg, lds_types);
 	stage_tables(T, knots_g, lds_knots, lds_nb);
 	const marsmd_particle_types& types = lds_types;
-	// the position-dependent force enters BD always (field and grids) and BAOAB with grids
-	const bool sample = !PMF_IN && (INTEG == MARSMD_INTEGRATOR_BROWNIAN || S.has_grids);
-	const marsmd_grid_view* gv = S.has_grids ? grids : nullptr;
 	const TileRange R = xcd_tiles(uint32_t((n + kBlock - 1) / kBlock));
-	int buf = 0;
 	for (uint32_t tile = R.tile; tile < R.end; tile += R.stride) {
-		const uint64_t base = uint64_t(tile) * kBlock;
-		const uint64_t r = base + threadIdx.x;
-		float4 pmf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		if (sample && r < n) {
-			const float4 q = pos[r];
-			pmf = position_dependent_force(q.x, q.y, q.z, int(meta_type(fb(q.w))), types, gv, S.ef, S.scheme);
+		const uint64_t t = uint64_t(tile) * kBlock + threadIdx.x;
+		if (t >= n)
+			continue;
+		const float4 p = pos[t];
+		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, lds_knots, knots_g, lds_nb, S.box, S.cutoff2, ENERGY);
+		if (has_bonded) {
+			const float4 b = force[t];
+			f.x += b.x;
+			f.y += b.y;
+			f.z += b.z;
+			f.w += b.w;
 		}
-		if (sample && slot_of_rank) {
-			lds_pmf[buf][threadIdx.x] = pmf;
-			__syncthreads();
-		}
-		if (r < n) {
-			// the list row of rank r is that of slot t (the same slot without a ranking)
-			const uint64_t t = slot_of_rank ? uint64_t(slot_of_rank[r]) : r;
-			if (PMF_IN)
-				pmf = pmf_in[t];
-			else if (sample && slot_of_rank)
-				pmf = lds_pmf[buf][t - base];
-			const float4 p = pos[t];
-			float4 f =
-				nb_force<ANALYTIC>(uint32_t(r), p, pos, L, T, lds_knots, knots_g, lds_nb, S.box, S.cutoff2, ENERGY);
-			if (has_bonded) {
-				const float4 b = force[t];
-				f.x += b.x;
-				f.y += b.y;
-				f.z += b.z;
-				f.w += b.w;
+		float4 pn;
+		if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
+			const float4 pmf = PMF_IN ? pmf_in[t]
+									  : position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types,
+																 S.has_grids ? grids : nullptr, S.ef, S.scheme);
+			pn = bd_update(p, f, types, pmf, S);
+		} else {
+			if (S.has_grids) {
+				const float4 g = PMF_IN ? pmf_in[t]
+										: position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types,
+																   grids, S.ef, S.scheme);
+				f.x += g.x;
+				f.y += g.y;
+				f.z += g.z;
+				f.w += g.w;
 			}
-			float4 pn;
-			if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
-				pn = bd_update(p, f, types, pmf, S);
-			} else {
-				if (S.has_grids) {
-					f.x += pmf.x;
-					f.y += pmf.y;
-					f.z += pmf.z;
-					f.w += pmf.w;
-				}
-				float4 mo = mom[t];
-				pn = baoab_update(p, mo, f, types, S);
-				mom[t] = mo;
-			}
-			pos_next[t] = pn;
-			if (ENERGY || store_force)
-				force[t] = f;
+			float4 mo = mom[t];
+			pn = baoab_update(p, mo, f, types, S);
+			mom[t] = mo;
 		}
-		buf ^= 1;
+		pos_next[t] = pn;
+		if (ENERGY || store_force)
+			force[t] = f;
 	}
 }
 
@@ -293,12 +215,18 @@ StepFn pick_step(bool bd, bool energy, bool an, bool pmf_in) {
 
 // Permute slot-indexed state into the new sorted order:
 // out[t] = in[old_slot_of_canon[sorted_canon[t]]].
+// The slot-order state after a build: sorted positions, momenta and ids gathered from
+// the old slots, and the new canonical -> slot map (into slot_out: old_slot is read).
 __global__ void k_gather_state(const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ old_slot, uint64_t n,
+							   const float4* __restrict__ spos, float4* __restrict__ pos_out,
+							   const uint32_t* __restrict__ inverse, uint32_t* __restrict__ slot_out,
 							   const float4* __restrict__ mom_in, float4* __restrict__ mom_out,
 							   const int32_t* __restrict__ ids_in, int32_t* __restrict__ ids_out) {
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
+	pos_out[t] = spos[t];
+	slot_out[t] = inverse[t];
 	const uint32_t s = old_slot[sidx[t]];
 	if (s >= n)
 		return;
@@ -704,15 +632,16 @@ void PatchImpl::build(float pairlist_cutoff, hipStream_t s) {
 	}
 	pl->excl = ev;
 	pl->build(pos[cur].data(), slot_of_canon.data(), true, nullptr, n, pairlist_cutoff, s);
-	bal_valid = false;
 	// The sorted positions become the slot-order state; gather the rest.
-	MMD_HIP(hipMemcpyAsync(pos[cur].data(), pl->spos.data(), 16 * n, hipMemcpyDeviceToDevice, s));
-	k_gather_state<<<grid_for(n, 256), 256, 0, s>>>(pl->sorter.sorted_idx.data(), slot_of_canon.data(), n, mom.data(),
-													mom_tmp.data(), ids.data(), ids_tmp.data());
+	slot_tmp.resize(std::max<uint64_t>(n, 1));
+	k_gather_state<<<grid_for(n, 256), 256, 0, s>>>(pl->sorter.sorted_idx.data(), slot_of_canon.data(), n,
+													pl->spos.data(), pos[cur].data(), pl->sorter.inverse.data(),
+													slot_tmp.data(), mom.data(), mom_tmp.data(), ids.data(),
+													ids_tmp.data());
 	MMD_LAUNCH_CHECK();
 	mom.swap(mom_tmp);
 	ids.swap(ids_tmp);
-	MMD_HIP(hipMemcpyAsync(slot_of_canon.data(), pl->sorter.inverse.data(), 4 * n, hipMemcpyDeviceToDevice, s));
+	slot_of_canon.swap(slot_tmp);
 	if (num_bonds) {
 		k_topo_to_slots<<<grid_for(2 * num_bonds, 256), 256, 0, s>>>(bonds_canon.data(), bonds_slot.data(),
 																	  2 * num_bonds, slot_of_canon.data(), n);
@@ -734,30 +663,6 @@ void PatchImpl::build(float pairlist_cutoff, hipStream_t s) {
 		MMD_LAUNCH_CHECK();
 	}
 	built = true;
-}
-
-NbrList PatchImpl::step_list(hipStream_t s, const uint32_t** slot_of_rank) {
-	*slot_of_rank = nullptr;
-	const uint64_t tiles = (n + 63) / 64;
-	const uint64_t cap_groups = (uint64_t(pl->max_neighbors) + 3) / 4;
-	const uint64_t storage = tiles * 64 * 4 * cap_groups;
-	if (storage >= (uint64_t(1) << 32))
-		return pl->list();
-	if (!bal_valid) {
-		bal_slot.resize(std::max<uint64_t>(n, 1));
-		bal_count.resize(std::max<uint64_t>(n, 1));
-		bal_tile_base.resize(tiles + 1);
-		bal_nbr.resize(std::max<uint64_t>(storage, 4));
-		k_tile_stride<<<grid_for(tiles + 1, 256), 256, 0, s>>>(bal_tile_base.data(), tiles, uint32_t(64 * 4 * cap_groups));
-		MMD_LAUNCH_CHECK();
-		k_balance<<<unsigned((n + kBalSlots - 1) / kBalSlots), kBalSlots, 0, s>>>(
-			pl->list(), n, uint32_t(cap_groups), bal_slot.data(), bal_count.data(),
-			reinterpret_cast<uint4*>(bal_nbr.data()));
-		MMD_LAUNCH_CHECK();
-		bal_valid = true;
-	}
-	*slot_of_rank = bal_slot.data();
-	return NbrList{bal_count.data(), bal_tile_base.data(), bal_nbr.data()};
 }
 
 bool PatchImpl::rebuild_policy(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period, hipStream_t s) {
@@ -985,8 +890,6 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 		const unsigned grid = persistent_grid(reinterpret_cast<const void*>(kern), n);
 		if (pmf_in)
 			pmf_buf.resize(n);
-		const uint32_t* slot_of_rank = nullptr;
-		const NbrList L = step_list(s, &slot_of_rank);
 		if (timing) {
 			while (timing_events.size() < timing_used + 2) {
 				hipEvent_t e;
@@ -999,9 +902,9 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 			k_pmf_force<<<g, kBlock, 0, s>>>(pos[cur].data(), n, types, grids_dev.data(), S, pmf_buf.data());
 			MMD_LAUNCH_CHECK();
 		}
-		kern<<<grid, kBlock, lds_bytes(), s>>>(pos[cur].data(), pnext, mom.data(), force.data(), n, L, tables->nb,
-											   tables->nb_knots.data(), types, grids_dev.data(), S, has_bonded, store,
-											   pmf_in ? pmf_buf.data() : nullptr, slot_of_rank);
+		kern<<<grid, kBlock, lds_bytes(), s>>>(pos[cur].data(), pnext, mom.data(), force.data(), n, pl->list(),
+											   tables->nb, tables->nb_knots.data(), types, grids_dev.data(), S,
+											   has_bonded, store, pmf_in ? pmf_buf.data() : nullptr);
 		if (timing) {
 			MMD_HIP(hipEventRecord(timing_events[timing_used + 1], s));
 			timing_used += 2;

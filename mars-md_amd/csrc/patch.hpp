@@ -39,7 +39,7 @@ struct PatchImpl {
 	int cur = 0;
 	DevArray<float4> mom, mom_tmp, force;
 	DevArray<int32_t> ids, ids_tmp, remap_tmp;
-	DevArray<uint32_t> slot_of_canon, relabel;
+	DevArray<uint32_t> slot_of_canon, slot_tmp, relabel;
 	std::unique_ptr<PairlistImpl> pl;
 	bool built = false, force_rebuild = false, deferred_kick = false, bonded_prepared = false;
 	float efield[3] = {0, 0, 0};
@@ -56,11 +56,6 @@ struct PatchImpl {
 	int blocks_per_cu = 0;
 	bool split_pmf = false;
 	DevArray<float4> pmf_buf;
-	// the step kernel's copy of the neighbour list, rows ordered by length within
-	// blocks of slots (bal_slot: rank -> slot; bal_count, bal_nbr indexed by rank), so
-	// that the lanes of a wave walk rows of about the same length
-	DevArray<uint32_t> bal_slot, bal_count, bal_tile_base, bal_nbr;
-	bool bal_valid = false;
 
 	explicit PatchImpl(const marsmd_patch_desc& d);
 	~PatchImpl();
@@ -86,9 +81,6 @@ struct PatchImpl {
 
   private:
 	void build(float pairlist_cutoff, hipStream_t s);
-	// the list the step kernel walks: the balanced copy when it fits 32-bit offsets,
-	// else the build's own (slot_of_rank null)
-	NbrList step_list(hipStream_t s, const uint32_t** slot_of_rank);
 	bool rebuild_policy(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period, hipStream_t s);
 	void rebuild_exclusions(hipStream_t s);
 	void k_remap_canon(int32_t* idx, uint64_t count, hipStream_t s);
