@@ -389,13 +389,21 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 	}
 	const int total = P.stencil ? int(P.stencil_n) : nax[0] * nax[1] * nax[2];
 	const uint64_t lt_mask = (uint64_t(1) << lane) - 1;
+	// The segments do not depend on the chunk of home particles: when the first chunk's
+	// enumeration fits LDS whole, the later chunks test the same segments.
+	uint32_t nseg = 0, ntot = 0; // wave-uniform
+	bool cached = false;
 	for (uint32_t c0 = hb; c0 < he; c0 += 64) {
 		const uint32_t t = c0 + uint32_t(lane);
 		const bool valid = t < he;
 		const float4 p = valid ? P.spos[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		uint32_t cnt = 0;
-		uint32_t nseg = 0, ntot = 0; // wave-uniform
-		for (int cb = 0; cb < total; cb += 64) {
+		if (!cached) {
+			nseg = 0;
+			ntot = 0;
+		}
+		bool flushed = false;
+		for (int cb = cached ? total : 0; cb < total; cb += 64) {
 			const int k = cb + lane;
 			uint32_t b = 0, e = 0;
 			if (P.stencil && k < total) {
@@ -448,6 +456,7 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 				home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr, home_pos);
 				nseg = 0;
 				ntot = 0;
+				flushed = true;
 			}
 			const uint32_t incl = wave_inclusive_scan(c);
 			const uint32_t sum = bcast_u32(incl, 63);
@@ -460,6 +469,7 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 			ntot += sum;
 		}
 		home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr, home_pos);
+		cached = !flushed;
 		if (valid) {
 			counts[t] = cnt;
 			if (FILL && cnt <= P.cap) {
