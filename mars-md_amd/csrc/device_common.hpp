@@ -33,6 +33,9 @@ struct DevBox {
 	float inv_L[3];
 	float origin[3];
 	int periodic[3];
+	// wrap_diff without a branch per axis: on an open axis a length of 3e38 with an
+	// inverse of 0 (the image is +-0 and the difference is returned unchanged)
+	float wrap_L[3], wrap_inv[3], wrap_half[3];
 };
 
 inline DevBox make_devbox(const marsmd_box& b) {
@@ -42,6 +45,9 @@ inline DevBox make_devbox(const marsmd_box& b) {
 		d.inv_L[a] = b.size[a] > 0.0f ? 1.0f / b.size[a] : 0.0f;
 		d.origin[a] = b.origin[a];
 		d.periodic[a] = b.periodic[a] != 0 && b.size[a] > 0.0f;
+		d.wrap_L[a] = d.periodic[a] ? d.L[a] : 3e38f;
+		d.wrap_inv[a] = d.periodic[a] ? d.inv_L[a] : 0.0f;
+		d.wrap_half[a] = 0.5f * d.wrap_L[a];
 	}
 	return d;
 }
@@ -69,6 +75,39 @@ __device__ __forceinline__ float3 wrap_diff(const DevBox& box, float3 d) {
 		d.y = wrap_diff_scalar(d.y, box.L[1], box.inv_L[1]);
 	if (box.periodic[2])
 		d.z = wrap_diff_scalar(d.z, box.L[2], box.inv_L[2]);
+	return d;
+}
+// Correctly rounded sqrt(x) for x = 0 and x >= 2^-96: the hardware square root (within
+// one ulp) corrected by the signs of the residuals x - s'.s of its two neighbours s', as
+// the compiler's lowering of sqrtf does, without that lowering's rescaling of inputs
+// below 2^-96 and its zero / infinity select (0 comes out as 0). Equal to sqrtf on every
+// float from 2^-96 to 2^24 and on 0 (checked on the GPU, bit for bit). Below 2^-96 the
+// result may differ from sqrtf; callers use it where such distances (< 4e-15) are
+// skipped whatever their value.
+__device__ __forceinline__ float sqrt_rn_nonneg(float x) {
+	const float s = __builtin_amdgcn_sqrtf(x);
+	const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+	const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+	float r = s;
+	if (fmaf(-sm, s, x) <= 0.0f)
+		r = sm;
+	if (fmaf(-sp, s, x) > 0.0f)
+		r = sp;
+	return r;
+}
+
+// wrap_diff with the same operations on every axis (see DevBox::wrap_L): on a periodic
+// axis they are wrap_diff_scalar's; on an open one x - (+-0) * 3e38 = x, and x never
+// reaches 1.5e38.
+__device__ __forceinline__ float wrap_axis(float x, float l, float inv_l, float half_l) {
+	const float image = floorf(x * inv_l);
+	x -= image * l;
+	return x >= half_l ? x - l : x;
+}
+__device__ __forceinline__ float3 wrap_diff_uniform(const DevBox& box, float3 d) {
+	d.x = wrap_axis(d.x, box.wrap_L[0], box.wrap_inv[0], box.wrap_half[0]);
+	d.y = wrap_axis(d.y, box.wrap_L[1], box.wrap_inv[1], box.wrap_half[1]);
+	d.z = wrap_axis(d.z, box.wrap_L[2], box.wrap_inv[2], box.wrap_half[2]);
 	return d;
 }
 __device__ __forceinline__ float3 sub3(const float4& a, const float4& b) {

@@ -70,8 +70,9 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		const bool tab = size != 0;
 		// py - px is q - p or its negation, exactly (rounding is sign-symmetric)
 		const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-		const float3 r = wrap_diff(box, t_is_x ? make_float3(dx, dy, dz) : make_float3(-dx, -dy, -dz));
-		const float d = sqrtf(len2(r)); // correctly rounded: d picks the table bin
+		const float3 r = wrap_diff_uniform(box, t_is_x ? make_float3(dx, dy, dz) : make_float3(-dx, -dy, -dz));
+		// correctly rounded (d picks the table bin); distances below 1e-6 are skipped
+		const float d = sqrt_rn_nonneg(len2(r));
 		// the unit vector only scales the force: the hardware reciprocal (1 ulp) in place
 		// of the correctly rounded division moves the force by ~1e-7 relative
 		// (floored at 1e-6, so the zero-distance pads give a finite product with +0)
@@ -89,8 +90,10 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		const float U1 = pot[hc + 1];
 		const float dU = U1 - U0;
 		const bool beyond = home >= size - 1;
-		const float fm = 0.0f + (beyond ? 0.0f : -dU * tp.y);
-		const float en = 0.0f + (beyond ? U1 : dU * w + U0);
+		// (the reference adds these to a +0 accumulator; without that add a -0 can reach
+		// the sums only as a product, and adding -0 to a sum leaves its bits unchanged)
+		const float fm = beyond ? 0.0f : -dU * tp.y;
+		const float en = beyond ? U1 : dU * w + U0;
 		// the lower canonical end takes -F: -(fm u) = (-fm) u exactly
 		const float sfm = (keep && d > 1e-6f) ? (t_is_x ? -fm : fm) : 0.0f;
 		fx += sfm * (inv * r.x);
