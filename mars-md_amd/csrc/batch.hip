@@ -475,8 +475,8 @@ BatchImpl::BatchImpl(const marsmd_batch_desc& d) : tables(new NbTables{}) {
 		off[r + 1] = off[r] + d.num_particles[r];
 	}
 	total = off[R];
-	if (total == 0 || total >= (uint64_t(1) << 32))
-		raise(MARSMD_ERR_VALUE, "batch: total particle count must be in [1, 2^32)");
+	if (total == 0 || total >= (uint64_t(1) << 28)) // position records addressed by 32-bit byte offsets
+		raise(MARSMD_ERR_VALUE, "batch: total particle count must be in [1, 2^28)");
 	while ((uint64_t(1) << rep_bits) < R)
 		++rep_bits;
 	// pair tables (as PatchImpl)

@@ -146,14 +146,20 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		e += en * 0.5f;
 }
 
+// Position record of slot j, addressed by a 32-bit byte offset from the array base
+// (scalar base + vector offset addressing; every caller holds fewer than 2^28 slots).
+__device__ __forceinline__ float4 pos_at(const float4* __restrict__ pos, uint32_t j) {
+	return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pos) + (j << 4));
+}
+
 // Sum of the nonbonded pair terms on slot t over its neighbour list, in list order.
 // A lane's list is read in groups of 4 indices (one 16-byte load) and pipelined three
 // deep: while group g is evaluated, the positions of group g+1 and the indices of
 // group g+2 are in flight. The padding entries of a row are t itself (skipped as d = 0).
 template<bool ANALYTIC = false>
 __device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, const float4* __restrict__ pos,
-										   const NbrList& L, const NbTables& T, const float* __restrict__ knots,
-										   const NbLds& S, const DevBox& box, float cutoff2, bool energy) {
+											   const NbrList& L, const NbTables& T, const float* __restrict__ knots,
+											   const NbLds& S, const DevBox& box, float cutoff2, bool energy) {
 	const uint32_t mt = fb(p.w);
 	const uint32_t ct = meta_canon(mt);
 	const uint32_t tt = meta_type(mt);
@@ -163,20 +169,20 @@ __device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, con
 		return make_float4(fx, fy, fz, e);
 	const uint4* __restrict__ row = reinterpret_cast<const uint4*>(L.nbr + L.tile_base[t >> 6]) + (t & 63u);
 	// two register sets (a, b) take turns: while one group is evaluated the positions
-	// of the next are in flight in the other set, and the indices of the one after
-	const uint4 ja = row[0];
-	float4 a0 = pos[ja.x], a1 = pos[ja.y], a2 = pos[ja.z], a3 = pos[ja.w];
+	// of the next are in flight in the other set, and the indices of the one after;
+	// each index register is reloaded in place once its gathers are issued
+	uint4 ja = row[0];
+	float4 a0 = pos_at(pos, ja.x), a1 = pos_at(pos, ja.y), a2 = pos_at(pos, ja.z), a3 = pos_at(pos, ja.w);
 	float4 b0, b1, b2, b3;
 	uint4 jb = ng > 1 ? row[64] : ja;
 	for (uint32_t g = 0;; g += 2) {
-		uint4 jc = jb;
 		if (g + 1 < ng) {
-			b0 = pos[jb.x];
-			b1 = pos[jb.y];
-			b2 = pos[jb.z];
-			b3 = pos[jb.w];
+			b0 = pos_at(pos, jb.x);
+			b1 = pos_at(pos, jb.y);
+			b2 = pos_at(pos, jb.z);
+			b3 = pos_at(pos, jb.w);
 			if (g + 2 < ng)
-				jc = row[uint64_t(g + 2) * 64u];
+				ja = row[uint64_t(g + 2) * 64u];
 		}
 		nb_pair<ANALYTIC>(p, ct, tt, a0, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
 		nb_pair<ANALYTIC>(p, ct, tt, a1, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
@@ -185,10 +191,10 @@ __device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, con
 		if (g + 1 >= ng)
 			break;
 		if (g + 2 < ng) {
-			a0 = pos[jc.x];
-			a1 = pos[jc.y];
-			a2 = pos[jc.z];
-			a3 = pos[jc.w];
+			a0 = pos_at(pos, ja.x);
+			a1 = pos_at(pos, ja.y);
+			a2 = pos_at(pos, ja.z);
+			a3 = pos_at(pos, ja.w);
 			if (g + 3 < ng)
 				jb = row[uint64_t(g + 3) * 64u];
 		}
