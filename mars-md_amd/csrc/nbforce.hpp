@@ -32,8 +32,9 @@ struct NbLds {
 	uint32_t tag[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES]; // [x type << 4 | y type]
 	float4 tab[kMaxTables];        // {start, step_inv, size, offset} (size, offset as int bits)
 	float4 coef[MARSMD_MAX_TYPES]; // {charge, softcore radius, softcore eps, 0}
-	// per type pair: its enabled table's tab record, size 0 when the pair has no
-	// enabled tabulated term (one read per pair on the tabulated-only path)
+	// per type pair: {start, step_inv, last, offset} of its enabled table, last = size - 2
+	// (int bits; -2 when the pair has no enabled tabulated term): one read per pair on
+	// the tabulated-only path
 	float4 pairtab[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
 };
 
@@ -66,8 +67,8 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		// never become -0). Straight-line code lets the four pairs of a group overlap
 		// their LDS reads.
 		const float4 tp = S.pairtab[(tx << 4) | ty];
-		const int size = __float_as_int(tp.z);
-		const bool tab = size != 0;
+		const int last = __float_as_int(tp.z); // table size - 2
+		const bool tab = last >= 0;
 		// py - px is q - p or its negation, exactly (rounding is sign-symmetric)
 		const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
 		const float3 r = wrap_diff_uniform(box, t_is_x ? make_float3(dx, dy, dz) : make_float3(-dx, -dy, -dz));
@@ -77,30 +78,31 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		// of the correctly rounded division moves the force by ~1e-7 relative
 		// (floored at 1e-6, so the zero-distance pads give a finite product with +0)
 		const float inv = __builtin_amdgcn_rcpf(fmaxf(d, 1e-6f));
-		// the pairs the reference evaluates; at d == 1e-6 exactly its unit vector is 0
-		const bool keep = tab && !(d < 1e-6f) && !(cutoff2 > 0.0f && d * d > cutoff2);
+		// the pairs the reference evaluates (d >= 1e-6, within the cutoff); at d == 1e-6
+		// exactly its unit vector is 0, so the force needs d > 1e-6
+		const bool in_cut = tab && !(cutoff2 > 0.0f && d * d > cutoff2);
 		// TabulatedPotential::compute (TabulatedPotential.h:19-52), non-periodic; a pair
-		// without the term reads knots 0 and 1 (size 0 clamps the bin to 0) and adds 0
+		// without the term reads knots 0 and 1 (last -2 clamps the bin to 0) and adds 0
 		const float* pot = knots + __float_as_int(tp.w);
 		float w = (d - tp.x) * tp.y;
 		const int home = int(floorf(w));
 		w = w - float(home);
-		const int hc = max(min(home, size - 2), 0);
+		const int hc = max(min(home, last), 0);
 		const float U0 = pot[hc];
 		const float U1 = pot[hc + 1];
 		const float dU = U1 - U0;
-		const bool beyond = home >= size - 1;
+		const bool beyond = home > last;
 		// (the reference adds these to a +0 accumulator; without that add a -0 can reach
 		// the sums only as a product, and adding -0 to a sum leaves its bits unchanged)
 		const float fm = beyond ? 0.0f : -dU * tp.y;
 		const float en = beyond ? U1 : dU * w + U0;
 		// the lower canonical end takes -F: -(fm u) = (-fm) u exactly
-		const float sfm = (keep && d > 1e-6f) ? (t_is_x ? -fm : fm) : 0.0f;
+		const float sfm = (in_cut && d > 1e-6f) ? (t_is_x ? -fm : fm) : 0.0f;
 		fx += sfm * (inv * r.x);
 		fy += sfm * (inv * r.y);
 		fz += sfm * (inv * r.z);
 		if (energy)
-			e += keep ? en * 0.5f : 0.0f;
+			e += (in_cut && !(d < 1e-6f)) ? en * 0.5f : 0.0f;
 		return;
 	}
 	const uint32_t tag = S.tag[(tx << 4) | ty];
@@ -218,9 +220,9 @@ __device__ __forceinline__ void stage_params(const NbTables& T, NbLds& S) {
 		S.tag[k] = tag;
 		const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
 		S.pairtab[k] = (tag & T.enabled & MARSMD_PAIR_TERM_TABULATED) && ti < T.num_tables
-						   ? make_float4(T.start[ti], T.step_inv[ti], __int_as_float(T.size[ti]),
+						   ? make_float4(T.start[ti], T.step_inv[ti], __int_as_float(T.size[ti] - 2),
 										 __int_as_float(T.offset[ti]))
-						   : make_float4(0.0f, 0.0f, __int_as_float(0), __int_as_float(0));
+						   : make_float4(0.0f, 0.0f, __int_as_float(-2), __int_as_float(0));
 	}
 	for (int i = threadIdx.x; i < T.num_tables; i += blockDim.x)
 		S.tab[i] = make_float4(T.start[i], T.step_inv[i], __int_as_float(T.size[i]), __int_as_float(T.offset[i]));
