@@ -97,11 +97,13 @@ __device__ __forceinline__ float sqrt_rn_nonneg(float x) {
 }
 
 // wrap_diff with the same operations on every axis (see DevBox::wrap_L): on a periodic
-// axis they are wrap_diff_scalar's; on an open one x - (+-0) * 3e38 = x, and x never
-// reaches 1.5e38.
+// axis they are wrap_diff_scalar's, with x - image * l as one fused multiply-add, which
+// is the same value whenever image * l is exact (every |image| <= 2, so every difference
+// of two positions within a box length of each other); on an open axis
+// x - (+-0) * 3e38 = x, and x never reaches 1.5e38.
 __device__ __forceinline__ float wrap_axis(float x, float l, float inv_l, float half_l) {
 	const float image = floorf(x * inv_l);
-	x -= image * l;
+	x = fmaf(-image, l, x);
 	return x >= half_l ? x - l : x;
 }
 __device__ __forceinline__ float3 wrap_diff_uniform(const DevBox& box, float3 d) {

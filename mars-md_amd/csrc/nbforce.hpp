@@ -97,10 +97,12 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		const float fm = beyond ? 0.0f : -dU * tp.y;
 		const float en = beyond ? U1 : dU * w + U0;
 		// the lower canonical end takes -F: -(fm u) = (-fm) u exactly
-		const float sfm = (in_cut && d > 1e-6f) ? (t_is_x ? -fm : fm) : 0.0f;
-		fx += sfm * (inv * r.x);
-		fy += sfm * (inv * r.y);
-		fz += sfm * (inv * r.z);
+		// (force over distance once, then per axis: the product rounds in another order
+		// than fm * (r / d), a relative change of about 1e-7)
+		const float g = ((in_cut && d > 1e-6f) ? (t_is_x ? -fm : fm) : 0.0f) * inv;
+		fx += g * r.x;
+		fy += g * r.y;
+		fz += g * r.z;
 		if (energy)
 			e += (in_cut && !(d < 1e-6f)) ? en * 0.5f : 0.0f;
 		return;
