@@ -85,12 +85,17 @@ __global__ __launch_bounds__(kBlock) void k_pmf_force(const float4* __restrict__
 	stage_types(types_arg, lds_types);
 	__syncthreads();
 	const marsmd_particle_types& types = lds_types;
-	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (t >= n)
-		return;
-	const float4 p = pos[t];
-	out[t] = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, S.has_grids ? grids : nullptr,
-									  S.ef, S.scheme);
+	// XCD-local tiles (a contiguous eighth of the Morton order per XCD): each XCD's L2
+	// then holds one region's grid lines, not lines from all over the grid
+	const TileRange R = xcd_tiles(uint32_t((n + kBlock - 1) / kBlock));
+	for (uint32_t tile = R.tile; tile < R.end; tile += R.stride) {
+		const uint64_t t = uint64_t(tile) * kBlock + threadIdx.x;
+		if (t >= n)
+			continue;
+		const float4 p = pos[t];
+		out[t] = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types,
+										  S.has_grids ? grids : nullptr, S.ef, S.scheme);
+	}
 }
 
 __global__ __launch_bounds__(kBlock) void k_integrate_pmf(const float4* __restrict__ pos, float4* __restrict__ force,
@@ -899,7 +904,8 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 			MMD_HIP(hipEventRecord(timing_events[timing_used], s));
 		}
 		if (pmf_in) {
-			k_pmf_force<<<g, kBlock, 0, s>>>(pos[cur].data(), n, types, grids_dev.data(), S, pmf_buf.data());
+			k_pmf_force<<<(g + 7u) / 8u * 8u, kBlock, 0, s>>>(pos[cur].data(), n, types, grids_dev.data(), S,
+															   pmf_buf.data());
 			MMD_LAUNCH_CHECK();
 		}
 		kern<<<grid, kBlock, lds_bytes(), s>>>(pos[cur].data(), pnext, mom.data(), force.data(), n, pl->list(),
