@@ -41,12 +41,14 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const float4* __restrict
 }
 
 // dL/dknot of one snapshot: per-workgroup FP64 sums in LDS (or global FP64 atomics when
-// the tables exceed the LDS budget), written to partial[blockIdx.x][knot].
+// the tables exceed the LDS budget), written to partial[blockIdx.x][knot] for the first
+// snapshot and added to it for the others (each workgroup owns its row, and the
+// snapshots run in stream order, so the sums are deterministic).
 template<bool LDS>
 __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restrict__ spos, NbrList L, uint64_t n,
 														 NbTables T, const float* __restrict__ knots_g, DevBox box,
 														 float cutoff2, const float4* __restrict__ grad,
-														 double* __restrict__ partial) {
+														 double* __restrict__ partial, int accumulate) {
 	extern __shared__ double lds_grad[];
 	__shared__ NbLds lds_nb;
 	stage_params(T, lds_nb);
@@ -104,8 +106,9 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restric
 	}
 	if (LDS) {
 		__syncthreads();
+		double* __restrict__ row = partial + uint64_t(blockIdx.x) * uint64_t(T.total_knots);
 		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
-			partial[uint64_t(blockIdx.x) * uint64_t(T.total_knots) + uint64_t(i)] = lds_grad[i];
+			row[i] = accumulate ? row[i] + lds_grad[i] : lds_grad[i];
 	}
 }
 
@@ -220,24 +223,28 @@ void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStr
 	partial.resize(uint64_t(blocks) * total);
 	k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(out, total);
 	MMD_LAUNCH_CHECK();
+	if (!lds) {
+		k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), total);
+		MMD_LAUNCH_CHECK();
+	}
+	// every snapshot's sums accumulate in the workgroup rows; one reduction at the end
 	for (uint64_t k = 0; k < S; ++k) {
 		Snap& sn = *snaps[k];
 		const NbrList L{sn.counts.data(), sn.tile_base.data(), sn.nbr.data()};
 		const float4* g = reinterpret_cast<const float4*>(grad4) + k * n;
 		if (lds) {
 			k_fm_adjoint<true><<<blocks, kFmBlock, total * 8u, s>>>(sn.spos.data(), L, n, *tables, knots,
-																	make_devbox(box), cutoff2, g, partial.data());
+																	make_devbox(box), cutoff2, g, partial.data(),
+																	k > 0 ? 1 : 0);
 		} else {
-			k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), total);
-			MMD_LAUNCH_CHECK();
 			k_fm_adjoint<false><<<grid_for(n, kFmBlock), kFmBlock, 0, s>>>(sn.spos.data(), L, n, *tables, knots,
 																		   make_devbox(box), cutoff2, g,
-																		   partial.data());
+																		   partial.data(), 1);
 		}
 		MMD_LAUNCH_CHECK();
-		k_fm_reduce<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), blocks, total, out);
-		MMD_LAUNCH_CHECK();
 	}
+	k_fm_reduce<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), blocks, total, out);
+	MMD_LAUNCH_CHECK();
 }
 
 } // namespace mmd
