@@ -26,29 +26,51 @@ namespace {
 constexpr int kFmBlock = 256;
 constexpr int kFmLdsGrad = 7936; // knots whose gradient a workgroup accumulates in LDS (62 KB)
 
-__global__ __launch_bounds__(kFmBlock) void k_fm_forces(const float4* __restrict__ spos,
-														const uint32_t* __restrict__ sorted_idx, NbrList L,
-														uint64_t n, NbTables T, const float* __restrict__ knots_g,
-														DevBox box, float cutoff2, float4* __restrict__ out) {
+}
+
+// One snapshot's device arrays (its sorted positions, sorted order and pairlist).
+struct FmSnapView {
+	const float4* spos;
+	const uint32_t* sorted_idx;
+	const uint32_t* counts;
+	const uint32_t* tile_base;
+	const uint32_t* nbr;
+};
+
+namespace {
+
+// The forces of every snapshot in one persistent launch: work item w is tile w % tiles
+// of snapshot w / tiles (snapshot-major, XCD-local chunks), the tables staged once per
+// workgroup.
+__global__ __launch_bounds__(kFmBlock) void k_fm_forces(const FmSnapView* __restrict__ snaps, uint64_t n,
+														uint32_t tiles, uint32_t items, NbTables T,
+														const float* __restrict__ knots_g, DevBox box, float cutoff2,
+														float4* __restrict__ out) {
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
 	stage_tables(T, knots_g, lds_knots, lds_nb);
-	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (t >= n)
-		return;
-	const float4 f = nb_force(uint32_t(t), spos[t], spos, L, T, lds_knots, knots_g, lds_nb, box, cutoff2, true);
-	out[sorted_idx[t]] = f;
+	const TileRange R = xcd_tiles(items);
+	for (uint32_t w = R.tile; w < R.end; w += R.stride) {
+		const uint32_t k = w / tiles;
+		const uint64_t t = uint64_t(w - k * tiles) * kFmBlock + threadIdx.x;
+		if (t >= n)
+			continue;
+		const FmSnapView V = snaps[k];
+		const NbrList L{V.counts, V.tile_base, V.nbr};
+		const float4 f = nb_force(uint32_t(t), V.spos[t], V.spos, L, T, lds_knots, knots_g, lds_nb, box, cutoff2, true);
+		out[uint64_t(k) * n + V.sorted_idx[t]] = f;
+	}
 }
 
-// dL/dknot of one snapshot: per-workgroup FP64 sums in LDS (or global FP64 atomics when
-// the tables exceed the LDS budget), written to partial[blockIdx.x][knot] for the first
-// snapshot and added to it for the others (each workgroup owns its row, and the
-// snapshots run in stream order, so the sums are deterministic).
+// dL/dknot of every snapshot in one persistent launch (work items as in k_fm_forces):
+// per-workgroup FP64 sums in LDS written to partial[blockIdx.x][knot] (each workgroup
+// visits a fixed sequence of work items, so the sums are deterministic), or global FP64
+// atomics into partial[0] when the tables exceed the LDS budget.
 template<bool LDS>
-__global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restrict__ spos, NbrList L, uint64_t n,
-														 NbTables T, const float* __restrict__ knots_g, DevBox box,
-														 float cutoff2, const float4* __restrict__ grad,
-														 double* __restrict__ partial, int accumulate) {
+__global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const FmSnapView* __restrict__ snaps, uint64_t n,
+														 uint32_t tiles, uint32_t items, NbTables T, DevBox box,
+														 float cutoff2, const float4* __restrict__ grad_all,
+														 double* __restrict__ partial) {
 	extern __shared__ double lds_grad[];
 	__shared__ NbLds lds_nb;
 	stage_params(T, lds_nb);
@@ -57,7 +79,16 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restric
 		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
 			lds_grad[i] = 0.0;
 	__syncthreads();
-	for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += uint64_t(gridDim.x) * blockDim.x) {
+	const TileRange R = xcd_tiles(items);
+	for (uint32_t w = R.tile; w < R.end; w += R.stride) {
+		const uint32_t snap = w / tiles;
+		const uint64_t t = uint64_t(w - snap * tiles) * kFmBlock + threadIdx.x;
+		if (t >= n)
+			continue;
+		const FmSnapView V = snaps[snap];
+		const float4* __restrict__ spos = V.spos;
+		const NbrList L{V.counts, V.tile_base, V.nbr};
+		const float4* __restrict__ grad = grad_all + uint64_t(snap) * n;
 		const float4 p = spos[t];
 		const uint32_t mt = __float_as_uint(p.w);
 		const uint32_t ct = meta_canon(mt);
@@ -108,20 +139,34 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const float4* __restric
 		__syncthreads();
 		double* __restrict__ row = partial + uint64_t(blockIdx.x) * uint64_t(T.total_knots);
 		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
-			row[i] = accumulate ? row[i] + lds_grad[i] : lds_grad[i];
+			row[i] = lds_grad[i];
 	}
 }
 
-// out[k] += sum over workgroups of partial[b][k], in workgroup order.
-__global__ void k_fm_reduce(const double* __restrict__ partial, uint32_t blocks, uint32_t total,
-							double* __restrict__ out) {
+// seg[y][k] = sum of partial[b][k] over the rows b of segment y (rows in order), then
+// out[k] = sum of seg[y][k] over the segments in order.
+constexpr uint32_t kFmSegs = 32;
+__global__ void k_fm_reduce_rows(const double* __restrict__ partial, uint32_t blocks, uint32_t total,
+								 double* __restrict__ seg) {
+	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= total)
+		return;
+	const uint32_t y = blockIdx.y;
+	const uint32_t per = (blocks + kFmSegs - 1) / kFmSegs;
+	const uint32_t b0 = y * per, b1 = min(blocks, b0 + per);
+	double s = 0.0;
+	for (uint32_t b = b0; b < b1; ++b)
+		s += partial[uint64_t(b) * total + k];
+	seg[uint64_t(y) * total + k] = s;
+}
+__global__ void k_fm_reduce_segs(const double* __restrict__ seg, uint32_t total, double* __restrict__ out) {
 	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= total)
 		return;
 	double s = 0.0;
-	for (uint32_t b = 0; b < blocks; ++b)
-		s += partial[uint64_t(b) * total + k];
-	out[k] += s;
+	for (uint32_t y = 0; y < kFmSegs; ++y)
+		s += seg[uint64_t(y) * total + k];
+	out[k] = s;
 }
 
 __global__ void k_fill_f64(double* p, uint64_t n) {
@@ -148,6 +193,8 @@ FMImpl::FMImpl(const marsmd_fm_desc& d) : tables(new NbTables{}) {
 		raise(MARSMD_ERR_VALUE, "force matching: particle count must be below 2^24");
 	n = d.num_particles;
 	S = d.num_snapshots;
+	if ((n + kFmBlock - 1) / kFmBlock * S >= (uint64_t(1) << 32))
+		raise(MARSMD_ERR_VALUE, "force matching: snapshots x particles / 256 must be below 2^32");
 	box = d.box;
 	cutoff2 = d.cutoff > 0.0f ? d.cutoff * d.cutoff : 0.0f;
 	NbTables& T = *tables;
@@ -195,6 +242,12 @@ FMImpl::FMImpl(const marsmd_fm_desc& d) : tables(new NbTables{}) {
 			MMD_HIP(hipMemcpyAsync(sn->nbr.data(), pl.nbr.data(), 4 * pl.nbr_storage, hipMemcpyDeviceToDevice, nullptr));
 		num_pairs += pl.num_pairs();
 	}
+	std::vector<FmSnapView> views(S);
+	for (uint64_t k = 0; k < S; ++k)
+		views[k] = FmSnapView{snaps[k]->spos.data(), snaps[k]->sorted_idx.data(), snaps[k]->counts.data(),
+							  snaps[k]->tile_base.data(), snaps[k]->nbr.data()};
+	snap_views.resize(S);
+	MMD_HIP(hipMemcpy(snap_views.data(), views.data(), sizeof(FmSnapView) * S, hipMemcpyHostToDevice));
 	MMD_HIP(hipDeviceSynchronize());
 }
 
@@ -204,46 +257,56 @@ FMImpl::~FMImpl() {
 	delete tables;
 }
 
+// Persistent workgroups for a kernel over the S x tiles work items: as many as are
+// resident (occupancy x CUs), a multiple of 8, at most the items rounded up to 8.
+static unsigned fm_grid(const void* kernel, unsigned lds, uint64_t items) {
+	int occ = 0, dev = 0, cus = 0;
+	MMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kFmBlock, lds));
+	MMD_HIP(hipGetDevice(&dev));
+	MMD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+	uint64_t g = uint64_t(std::max(occ, 1)) * uint64_t(std::max(cus, 1));
+	g = std::min<uint64_t>(g, (items + 7) / 8 * 8);
+	return unsigned(std::max<uint64_t>((g + 7) / 8 * 8, 8));
+}
+
 void FMImpl::forces(const float* knots, float* out4, hipStream_t s) {
 	const unsigned lds = tables->lds ? unsigned(std::max(tables->total_knots, 2)) * 4u : 0u;
-	for (uint64_t k = 0; k < S; ++k) {
-		Snap& sn = *snaps[k];
-		const NbrList L{sn.counts.data(), sn.tile_base.data(), sn.nbr.data()};
-		k_fm_forces<<<grid_for(n, kFmBlock), kFmBlock, lds, s>>>(sn.spos.data(), sn.sorted_idx.data(), L, n, *tables,
-																 knots, make_devbox(box), cutoff2,
-																 reinterpret_cast<float4*>(out4) + k * n);
-		MMD_LAUNCH_CHECK();
-	}
+	const uint32_t tiles = uint32_t((n + kFmBlock - 1) / kFmBlock);
+	const uint64_t items = uint64_t(tiles) * S;
+	const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_forces), lds, items);
+	k_fm_forces<<<grid, kFmBlock, lds, s>>>(snap_views.data(), n, tiles, uint32_t(items), *tables, knots,
+											make_devbox(box), cutoff2, reinterpret_cast<float4*>(out4));
+	MMD_LAUNCH_CHECK();
 }
 
 void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStream_t s) {
+	(void)knots; // the gradient of a linear table force does not depend on the knots
 	const uint32_t total = uint32_t(tables->total_knots);
 	const bool lds = total <= uint32_t(kFmLdsGrad);
-	const unsigned blocks = lds ? unsigned(std::min<uint64_t>(grid_for(n, kFmBlock), 512)) : 1u;
-	partial.resize(uint64_t(blocks) * total);
-	k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(out, total);
-	MMD_LAUNCH_CHECK();
-	if (!lds) {
+	const uint32_t tiles = uint32_t((n + kFmBlock - 1) / kFmBlock);
+	const uint64_t items = uint64_t(tiles) * S;
+	const float4* g = reinterpret_cast<const float4*>(grad4);
+	uint32_t rows = 1;
+	if (lds) {
+		const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_adjoint<true>), total * 8u, items);
+		rows = grid;
+		partial.resize(uint64_t(rows) * total);
+		k_fm_adjoint<true><<<grid, kFmBlock, total * 8u, s>>>(snap_views.data(), n, tiles, uint32_t(items), *tables,
+															 make_devbox(box), cutoff2, g, partial.data());
+	} else {
+		const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_adjoint<false>), 0u, items);
+		partial.resize(total);
 		k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), total);
 		MMD_LAUNCH_CHECK();
+		k_fm_adjoint<false><<<grid, kFmBlock, 0, s>>>(snap_views.data(), n, tiles, uint32_t(items), *tables,
+													  make_devbox(box), cutoff2, g, partial.data());
 	}
-	// every snapshot's sums accumulate in the workgroup rows; one reduction at the end
-	for (uint64_t k = 0; k < S; ++k) {
-		Snap& sn = *snaps[k];
-		const NbrList L{sn.counts.data(), sn.tile_base.data(), sn.nbr.data()};
-		const float4* g = reinterpret_cast<const float4*>(grad4) + k * n;
-		if (lds) {
-			k_fm_adjoint<true><<<blocks, kFmBlock, total * 8u, s>>>(sn.spos.data(), L, n, *tables, knots,
-																	make_devbox(box), cutoff2, g, partial.data(),
-																	k > 0 ? 1 : 0);
-		} else {
-			k_fm_adjoint<false><<<grid_for(n, kFmBlock), kFmBlock, 0, s>>>(sn.spos.data(), L, n, *tables, knots,
-																		   make_devbox(box), cutoff2, g,
-																		   partial.data(), 1);
-		}
-		MMD_LAUNCH_CHECK();
-	}
-	k_fm_reduce<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), blocks, total, out);
+	MMD_LAUNCH_CHECK();
+	seg_sums.resize(uint64_t(kFmSegs) * total);
+	k_fm_reduce_rows<<<dim3(grid_for(total, 256), kFmSegs), 256, 0, s>>>(partial.data(), rows, total,
+																		 seg_sums.data());
+	MMD_LAUNCH_CHECK();
+	k_fm_reduce_segs<<<grid_for(total, 256), 256, 0, s>>>(seg_sums.data(), total, out);
 	MMD_LAUNCH_CHECK();
 }
 

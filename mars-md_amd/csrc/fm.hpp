@@ -8,6 +8,7 @@
 namespace mmd {
 
 struct NbTables;
+struct FmSnapView;
 
 struct FMImpl {
 	struct Snap;
@@ -16,7 +17,8 @@ struct FMImpl {
 	float cutoff2 = 0.0f;
 	NbTables* tables = nullptr;
 	std::vector<Snap*> snaps;
-	DevArray<double> partial;
+	DevArray<double> partial, seg_sums;
+	DevArray<FmSnapView> snap_views; // per snapshot: its device arrays
 
 	explicit FMImpl(const marsmd_fm_desc& d);
 	~FMImpl();
