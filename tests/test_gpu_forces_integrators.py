@@ -296,6 +296,37 @@ def test_patch_steps_config2_langevin_grid(cuda):
         _check_step(cuda, patch, spec, step)
 
 
+def _open_y_spec():
+    """The config-0 system in a box open along y (periodic in x and z)."""
+    spec = configs.config0(n=4000, L=170.0, seed=13)
+    spec.box = PeriodicBox(spec.box.size, periodic=(True, False, True))
+    return spec
+
+
+def test_patch_steps_open_axis(cuda):
+    # the pair term's wrap runs the same operations on an open axis (a 3e38 length with
+    # a zero inverse): stage by stage against the oracle, rebuilds at 1 and 21
+    spec = _open_y_spec()
+    patch = marsmd.Patch(spec)
+    for step in range(1, 24):
+        _check_step(cuda, patch, spec, step)
+
+
+def test_fused_run_open_axis(cuda):
+    spec = _open_y_spec()
+    fused = marsmd.Patch(spec)
+    unfused = marsmd.Patch(spec)
+    steps = 25
+    fused.run(steps)
+    for s in range(1, steps + 1):
+        unfused.step_unfused(s)
+    cuda.cuda.synchronize()
+    a = fused.copy_particles_to_host()
+    b = unfused.copy_particles_to_host()
+    assert np.array_equal(a["id"], b["id"])
+    assert np.abs(a["pos"][:, :3] - b["pos"][:, :3]).max() < 1e-3
+
+
 @pytest.mark.parametrize("which", ["config0", "config1", "config2"])
 def test_fused_run_matches_unfused_and_oracle(cuda, which):
     if which == "config0":
