@@ -63,9 +63,10 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const FmSnapView* __rest
 }
 
 // dL/dknot of every snapshot in one persistent launch (work items as in k_fm_forces):
-// per-workgroup FP64 sums in LDS written to partial[blockIdx.x][knot] (each workgroup
-// visits a fixed sequence of work items, so the sums are deterministic), or global FP64
-// atomics into partial[0] when the tables exceed the LDS budget.
+// per-workgroup FP64 sums in LDS written to partial[blockIdx.x][knot], or global FP64
+// atomics into partial[0] when the tables exceed the LDS budget. Each workgroup visits
+// a fixed sequence of work items; the order of the atomic additions within it is not
+// fixed, which FP64 keeps at the level of its rounding (about 1e-16 relative).
 template<bool LDS>
 __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const FmSnapView* __restrict__ snaps, uint64_t n,
 														 uint32_t tiles, uint32_t items, NbTables T, DevBox box,
@@ -94,42 +95,48 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const FmSnapView* __res
 		const uint32_t ct = meta_canon(mt);
 		const uint32_t tt = meta_type(mt);
 		const float4 gx = grad[ct];
-		const uint32_t cnt = L.count[t];
-		const uint64_t base = L.tile_base[t >> 6];
-		for (uint32_t k = 0; k < cnt; ++k) {
-			const uint32_t j = L.nbr[nbr_slot_index(base, uint32_t(t), k)];
-			const float4 q = spos[j];
-			const uint32_t mj = __float_as_uint(q.w);
-			const uint32_t cj = meta_canon(mj);
-			if (!(ct < cj))
-				continue; // the pair is taken from its lower canonical end
-			const uint32_t tag = lds_nb.tag[(tt << 4) | meta_type(mj)];
-			if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
-				continue;
-			const float3 r = wrap_diff(box, sub3(q, p));
-			const float d = sqrtf(len2(r));
-			if (d < 1e-6f)
-				continue;
-			if (cutoff2 > 0.0f && d * d > cutoff2)
-				continue;
-			const float inv = 1.0f / d;
-			const float ux = inv * r.x, uy = inv * r.y, uz = inv * r.z;
-			const float4 tp = lds_nb.tab[tag >> MARSMD_PAIR_TABLE_SHIFT]; // {start, step_inv, size, offset}
-			const float w = (d - tp.x) * tp.y;
-			int h = int(floorf(w));
-			if (h >= __float_as_int(tp.z) - 1)
-				continue;
-			if (h < 0)
-				h = 0;
-			const float4 gy = grad[cj];
-			const double dfm = (double(gy.x) - double(gx.x)) * double(ux) + (double(gy.y) - double(gx.y)) * double(uy) +
-							   (double(gy.z) - double(gx.z)) * double(uz);
-			const double g = dfm * double(tp.y);
-			const int k0 = __float_as_int(tp.w) + h;
-			if (LDS) {
-				atomicAdd(&acc[k0], g);
-				atomicAdd(&acc[k0 + 1], -g);
-			} else {
+		// the row in groups of 4 entries (one 16-byte index load, 4 independent position
+		// gathers); the pads of the last group are t itself, skipped as ct == cj
+		const uint32_t ng = (L.count[t] + 3u) >> 2;
+		const uint4* __restrict__ row = reinterpret_cast<const uint4*>(L.nbr + L.tile_base[t >> 6]) + (t & 63u);
+		for (uint32_t gi = 0; gi < ng; ++gi) {
+			const uint4 jj = row[uint64_t(gi) * 64u];
+			const uint32_t js[4] = {jj.x, jj.y, jj.z, jj.w};
+			float4 qs[4];
+#pragma unroll
+			for (int e = 0; e < 4; ++e)
+				qs[e] = spos[js[e]];
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				const float4 q = qs[e];
+				const uint32_t mj = __float_as_uint(q.w);
+				const uint32_t cj = meta_canon(mj);
+				if (!(ct < cj))
+					continue; // the pair is taken from its lower canonical end
+				const uint32_t tag = lds_nb.tag[(tt << 4) | meta_type(mj)];
+				if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
+					continue;
+				const float3 r = wrap_diff(box, sub3(q, p));
+				const float d = sqrtf(len2(r));
+				if (d < 1e-6f)
+					continue;
+				if (cutoff2 > 0.0f && d * d > cutoff2)
+					continue;
+				const float inv = 1.0f / d;
+				const float ux = inv * r.x, uy = inv * r.y, uz = inv * r.z;
+				const float4 tp = lds_nb.tab[tag >> MARSMD_PAIR_TABLE_SHIFT]; // {start, step_inv, size, offset}
+				const float w = (d - tp.x) * tp.y;
+				int h = int(floorf(w));
+				if (h >= __float_as_int(tp.z) - 1)
+					continue;
+				if (h < 0)
+					h = 0;
+				const float4 gy = grad[cj];
+				const double dfm = (double(gy.x) - double(gx.x)) * double(ux) +
+								   (double(gy.y) - double(gx.y)) * double(uy) +
+								   (double(gy.z) - double(gx.z)) * double(uz);
+				const double g = dfm * double(tp.y);
+				const int k0 = __float_as_int(tp.w) + h;
 				atomicAdd(&acc[k0], g);
 				atomicAdd(&acc[k0 + 1], -g);
 			}
