@@ -88,9 +88,7 @@ class ForceMatching:
         k = knots.detach().to(device="cuda", dtype=torch.float32).contiguous()
         g = grad_forces.detach().to(device="cuda", dtype=torch.float32)
         if g.shape[-1] == 3:
-            g4 = torch.zeros((self.S, self.n, 4), dtype=torch.float32, device="cuda")
-            g4[..., :3] = g
-            g = g4
+            g = torch.nn.functional.pad(g, (0, 1))
         g = g.contiguous()
         out = torch.empty(self.total_knots, dtype=torch.float64, device="cuda")
         api.call("marsmd_fm_adjoint", self._h, k.data_ptr(), g.data_ptr(), out.data_ptr(), _stream(stream))
@@ -129,7 +127,34 @@ def tabulated_pair_forces(knots, fm: ForceMatching):
     return _FN.apply(knots, fm)
 
 
+_MSE = None
+
+
+def _mse_function():
+    torch = _torch()
+
+    class ForceMSE(torch.autograd.Function):
+        """mean((f - target)^2): the differences in FP32 (one pass), their squares summed
+        in FP64 (one reduction); the gradient 2 (f - target) / count in one pass."""
+
+        @staticmethod
+        def forward(ctx, f, target):
+            d = f - target
+            ctx.save_for_backward(d)
+            return torch.linalg.vector_norm(d, dtype=torch.float64).square() / d.numel()
+
+        @staticmethod
+        def backward(ctx, g):
+            (d,) = ctx.saved_tensors
+            return d * (2.0 * g / d.numel()).to(d.dtype), None
+
+    return ForceMSE
+
+
 def force_matching_loss(knots, fm: ForceMatching, target):
-    """Mean squared force error over snapshots, particles and components."""
+    """Mean squared force error over snapshots, particles and components (FP64 sum)."""
+    global _MSE
+    if _MSE is None:
+        _MSE = _mse_function()
     f = tabulated_pair_forces(knots, fm)
-    return ((f.double() - target.double()) ** 2).mean()
+    return _MSE.apply(f, target)

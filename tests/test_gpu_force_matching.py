@@ -51,9 +51,11 @@ def test_autograd_is_the_adjoint_and_adam_reduces_the_loss(cuda):
     knots = fm.knots().clone().requires_grad_(True)
     loss = force_matching_loss(knots, fm, target)
     loss.backward()
+    # dL/dF as the loss defines it: the FP32 differences times 2 / count
     f = tabulated_pair_forces(knots.detach(), fm).detach()
-    G = 2.0 * (f.double() - target.double()) / f.numel()
-    g_adj = fm.adjoint(knots.detach(), G.float())
+    d = f - target
+    G = d * torch.tensor(2.0 / d.numel(), dtype=torch.float64).to(d.dtype)
+    g_adj = fm.adjoint(knots.detach(), G)
     assert torch.allclose(knots.grad.double(), g_adj.to(knots.grad.device), rtol=1e-6, atol=1e-12)
     # central differences of the GPU loss on a few used knots (FP32 forces: loose bound)
     used = torch.nonzero(knots.grad).flatten().cpu().numpy()
