@@ -352,6 +352,12 @@ marsmd_status marsmd_launch_pmf(const marsmd_particle_view* view, const marsmd_p
 marsmd_status marsmd_philox_draw(uint64_t seed, uint32_t base_ctr, uint32_t global_seed,
                                  uint32_t ctr1, uint32_t draws, uint64_t n, uint32_t* out,
                                  marsmd_stream stream);
+/* No counterpart in the reference: a self-test of the square root the step kernels use
+ * for pair distances (the hardware square root corrected to correct rounding). Compares
+ * it with the correctly rounded sqrtf on every float whose bit pattern lies in
+ * [lo_bits, hi_bits) and writes the number of differences to *mismatches (host). */
+marsmd_status marsmd_selftest_pair_sqrt(uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
+                                        marsmd_stream stream);
 
 /* ------------------------------------------------------------------ Patch
  * Replaces class Patch (src/PatchOperation/Patch.h:48-645, Patch.cpp:43-480) for one

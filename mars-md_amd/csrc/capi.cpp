@@ -362,6 +362,14 @@ marsmd_status marsmd_philox_draw(uint64_t seed, uint32_t base_ctr, uint32_t glob
 	return guard([&] { mmd::philox_draw(seed, base_ctr, global_seed, ctr1, draws, n, out, st(stream)); });
 }
 
+marsmd_status marsmd_selftest_pair_sqrt(uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
+										marsmd_stream stream) {
+	return guard([&] {
+		need(mismatches, "mismatches");
+		*mismatches = mmd::selftest_pair_sqrt(lo_bits, hi_bits, st(stream));
+	});
+}
+
 // ---------------------------------------------------------------- Patch
 marsmd_status marsmd_patch_create(const marsmd_patch_desc* desc, marsmd_patch** out) {
 	return guard([&] {

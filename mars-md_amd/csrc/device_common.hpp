@@ -81,9 +81,9 @@ __device__ __forceinline__ float3 wrap_diff(const DevBox& box, float3 d) {
 // one ulp) corrected by the signs of the residuals x - s'.s of its two neighbours s', as
 // the compiler's lowering of sqrtf does, without that lowering's rescaling of inputs
 // below 2^-96 and its zero / infinity select (0 comes out as 0). Equal to sqrtf on every
-// float from 2^-96 to 2^24 and on 0 (checked on the GPU, bit for bit). Below 2^-96 the
-// result may differ from sqrtf; callers use it where such distances (< 4e-15) are
-// skipped whatever their value.
+// float from 2^-96 to 2^24 and on 0, bit for bit (marsmd_selftest_pair_sqrt compares the
+// two over any range of bit patterns). Below 2^-96 the result may differ from sqrtf;
+// callers use it where such distances (< 4e-15) are skipped whatever their value.
 __device__ __forceinline__ float sqrt_rn_nonneg(float x) {
 	const float s = __builtin_amdgcn_sqrtf(x);
 	const float sm = __uint_as_float(__float_as_uint(s) - 1u);

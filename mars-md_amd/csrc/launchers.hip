@@ -514,6 +514,32 @@ void launch_pmf(const marsmd_particle_view& v, const marsmd_particle_types& t, u
 	MMD_LAUNCH_CHECK();
 }
 
+namespace {
+// sqrt_rn_nonneg against sqrtf on the bit patterns lo + i, i < count.
+__global__ void k_selftest_sqrt(uint32_t lo, uint32_t count, unsigned long long* __restrict__ bad) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= count)
+		return;
+	const float x = __uint_as_float(lo + i);
+	if (__float_as_uint(sqrt_rn_nonneg(x)) != __float_as_uint(sqrtf(x)))
+		atomicAdd(bad, 1ull);
+}
+} // namespace
+
+uint64_t selftest_pair_sqrt(uint32_t lo_bits, uint32_t hi_bits, hipStream_t s) {
+	if (hi_bits <= lo_bits)
+		return 0;
+	DevArray<unsigned long long> bad(1);
+	fill_u32(reinterpret_cast<uint32_t*>(bad.data()), 0u, 2, s);
+	const uint32_t count = hi_bits - lo_bits;
+	k_selftest_sqrt<<<(count + 255u) / 256u, 256, 0, s>>>(lo_bits, count, bad.data());
+	MMD_LAUNCH_CHECK();
+	unsigned long long host = 0;
+	MMD_HIP(hipMemcpyAsync(&host, bad.data(), sizeof(host), hipMemcpyDeviceToHost, s));
+	MMD_HIP(hipStreamSynchronize(s));
+	return uint64_t(host);
+}
+
 void philox_draw(uint64_t seed, uint32_t base_ctr, uint32_t gs, uint32_t c1, uint32_t draws, uint64_t n,
 				 uint32_t* out, hipStream_t s) {
 	if (n == 0)

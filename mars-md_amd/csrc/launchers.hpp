@@ -35,6 +35,7 @@ void launch_baoab(const marsmd_particle_view& v, const marsmd_particle_types& t,
 void launch_baoab_last_update(const marsmd_particle_view& v, float dt, uint64_t n, hipStream_t s);
 void launch_pmf(const marsmd_particle_view& v, const marsmd_particle_types& t, uint64_t n,
 				const marsmd_grid_view* grids, const float ef[3], int scheme, hipStream_t s);
+uint64_t selftest_pair_sqrt(uint32_t lo_bits, uint32_t hi_bits, hipStream_t s);
 void philox_draw(uint64_t seed, uint32_t base_ctr, uint32_t gs, uint32_t c1, uint32_t draws, uint64_t n,
 				 uint32_t* out, hipStream_t s);
 

@@ -272,6 +272,7 @@ PROTOTYPES = {
     "marsmd_launch_baoab_last_update": (_I32, [C.POINTER(ParticleView), C.POINTER(ParticleTypes), _F, _U64, _P]),
     "marsmd_launch_pmf": (_I32, [C.POINTER(ParticleView), C.POINTER(ParticleTypes), _U64, _P, _FP3, _I32, _P]),
     "marsmd_philox_draw": (_I32, [_U64, _U32, _U32, _U32, _U32, _U64, _P, _P]),
+    "marsmd_selftest_pair_sqrt": (_I32, [_U32, _U32, _P, _P]),
     "marsmd_patch_create": (_I32, [C.POINTER(PatchDesc), _PP]),
     "marsmd_patch_destroy": (_I32, [_P]),
     "marsmd_patch_set_base_seed": (_I32, [_P, _U64]),
