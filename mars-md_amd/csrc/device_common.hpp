@@ -453,9 +453,10 @@ __device__ __forceinline__ float dot4(const float w[4], float v0, float v1, floa
 }
 
 // sample_grid_cubic (BaseGridDevice.h:405-475): the x pass runs over the 4 x 4 (y, z)
-// tap lines, then y, then z, as the reference does. Each tap axis is mapped once and
-// the four z taps of a (x, y) line are one 16-byte load when they do not cross a
-// grid face.
+// tap lines, then y, then z, as the reference does. A stencil inside the grid reads its
+// 16 runs of 4 z taps at fixed offsets from its first tap; otherwise each tap axis is
+// mapped once (boundary condition) and the four z taps of a (x, y) line are one 16-byte
+// load when they do not cross a grid face.
 __device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, float py, float pz, int bc) {
 	const float v[3] = {px - g.origin[0], py - g.origin[1], pz - g.origin[2]};
 	float gp[3];
@@ -465,29 +466,47 @@ __device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, f
 	const CrWeights Wx = cr_weights(wx), Wy = cr_weights(wy), Wz = cr_weights(wz);
 	const int nx = int(g.dims[0]), ny = int(g.dims[1]), nz = int(g.dims[2]);
 	const int64_t sx = int64_t(ny) * int64_t(nz);
-	int64_t ox[4];
+	float rr[4][4][4]; // [y][x][z] taps
+	if (hx >= 1 && hx + 2 < nx && hy >= 1 && hy + 2 < ny && hz >= 1 && hz + 2 < nz) {
+		// every tap inside the grid: the 16 runs at offsets (x sx + y nz) from the first
+		gfloat* __restrict__ base = grid_data(g) + ((int64_t(hx - 1) * ny + (hy - 1)) * nz + (hz - 1));
 #pragma unroll
-	for (int i = 0; i < 4; ++i)
-		ox[i] = tap_offset(hx - 1 + i, nx, bc, sx);
+		for (int iy = 0; iy < 4; ++iy) {
+#pragma unroll
+			for (int ix = 0; ix < 4; ++ix) {
+				const GridRun4 r = *(MMD_GLOBAL_AS const GridRun4*)(base + (ix * sx + iy * nz));
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					rr[iy][ix][q] = r.v[q];
+			}
+		}
+	} else {
+		int64_t ox[4];
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+			ox[i] = tap_offset(hx - 1 + i, nx, bc, sx);
+#pragma unroll
+		for (int iy = 0; iy < 4; ++iy) {
+			const int64_t oy = tap_offset(hy - 1 + iy, ny, bc, int64_t(nz));
+#pragma unroll
+			for (int ix = 0; ix < 4; ++ix) {
+				if (ox[ix] < 0 || oy < 0) {
+#pragma unroll
+					for (int q = 0; q < 4; ++q)
+						rr[iy][ix][q] = 0.0f;
+				} else {
+					load_run<4>(grid_data(g) + ox[ix] + oy, hz - 1, nz, bc, rr[iy][ix]);
+				}
+			}
+		}
+	}
 	float d1[4][4], v1[4][4];
 #pragma unroll
 	for (int iy = 0; iy < 4; ++iy) {
-		const int64_t oy = tap_offset(hy - 1 + iy, ny, bc, int64_t(nz));
-		float r[4][4]; // [x][z]
-#pragma unroll
-		for (int ix = 0; ix < 4; ++ix) {
-			if (ox[ix] < 0 || oy < 0) {
-#pragma unroll
-				for (int q = 0; q < 4; ++q)
-					r[ix][q] = 0.0f;
-			} else {
-				load_run<4>(grid_data(g) + ox[ix] + oy, hz - 1, nz, bc, r[ix]);
-			}
-		}
 #pragma unroll
 		for (int iz = 0; iz < 4; ++iz) {
-			d1[iy][iz] = dot4(Wx.d, r[0][iz], r[1][iz], r[2][iz], r[3][iz]);
-			v1[iy][iz] = dot4(Wx.w, r[0][iz], r[1][iz], r[2][iz], r[3][iz]);
+			d1[iy][iz] = dot4(Wx.d, rr[iy][0][iz], rr[iy][1][iz], rr[iy][2][iz], rr[iy][3][iz]);
+			v1[iy][iz] = dot4(Wx.w, rr[iy][0][iz], rr[iy][1][iz], rr[iy][2][iz], rr[iy][3][iz]);
 		}
 	}
 	float d2x[4], d2y[4], v2[4];
