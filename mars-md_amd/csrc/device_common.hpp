@@ -457,6 +457,9 @@ __device__ __forceinline__ float dot4(const float w[4], float v0, float v1, floa
 // 16 runs of 4 z taps at fixed offsets from its first tap; otherwise each tap axis is
 // mapped once (boundary condition) and the four z taps of a (x, y) line are one 16-byte
 // load when they do not cross a grid face.
+// INTERIOR: read a stencil that lies inside the grid at fixed offsets (fewer address
+// operations, more registers; for kernels that have registers to spare).
+template<bool INTERIOR = false>
 __device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, float py, float pz, int bc) {
 	const float v[3] = {px - g.origin[0], py - g.origin[1], pz - g.origin[2]};
 	float gp[3];
@@ -467,7 +470,7 @@ __device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, f
 	const int nx = int(g.dims[0]), ny = int(g.dims[1]), nz = int(g.dims[2]);
 	const int64_t sx = int64_t(ny) * int64_t(nz);
 	float rr[4][4][4]; // [y][x][z] taps
-	if (hx >= 1 && hx + 2 < nx && hy >= 1 && hy + 2 < ny && hz >= 1 && hz + 2 < nz) {
+	if (INTERIOR && hx >= 1 && hx + 2 < nx && hy >= 1 && hy + 2 < ny && hz >= 1 && hz + 2 < nz) {
 		// every tap inside the grid: the 16 runs at offsets (x sx + y nz) from the first
 		gfloat* __restrict__ base = grid_data(g) + ((int64_t(hx - 1) * ny + (hy - 1)) * nz + (hz - 1));
 #pragma unroll
@@ -544,7 +547,7 @@ __device__ __forceinline__ void for_each_grid(int gid, const marsmd_grid_view* _
 // compute_position_dependent_force (src/Interactions/Nonbonded/Pmf.h:68-124) for one
 // interpolation scheme. The returned .w is the PMF energy (the reference's .t). Each
 // lane adds its type's terms in term order, as the reference does.
-template<bool CUBIC>
+template<bool CUBIC, bool INTERIOR = false>
 __device__ inline float4 position_dependent_force_t(float px, float py, float pz, int type,
 													const marsmd_particle_types& types,
 													const marsmd_grid_view* __restrict__ grids, const float ef[3]) {
@@ -566,7 +569,7 @@ __device__ inline float4 position_dependent_force_t(float px, float py, float pz
 			if (g.data == nullptr)
 				return;
 			const int bc = term.boundary_condition >= 0 ? term.boundary_condition : g.boundary_condition;
-			const GridSample s = CUBIC ? sample_cubic(g, px, py, pz, bc) : sample_linear(g, px, py, pz, bc);
+			const GridSample s = CUBIC ? sample_cubic<INTERIOR>(g, px, py, pz, bc) : sample_linear(g, px, py, pz, bc);
 			const float ns = -term.scale;
 			f.x += s.grad[0] * ns;
 			f.y += s.grad[1] * ns;
@@ -579,7 +582,7 @@ __device__ inline float4 position_dependent_force_t(float px, float py, float pz
 		for_each_grid(types.force_grid_id[type][a], grids, [&](const marsmd_grid_view& g) {
 			if (g.data == nullptr)
 				return;
-			const float v = CUBIC ? sample_cubic(g, px, py, pz, g.boundary_condition).value
+			const float v = CUBIC ? sample_cubic<INTERIOR>(g, px, py, pz, g.boundary_condition).value
 								  : sample_linear(g, px, py, pz, g.boundary_condition).value;
 			const float add = types.force_grid_scale[type][a] * v;
 			if (a == 0)
@@ -593,12 +596,14 @@ __device__ inline float4 position_dependent_force_t(float px, float py, float pz
 	return f;
 }
 
+template<bool INTERIOR = false>
 __device__ inline float4 position_dependent_force(float px, float py, float pz, int type,
 												  const marsmd_particle_types& types,
 												  const marsmd_grid_view* __restrict__ grids, const float ef[3],
 												  int scheme) {
-	return scheme == MARSMD_INTERP_LINEAR ? position_dependent_force_t<false>(px, py, pz, type, types, grids, ef)
-										  : position_dependent_force_t<true>(px, py, pz, type, types, grids, ef);
+	return scheme == MARSMD_INTERP_LINEAR
+			   ? position_dependent_force_t<false, INTERIOR>(px, py, pz, type, types, grids, ef)
+			   : position_dependent_force_t<true, INTERIOR>(px, py, pz, type, types, grids, ef);
 }
 
 __device__ __forceinline__ float atomic_add_f32(float* p, float v) {
