@@ -102,7 +102,10 @@ struct PairlistImpl {
 	DevArray<uint32_t> ref_begin, ref_end, ref_neighbors;
 
 	PairlistImpl(uint64_t max_particles_, uint64_t max_pairs_ceiling_);
-	void set_box(const marsmd_box& b) { box = b; }
+	void set_box(const marsmd_box& b) {
+		box = b;
+		presorted_next = false; // the keys of a new box are not the reorder's
+	}
 	// Build from positions in slot order. slot_of_canon maps canonical index -> slot
 	// (NULL = identity); canon_from_meta takes canonical indices (and types) from
 	// pos[s].w, else from the slot index and `types`.
