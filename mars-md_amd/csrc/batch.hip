@@ -251,7 +251,11 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restr
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
 	__shared__ marsmd_particle_types lds_types;
+	__shared__ TypeStep lds_steps[MARSMD_MAX_TYPES]; // the integrator's per-type constants
 	stage_types(types_arg, lds_types);
+	if (threadIdx.x < uint32_t(types_arg.num_types))
+		lds_steps[threadIdx.x] = INTEG == MARSMD_INTEGRATOR_BROWNIAN ? bd_type_step(types_arg, int(threadIdx.x), S)
+																	 : baoab_type_step(types_arg, int(threadIdx.x), S);
 	stage_tables(T, knots_g, lds_knots, lds_nb);
 	const marsmd_particle_types& types = lds_types;
 	// a replayed segment carries the step's offset in S.step and its first step in memory
@@ -272,10 +276,10 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restr
 		if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
 			const float4 pmf = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, nullptr, P.ef,
 														P.scheme);
-			pn = bd_update(p, f, types, pmf, P);
+			pn = bd_update(p, f, lds_steps[meta_type(fb(p.w))], pmf, P);
 		} else {
 			float4 mo = mom[t];
-			pn = baoab_update(p, mo, f, types, P);
+			pn = baoab_update(p, mo, f, lds_steps[meta_type(fb(p.w))], P);
 			mom[t] = mo;
 		}
 		pos_next[t] = pn;

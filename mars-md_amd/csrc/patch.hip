@@ -155,7 +155,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
 	__shared__ marsmd_particle_types lds_types;
+	__shared__ TypeStep lds_steps[MARSMD_MAX_TYPES]; // the integrator's per-type constants
 	stage_types(types_arg, lds_types);
+	if (threadIdx.x < uint32_t(types_arg.num_types))
+		lds_steps[threadIdx.x] = INTEG == MARSMD_INTEGRATOR_BROWNIAN ? bd_type_step(types_arg, int(threadIdx.x), S)
+																	 : baoab_type_step(types_arg, int(threadIdx.x), S);
 	stage_tables(T, knots_g, lds_knots, lds_nb);
 	const marsmd_particle_types& types = lds_types;
 	const TileRange R = xcd_tiles(uint32_t((n + kBlock - 1) / kBlock));
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 			const float4 pmf = PMF_IN ? pmf_in[t]
 									  : position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types,
 																 S.has_grids ? grids : nullptr, S.ef, S.scheme);
-			pn = bd_update(p, f, types, pmf, S);
+			pn = bd_update(p, f, lds_steps[meta_type(fb(p.w))], pmf, S);
 		} else {
 			if (S.has_grids) {
 				const float4 g = PMF_IN ? pmf_in[t]
@@ -189,7 +193,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 				f.w += g.w;
 			}
 			float4 mo = mom[t];
-			pn = baoab_update(p, mo, f, types, S);
+			pn = baoab_update(p, mo, f, lds_steps[meta_type(fb(p.w))], S);
 			mom[t] = mo;
 		}
 		pos_next[t] = pn;
