@@ -308,6 +308,18 @@ def main():
         rec = json.loads(pmc.read_text())
         if rec.get("workload") == spec.name:
             traffic, traffic_src = rec["hbm_bytes_per_launch"], str(pmc.relative_to(ROOT))
+    # the step kernel is issue-bound on VALU work rather than on HBM: quote the measured
+    # VALU share of its issue cycles (rocprofv3 SQ counters) for the launch shape kept
+    valu = None
+    sq = ROOT / "profiles" / "k_step_langevin_1M_grid256_sq.json"
+    if sq.exists() and not choice["split_pmf"]:
+        rec = json.loads(sq.read_text())
+        # the persistent step kernel's threads: workgroups per CU (0: all that fit, 4 at its
+        # 128-VGPR bound) x CUs x 256
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        threads = (choice["blocks_per_cu"] or 4) * cus * 256
+        if rec.get("workload") == spec.name and threads is not None and str(threads) in rec["by_threads"]:
+            valu = {"valu_share": rec["by_threads"][str(threads)]["valu_share"], "source": str(sq.relative_to(ROOT))}
     achieved = acc["step_kernel_bytes_per_launch"] / t_kernel
     out = {
         "metric": "particle-steps/s at 1M beads",
@@ -337,7 +349,8 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_unit": "B/launch",
                      "traffic_source": traffic_src,
                      "avg_launch_us": t_kernel * 1e6, "launches": launches,
-                     "other_ms_per_step": ms_step - kernel_ms / max(launches, 1)},
+                     "other_ms_per_step": ms_step - kernel_ms / max(launches, 1),
+                     "issue": valu},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(spec, args.cpu_steps)
