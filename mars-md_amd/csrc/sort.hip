@@ -3,14 +3,18 @@
 //
 // Replaces the CUB DeviceRadixSort::SortPairs call of the reference
 // (src/PatchOperation/ZOrderKernels/CUDASort.cu:13-98; stability contract pinned by
-// Unit_test/System/zorder_tests.cpp:139-151). One pass per 8-bit digit:
+// Unit_test/System/zorder_tests.cpp:139-151). One pass per digit of 8 bits, or of 10
+// bits when that takes fewer passes (3 instead of 4 for 30-bit Morton keys, 1 instead
+// of 2 for keys below 2^10); the last digit is masked to key_bits:
 //   1. k_rs_hist     per-tile digit histogram (wave-aggregated LDS counts),
 //   2. exclusive scan over [digit][tile] (digit-major; reduce / spine / apply),
-//   3. k_rs_scatter  per-tile stable ranks from wave64 ballot peer masks, then
-//                    scatter to offset[digit][tile] + rank.
+//   3. k_rs_scatter  per-tile stable ranks from wave64 ballot peer masks, staged in
+//                    LDS in digit order, then written to offset[digit][tile] + rank.
 // A tile is 4 waves x 16 rows x 64 lanes = 4096 keys; wave w owns the contiguous
 // quarter [w*1024, (w+1)*1024) of its tile, so ranks follow index order and the
-// sort is stable.
+// sort is stable. Both kernels issue all 16 row loads of a wave before ranking, so a
+// wave waits on memory once per tile rather than once per row; the scatter stages the
+// ranked tile in LDS and writes it out in digit order.
 #include "internal.hpp"
 
 namespace mmd {
@@ -20,18 +24,19 @@ namespace {
 constexpr int kRsBlock = 256;
 constexpr int kRsRows = 16;
 constexpr int kRsTile = kRsBlock * kRsRows;
-constexpr int kRsBuckets = 256;
+constexpr int kRsMaxBuckets = 1024; // 10-bit digits
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
 	const int lane = threadIdx.x & 63;
 	return (uint64_t(1) << lane) - 1;
 }
 
-// Lanes of this wave holding the same 8-bit digit (and valid).
+// Lanes of this wave holding the same BITS-bit digit (and valid).
+template<int BITS>
 __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
 	uint64_t peers = __ballot(valid);
 #pragma unroll
-	for (int b = 0; b < 8; ++b) {
+	for (int b = 0; b < BITS; ++b) {
 		const bool bit = (d >> b) & 1u;
 		const uint64_t m = __ballot(bit);
 		peers &= bit ? m : ~m;
@@ -39,25 +44,37 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
 	return valid ? peers : 0;
 }
 
+// Digit d of a key is (key >> shift) & mask, mask < 2^BITS.
+template<int BITS>
 __global__ __launch_bounds__(kRsBlock) void k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n,
-													  int shift, uint32_t* __restrict__ tile_hist,
+													  int shift, uint32_t mask,
+													  uint32_t* __restrict__ tile_hist,
 													  uint32_t num_tiles) {
-	__shared__ uint32_t h[kRsBuckets];
-	h[threadIdx.x] = 0;
-	__syncthreads();
+	constexpr uint32_t B = 1u << BITS;
+	__shared__ uint32_t h[B];
+	for (uint32_t i = threadIdx.x; i < B; i += kRsBlock)
+		h[i] = 0;
 	const int lane = threadIdx.x & 63;
 	const int w = threadIdx.x >> 6;
 	const uint64_t wbase = uint64_t(blockIdx.x) * kRsTile + uint64_t(w) * (kRsTile / 4);
+	uint32_t mk[kRsRows];
+#pragma unroll
 	for (int r = 0; r < kRsRows; ++r) {
 		const uint64_t i = wbase + uint64_t(r) * 64 + lane;
-		const bool valid = i < n;
-		const uint32_t d = valid ? (keys[i] >> shift) & 255u : 0u;
-		const uint64_t peers = digit_peers(d, valid);
+		mk[r] = i < n ? keys[i] : 0u;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < kRsRows; ++r) {
+		const bool valid = wbase + uint64_t(r) * 64 + lane < n;
+		const uint32_t d = (mk[r] >> shift) & mask;
+		const uint64_t peers = digit_peers<BITS>(d, valid);
 		if (valid && (peers & lanemask_lt()) == 0)
 			atomicAdd(&h[d], uint32_t(__popcll(peers)));
 	}
 	__syncthreads();
-	tile_hist[uint64_t(threadIdx.x) * num_tiles + blockIdx.x] = h[threadIdx.x];
+	for (uint32_t i = threadIdx.x; i < B; i += kRsBlock)
+		tile_hist[uint64_t(i) * num_tiles + blockIdx.x] = h[i];
 }
 
 // Block-wide exclusive scan for blockDim.x = nthreads (multiple of 64, <= 1024).
@@ -113,53 +130,97 @@ __global__ __launch_bounds__(1024) void k_scan_one_block(uint32_t* __restrict__ 
 		*total_out = total;
 }
 
+// Two-phase scatter: ranks keys within the tile by digit (stable), stages the tile in
+// LDS in digit order, then writes it out with consecutive lanes on consecutive output
+// addresses (one run per digit present in the tile) instead of one scattered address
+// per lane.
+template<int BITS>
 __global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const uint32_t* __restrict__ keys,
 														 const uint32_t* __restrict__ vals,
 														 uint32_t* __restrict__ keys_out,
 														 uint32_t* __restrict__ vals_out, uint64_t n,
-														 int shift,
+														 int shift, uint32_t mask,
 														 const uint32_t* __restrict__ tile_offsets,
 														 uint32_t num_tiles) {
-	__shared__ uint32_t base[kRsBuckets];
-	__shared__ uint32_t wcnt[4][kRsBuckets];
+	constexpr uint32_t B = 1u << BITS;
+	constexpr int kPer = int(B) / kRsBlock; // digits per thread in the tile-local scan
+	static_assert(kPer >= 1 && kPer * kRsBlock == int(B), "digit count must be a multiple of the block");
+	__shared__ uint32_t adj[B];		// output offset of the digit's run minus its start in the tile
+	__shared__ uint32_t wcnt[4][B]; // per-wave digit counts, then per-wave starts in the tile
+	__shared__ uint32_t stage_k[kRsTile];
+	__shared__ uint32_t stage_v[kRsTile];
+	__shared__ uint32_t wsum[kRsBlock / 64 + 1];
 	const int lane = threadIdx.x & 63;
 	const int w = threadIdx.x >> 6;
-	base[threadIdx.x] = tile_offsets[uint64_t(threadIdx.x) * num_tiles + blockIdx.x];
-#pragma unroll
-	for (int k = 0; k < 4; ++k)
-		wcnt[w][lane + 64 * k] = 0;
-	__syncthreads();
 	uint32_t mk[kRsRows], mv[kRsRows], mr[kRsRows];
-	const uint64_t wbase = uint64_t(blockIdx.x) * kRsTile + uint64_t(w) * (kRsTile / 4);
+	const uint64_t tile0 = uint64_t(blockIdx.x) * kRsTile;
+	const uint64_t wbase = tile0 + uint64_t(w) * (kRsTile / 4);
 #pragma unroll
 	for (int r = 0; r < kRsRows; ++r) {
 		const uint64_t i = wbase + uint64_t(r) * 64 + lane;
-		const bool valid = i < n;
-		const uint32_t k = valid ? keys[i] : 0u;
-		const uint32_t v = valid ? vals[i] : 0u;
-		const uint32_t d = (k >> shift) & 255u;
-		const uint64_t peers = digit_peers(d, valid);
+		mk[r] = i < n ? keys[i] : 0u;
+		mv[r] = i < n ? vals[i] : 0u;
+	}
+	for (uint32_t i = threadIdx.x; i < B; i += kRsBlock)
+		adj[i] = tile_offsets[uint64_t(i) * num_tiles + blockIdx.x];
+	for (uint32_t i = uint32_t(lane); i < B; i += 64)
+		wcnt[w][i] = 0;
+	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < kRsRows; ++r) {
+		const bool valid = wbase + uint64_t(r) * 64 + lane < n;
+		const uint32_t d = (mk[r] >> shift) & mask;
+		const uint64_t peers = digit_peers<BITS>(d, valid);
 		const uint64_t below = peers & lanemask_lt();
 		const uint32_t rank = wcnt[w][d] + uint32_t(__popcll(below));
 		__builtin_amdgcn_wave_barrier();
 		if (valid && below == 0)
 			wcnt[w][d] += uint32_t(__popcll(peers));
 		__builtin_amdgcn_wave_barrier();
-		mk[r] = k;
-		mv[r] = v;
 		mr[r] = valid ? rank : 0xFFFFFFFFu;
+	}
+	__syncthreads();
+	// Tile-local start of each digit (exclusive scan over digits), then each wave's start
+	// within the digit; thread t owns digits [t*kPer, (t+1)*kPer).
+	uint32_t total = 0;
+#pragma unroll
+	for (int k = 0; k < kPer; ++k) {
+		const uint32_t d = uint32_t(threadIdx.x) * kPer + k;
+		total += wcnt[0][d] + wcnt[1][d] + wcnt[2][d] + wcnt[3][d];
+	}
+	uint32_t run = block_exclusive_scan(total, wsum, nullptr);
+#pragma unroll
+	for (int k = 0; k < kPer; ++k) {
+		const uint32_t d = uint32_t(threadIdx.x) * kPer + k;
+		adj[d] -= run;
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			const uint32_t c = wcnt[q][d];
+			wcnt[q][d] = run;
+			run += c;
+		}
 	}
 	__syncthreads();
 #pragma unroll
 	for (int r = 0; r < kRsRows; ++r) {
 		if (mr[r] == 0xFFFFFFFFu)
 			continue;
-		const uint32_t d = (mk[r] >> shift) & 255u;
-		uint32_t dst = base[d] + mr[r];
-		for (int q = 0; q < w; ++q)
-			dst += wcnt[q][d];
-		keys_out[dst] = mk[r];
-		vals_out[dst] = mv[r];
+		const uint32_t d = (mk[r] >> shift) & mask;
+		const uint32_t pos = wcnt[w][d] + mr[r];
+		stage_k[pos] = mk[r];
+		stage_v[pos] = mv[r];
+	}
+	__syncthreads();
+	const uint32_t tile_n = n - tile0 < uint64_t(kRsTile) ? uint32_t(n - tile0) : uint32_t(kRsTile);
+#pragma unroll
+	for (int r = 0; r < kRsRows; ++r) {
+		const uint32_t j = uint32_t(r) * kRsBlock + threadIdx.x;
+		if (j < tile_n) {
+			const uint32_t k = stage_k[j];
+			const uint32_t dst = adj[(k >> shift) & mask] + j;
+			keys_out[dst] = k;
+			vals_out[dst] = stage_v[j];
+		}
 	}
 }
 
@@ -221,7 +282,8 @@ inline uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
 
 uint64_t radix_sort_scratch_bytes(uint64_t n) {
 	const uint64_t tiles = (n + kRsTile - 1) / kRsTile;
-	return align256(n * 4) * 2 + align256(tiles * kRsBuckets * 4 + 4) + scan_scratch_bytes(tiles * kRsBuckets);
+	return align256(n * 4) * 2 + align256(tiles * kRsMaxBuckets * 4 + 4) +
+		   scan_scratch_bytes(tiles * kRsMaxBuckets);
 }
 
 void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
@@ -240,19 +302,29 @@ void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
 	uint32_t* tmp_k = reinterpret_cast<uint32_t*>(sp);
 	uint32_t* tmp_v = reinterpret_cast<uint32_t*>(sp + align256(n * 4));
 	uint32_t* hist = reinterpret_cast<uint32_t*>(sp + 2 * align256(n * 4));
-	void* scan_tmp = sp + 2 * align256(n * 4) + align256(uint64_t(tiles) * kRsBuckets * 4 + 4);
-	const int passes = (key_bits + 7) / 8;
+	void* scan_tmp = sp + 2 * align256(n * 4) + align256(uint64_t(tiles) * kRsMaxBuckets * 4 + 4);
+	const int bits = (key_bits + 9) / 10 < (key_bits + 7) / 8 ? 10 : 8;
+	const int passes = (key_bits + bits - 1) / bits;
+	const uint64_t buckets = uint64_t(1) << bits;
 	const uint32_t* src_k = keys_in;
 	const uint32_t* src_v = vals_in;
 	for (int p = 0; p < passes; ++p) {
 		const bool to_out = ((passes - 1 - p) % 2) == 0;
 		uint32_t* dk = to_out ? keys_out : tmp_k;
 		uint32_t* dv = to_out ? vals_out : tmp_v;
-		const int shift = 8 * p;
-		k_rs_hist<<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, hist, tiles);
+		const int shift = bits * p;
+		const int width = key_bits - shift < bits ? key_bits - shift : bits;
+		const uint32_t mask = (1u << width) - 1u;
+		if (bits == 10)
+			k_rs_hist<10><<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles);
+		else
+			k_rs_hist<8><<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles);
 		MMD_LAUNCH_CHECK();
-		exclusive_scan_u32(hist, hist, uint64_t(tiles) * kRsBuckets, scan_tmp, s);
-		k_rs_scatter<<<tiles, kRsBlock, 0, s>>>(src_k, src_v, dk, dv, n, shift, hist, tiles);
+		exclusive_scan_u32(hist, hist, uint64_t(tiles) * buckets, scan_tmp, s);
+		if (bits == 10)
+			k_rs_scatter<10><<<tiles, kRsBlock, 0, s>>>(src_k, src_v, dk, dv, n, shift, mask, hist, tiles);
+		else
+			k_rs_scatter<8><<<tiles, kRsBlock, 0, s>>>(src_k, src_v, dk, dv, n, shift, mask, hist, tiles);
 		MMD_LAUNCH_CHECK();
 		src_k = dk;
 		src_v = dv;

@@ -56,11 +56,12 @@ def test_sort_stability_with_heavy_duplicates(cuda):
     assert np.array_equal(z.get_sorted_indices().cpu().numpy(), np.argsort(keys, kind="stable"))
 
 
-def test_radix_sort_full_32bit_keys(cuda):
-    import ctypes as C
-
-    n = 300_000
-    rs = np.random.default_rng(4)
+@pytest.mark.parametrize("key_bits,n", [(32, 300_000), (30, 100_003), (25, 70_001), (20, 4097), (9, 200_000)])
+def test_radix_sort_orders_by_the_low_key_bits(cuda, key_bits, n):
+    # 8-bit digits for 32 bits; 10-bit digits for 30, 25, 20 and 9 bits (fewer passes).
+    # Keys carry bits above key_bits: the sort orders by bits [0, key_bits) only and
+    # keeps the input order among equal low bits.
+    rs = np.random.default_rng(key_bits)
     keys = rs.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
     keys[::7] = keys[3]  # duplicates
     vals = np.arange(n, dtype=np.uint32)
@@ -68,10 +69,11 @@ def test_radix_sort_full_32bit_keys(cuda):
     ko, vo = cuda.empty_like(kd), cuda.empty_like(vd)
     nbytes = marsmd.lib().marsmd_radix_sort_scratch_bytes(n)
     scratch = cuda.empty(nbytes, dtype=cuda.uint8, device="cuda")
-    marsmd._capi.call("marsmd_radix_sort_pairs", kd.data_ptr(), vd.data_ptr(), ko.data_ptr(), vo.data_ptr(), n, 32,
-                      scratch.data_ptr(), nbytes, cuda.cuda.current_stream().cuda_stream)
+    marsmd._capi.call("marsmd_radix_sort_pairs", kd.data_ptr(), vd.data_ptr(), ko.data_ptr(), vo.data_ptr(), n,
+                      key_bits, scratch.data_ptr(), nbytes, cuda.cuda.current_stream().cuda_stream)
     cuda.cuda.synchronize()
-    order = np.argsort(keys, kind="stable")
+    low = keys & np.uint32((1 << key_bits) - 1) if key_bits < 32 else keys
+    order = np.argsort(low, kind="stable")
     assert np.array_equal(vo.cpu().numpy().view(np.uint32), order)
     assert np.array_equal(ko.cpu().numpy().view(np.uint32), keys[order])
 
