@@ -212,7 +212,9 @@ marsmd_status marsmd_zorder_reorder_data(marsmd_zorder* z, const void* in, void*
 /* ZOrderSort::remap_indices (RemapIndicesKernel ZOrderKernels.h:87-101). */
 marsmd_status marsmd_zorder_remap_indices(marsmd_zorder* z, int32_t* indices, uint64_t n,
                                           marsmd_stream stream);
-/* Stand-alone stable key/index sort used by the above (device keys in, sorted out). */
+/* Stand-alone stable key/index sort used by the above (device keys in, sorted out):
+ * orders by bits [0, key_bits) of the keys; the four buffers must not overlap
+ * (MARSMD_ERR_VALUE otherwise). */
 marsmd_status marsmd_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in,
                                       uint32_t* keys_out, uint32_t* vals_out, uint64_t n,
                                       int32_t key_bits, void* scratch, uint64_t scratch_bytes,

@@ -297,6 +297,11 @@ void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
 		raise(MARSMD_ERR_VALUE, "radix sort: scratch too small");
 	if (key_bits < 1 || key_bits > 32)
 		raise(MARSMD_ERR_VALUE, "radix sort: key_bits must be in [1, 32]");
+	// the first pass may write the outputs while it reads the inputs
+	const auto overlap = [n](const uint32_t* a, const uint32_t* b) { return a < b + n && b < a + n; };
+	if (overlap(keys_in, keys_out) || overlap(keys_in, vals_out) || overlap(vals_in, keys_out) ||
+		overlap(vals_in, vals_out) || overlap(keys_out, vals_out))
+		raise(MARSMD_ERR_VALUE, "radix sort: output buffers must not overlap the inputs or each other");
 	const uint32_t tiles = uint32_t((n + kRsTile - 1) / kRsTile);
 	char* sp = static_cast<char*>(scratch);
 	uint32_t* tmp_k = reinterpret_cast<uint32_t*>(sp);

@@ -43,6 +43,16 @@ def test_null_arguments_are_rejected_without_touching_the_device():
         api.call("marsmd_patch_run", None, None, 0, 1, None)
 
 
+def test_radix_sort_rejects_overlapping_buffers_without_touching_the_device():
+    # the checks run on the host before any launch; the addresses are never dereferenced
+    n = 1000
+    nbytes = api.lib().marsmd_radix_sort_scratch_bytes(n)
+    a, b, c, d, scratch = (0x10000000 * k for k in range(1, 6))
+    for keys_in, vals_in, keys_out, vals_out in [(a, b, a, d), (a, b, c, b), (a, b, c, c), (a, b, a + 4 * (n - 1), d)]:
+        with pytest.raises(api.MarsError, match="overlap"):
+            api.call("marsmd_radix_sort_pairs", keys_in, vals_in, keys_out, vals_out, n, 30, scratch, nbytes, None)
+
+
 STRUCTS = {
     "marsmd_box": api.Box,
     "marsmd_table": api.Table,
