@@ -70,41 +70,45 @@ __global__ void k_b_encode(const float4* __restrict__ pos, uint64_t total, const
 	const uint32_t qx = morton_quantize(p.x, G.lo[0], G.hi[0], out);
 	const uint32_t qy = morton_quantize(p.y, G.lo[1], G.hi[1], out);
 	const uint32_t qz = morton_quantize(p.z, G.lo[2], G.hi[2], out);
+	// stored at the canonical index, so the stable sort breaks ties by it; the value is
+	// the particle's current slot, which the gather reads from
 	keys[g] = split_by3(qz) | (split_by3(qy) << 1) | (split_by3(qx) << 2);
-	vals[g] = g;
+	vals[g] = uint32_t(s);
 	if (out)
 		atomicOr(outside, 1u);
 }
 
-// Second pass key: the replica of each Morton-sorted entry (its global canonical index).
+// Second pass key: the replica of each Morton-sorted entry (a slot of that replica),
+// carrying the entry's current slot along.
 __global__ void k_b_rep_keys(const uint32_t* __restrict__ vals_s, uint64_t total,
 							 const uint32_t* __restrict__ rep_of_slot, uint32_t* __restrict__ rkeys,
 							 uint32_t* __restrict__ rvals) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= total)
 		return;
-	rkeys[i] = rep_of_slot[vals_s[i]]; // canonical positions of replica r are its slots' range
-	rvals[i] = uint32_t(i);
+	const uint32_t old = vals_s[i];
+	rkeys[i] = rep_of_slot[old];
+	rvals[i] = old;
 }
 
-// New slot i holds the particle of global canonical index perm[i] = vals_s[rvals_s[i]],
-// with Morton code keys_s[rvals_s[i]].
-__global__ void k_b_gather(const uint32_t* __restrict__ rvals_s, const uint32_t* __restrict__ vals_s,
-						   const uint32_t* __restrict__ keys_s, uint64_t total,
+// New slot i holds the particle of current slot old_slot[i], its Morton code computed
+// again from the same position with the same operations as k_b_encode.
+__global__ void k_b_gather(const uint32_t* __restrict__ old_slot, uint64_t total,
 						   const uint32_t* __restrict__ rep_of_slot, const RepGeom* __restrict__ geom,
-						   const uint32_t* __restrict__ slot_of_canon, const float4* __restrict__ pos_in,
-						   const float4* __restrict__ mom_in, const int32_t* __restrict__ ids_in, bool relabel,
-						   float4* __restrict__ pos_out, float4* __restrict__ snap, float4* __restrict__ mom_out,
-						   int32_t* __restrict__ ids_out, uint32_t* __restrict__ codes,
-						   uint32_t* __restrict__ slot_of_canon_out) {
+						   const float4* __restrict__ pos_in, const float4* __restrict__ mom_in,
+						   const int32_t* __restrict__ ids_in, bool relabel, float4* __restrict__ pos_out,
+						   float4* __restrict__ snap, float4* __restrict__ mom_out, int32_t* __restrict__ ids_out,
+						   uint32_t* __restrict__ codes, uint32_t* __restrict__ slot_of_canon_out) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= total)
 		return;
-	const uint32_t k = rvals_s[i];
-	const uint32_t g = vals_s[k];
-	const uint32_t old = slot_of_canon[g];
+	const uint32_t old = old_slot[i];
 	const RepGeom& G = geom[rep_of_slot[i]];
 	float4 p = pos_in[old];
+	bool out = false;
+	const uint32_t qx = morton_quantize(p.x, G.lo[0], G.hi[0], out);
+	const uint32_t qy = morton_quantize(p.y, G.lo[1], G.hi[1], out);
+	const uint32_t qz = morton_quantize(p.z, G.lo[2], G.hi[2], out);
 	const uint32_t m = __float_as_uint(p.w);
 	const uint32_t canon = relabel ? uint32_t(i) - G.slot0 : meta_canon(m);
 	p.w = __uint_as_float(make_meta(meta_type(m), canon));
@@ -112,7 +116,7 @@ __global__ void k_b_gather(const uint32_t* __restrict__ rvals_s, const uint32_t*
 	snap[i] = p;
 	mom_out[i] = mom_in[old];
 	ids_out[i] = ids_in[old];
-	codes[i] = keys_s[k];
+	codes[i] = split_by3(qz) | (split_by3(qy) << 1) | (split_by3(qx) << 2);
 	slot_of_canon_out[G.slot0 + canon] = uint32_t(i);
 }
 
@@ -739,8 +743,7 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 	radix_sort_pairs(rkeys.data(), rvals.data(), rkeys_s.data(), rvals_s.data(), total, rep_bits, sort_scratch.data(),
 					 sort_scratch.count, s);
 	const int nxt = 1 - cur;
-	k_b_gather<<<grid_for(total, 256), 256, 0, s>>>(rvals_s.data(), vals_s.data(), keys_s.data(), total,
-													rep_of_slot.data(), gd, slot_of_canon.data(), pos[cur].data(),
+	k_b_gather<<<grid_for(total, 256), 256, 0, s>>>(rvals_s.data(), total, rep_of_slot.data(), gd, pos[cur].data(),
 													mom.data(), ids.data(), relabel, pos[nxt].data(), snap.data(),
 													mom_tmp.data(), ids_tmp.data(), codes.data(),
 													slot_of_canon_tmp.data());
