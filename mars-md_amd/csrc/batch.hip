@@ -358,25 +358,24 @@ __global__ void k_b_write_pairs(NbrList L, const float4* __restrict__ pos, uint6
 	}
 }
 
-// Full-list entries per replica (twice its pair count).
-__global__ __launch_bounds__(256) void k_b_rep_pairs(const uint32_t* __restrict__ counts, uint64_t total,
-													 const uint32_t* __restrict__ rep_of_slot,
+// Full-list entries of each replica (twice its pair count): one workgroup per replica
+// sums the counts of its slots, which are one contiguous range.
+__global__ __launch_bounds__(256) void k_b_rep_pairs(const uint32_t* __restrict__ counts,
+													 const RepGeom* __restrict__ geom,
 													 unsigned long long* __restrict__ out) {
-	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	const uint32_t r = t < total ? rep_of_slot[t] : 0xFFFFFFFFu;
-	const uint32_t c = t < total ? counts[t] : 0u;
-	// lanes of one replica are contiguous: reduce runs of equal r within the wave
-	const uint32_t r0 = uint32_t(__builtin_amdgcn_readfirstlane(int(r)));
-	const bool uniform = __ballot(r == r0) == __ballot(true);
-	if (uniform) {
-		uint32_t sum = c;
-		for (int o = 32; o > 0; o >>= 1)
-			sum += uint32_t(__shfl_xor(int(sum), o, 64));
-		if ((threadIdx.x & 63) == 0 && r0 != 0xFFFFFFFFu)
-			atomicAdd(&out[r0], (unsigned long long)sum);
-	} else if (t < total) {
-		atomicAdd(&out[r], (unsigned long long)c);
-	}
+	__shared__ unsigned long long wsum[4];
+	const uint32_t b = geom[blockIdx.x].slot0;
+	const uint32_t e = b + geom[blockIdx.x].n;
+	unsigned long long sum = 0;
+	for (uint32_t i = b + threadIdx.x; i < e; i += 256u)
+		sum += counts[i];
+	for (int o = 32; o > 0; o >>= 1)
+		sum += __shfl_xor(sum, o, 64);
+	if ((threadIdx.x & 63) == 0)
+		wsum[threadIdx.x >> 6] = sum;
+	__syncthreads();
+	if (threadIdx.x == 0)
+		out[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
 // BAOAB_LastUpdate (BAOAB.h:151-172) on every replica's particles.
@@ -807,8 +806,7 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 	}
 	// Pairlist::grow_pair_capacity (Pairlist.h:244-251) bookkeeping per replica: it sets
 	// the reference grid of the next build
-	fill_u32(reinterpret_cast<uint32_t*>(rep_pairs.data()), 0u, 2 * R, s);
-	k_b_rep_pairs<<<grid_for(total, 256), 256, 0, s>>>(counts.data(), total, rep_of_slot.data(), rep_pairs.data());
+	k_b_rep_pairs<<<unsigned(R), 256, 0, s>>>(counts.data(), gd, rep_pairs.data());
 	MMD_LAUNCH_CHECK();
 	MMD_HIP(hipMemcpyAsync(rep_pairs_host.data(), rep_pairs.data(), 8 * R, hipMemcpyDeviceToHost, s));
 	MMD_HIP(hipStreamSynchronize(s));
