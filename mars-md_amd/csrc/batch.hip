@@ -367,8 +367,13 @@ __global__ __launch_bounds__(256) void k_b_rep_pairs(const uint32_t* __restrict_
 	const uint32_t b = geom[blockIdx.x].slot0;
 	const uint32_t e = b + geom[blockIdx.x].n;
 	unsigned long long sum = 0;
-	for (uint32_t i = b + threadIdx.x; i < e; i += 256u)
-		sum += counts[i];
+	for (uint32_t i = b + threadIdx.x; i < e; i += 1024u) { // four loads in flight
+		const uint32_t c0 = counts[i];
+		const uint32_t c1 = i + 256u < e ? counts[i + 256u] : 0u;
+		const uint32_t c2 = i + 512u < e ? counts[i + 512u] : 0u;
+		const uint32_t c3 = i + 768u < e ? counts[i + 768u] : 0u;
+		sum += (unsigned long long)c0 + c1 + c2 + c3;
+	}
 	for (int o = 32; o > 0; o >>= 1)
 		sum += __shfl_xor(sum, o, 64);
 	if ((threadIdx.x & 63) == 0)
