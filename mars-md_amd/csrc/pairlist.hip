@@ -384,10 +384,15 @@ __global__ __launch_bounds__(256) void k_count_stats(const uint32_t* __restrict_
 	__shared__ unsigned long long wsum[4];
 	uint32_t m = 0;
 	unsigned long long sum = 0;
-	for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += uint64_t(gridDim.x) * blockDim.x) {
-		const uint32_t c = counts[t];
-		m = max(m, c);
-		sum += c;
+	const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+	for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += 4 * stride) {
+		// four loads in flight per thread (0 is neutral for both the sum and the maximum)
+		const uint32_t c0 = counts[t];
+		const uint32_t c1 = t + stride < n ? counts[t + stride] : 0u;
+		const uint32_t c2 = t + 2 * stride < n ? counts[t + 2 * stride] : 0u;
+		const uint32_t c3 = t + 3 * stride < n ? counts[t + 3 * stride] : 0u;
+		m = max(max(m, c0), max(c1, max(c2, c3)));
+		sum += (unsigned long long)c0 + c1 + c2 + c3;
 	}
 	for (int o = 32; o > 0; o >>= 1) {
 		m = max(m, uint32_t(__shfl_xor(int(m), o, 64)));
