@@ -963,23 +963,31 @@ void PatchImpl::autotune(const marsmd_run_params& P, uint64_t first_step, uint64
 		for (int v : per_cu)
 			cands.push_back(Cand{v, sp});
 	}
+	// the candidates take turns step by step, so each is timed at every point of the
+	// rebuild period (the steps after a rebuild are slower: cold caches); each keeps the
+	// median of its step-kernel times
 	const bool was_timing = timing;
-	const uint64_t passes = 2;
-	const uint64_t chunk = num_steps / (passes * cands.size());
+	const uint64_t rounds = num_steps / cands.size();
+	std::vector<std::vector<double>> times(cands.size());
 	uint64_t step = first_step;
-	if (chunk >= 1) {
-		for (uint64_t pass = 0; pass < passes; ++pass)
-			for (Cand& c : cands) {
-				set_launch(c.bpc, c.split);
-				set_kernel_timing(true);
-				run(P, step, chunk, s);
-				step += chunk;
-				double ms = 0.0;
-				uint64_t k = 0;
-				kernel_timing(&ms, &k);
-				if (k)
-					c.ms = std::min(c.ms, ms / double(k));
-			}
+	for (uint64_t r = 0; r < rounds; ++r)
+		for (size_t ci = 0; ci < cands.size(); ++ci) {
+			set_launch(cands[ci].bpc, cands[ci].split);
+			set_kernel_timing(true);
+			run(P, step, 1, s);
+			step += 1;
+			double ms = 0.0;
+			uint64_t k = 0;
+			kernel_timing(&ms, &k);
+			if (k)
+				times[ci].push_back(ms / double(k));
+		}
+	for (size_t ci = 0; ci < cands.size(); ++ci) {
+		std::vector<double>& t = times[ci];
+		if (t.empty())
+			continue;
+		std::sort(t.begin(), t.end());
+		cands[ci].ms = t.size() % 2 ? t[t.size() / 2] : 0.5 * (t[t.size() / 2 - 1] + t[t.size() / 2]);
 	}
 	const Cand* best = &cands.front();
 	for (const Cand& c : cands)
