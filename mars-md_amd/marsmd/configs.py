@@ -354,3 +354,33 @@ def config4(n: int = 100_000, L: float = 1000.0, snapshots: int = 64, knots: int
     return FMWorkload(name=f"force-matching-{frames.shape[0]}x{n}-beads-4types-L{L:g}-10x{knots}-knots",
                       box=PeriodicBox((L, L, L)), positions=frames, type_ids=type_ids, tables=tables,
                       target_tables=perturbed_tables(tables, perturbation, seed), term_matrix=tm, cutoff=30.0)
+
+
+def bd_snapshots(n: int = 100_000, L: float = 1000.0, snapshots: int = 64, seed: int = 42, warmup: int = 100,
+                 stride: int = 10):
+    """Brownian-dynamics snapshots of the 4-type WCA+DH system of the force-matching
+    workload (the fm_tables() pair tables, D = 150, dt = 100 fs, cutoff 30 + 5 A skin),
+    stepped by a GPU Patch from uniform positions: `warmup` steps, then one frame every
+    `stride` steps. Returns (frames (S, n, 3) float32 in global-id order, type ids)."""
+    from .patch import Patch
+
+    tables, tm = fm_tables()
+    type_ids = np.minimum((rng.uniform(seed, rng.STREAM_POSITIONS ^ 0x4, n)[:, 0] * 4).astype(np.int32), 3)
+    q = [1.0, -1.0, 0.5, -0.5]
+    spec = SystemSpec(box=PeriodicBox((L, L, L)),
+                      types=[ParticleType(mass=1.0, charge=qi, diffusion=(150.0,) * 3) for qi in q],
+                      positions=uniform_positions(n, L, seed), type_ids=type_ids,
+                      params=RunParams(cutoff=30.0, pairlist_cutoff=35.0, rebuild_period=20, timestep=1e-4, kT=KT,
+                                       integrator=api.INTEGRATOR_BROWNIAN),
+                      nb_tables=tables, term_matrix=tm, seed=seed, name="fm-bd")
+    patch = Patch(spec)
+    patch.run(warmup)
+    out = np.zeros((snapshots, n, 3), np.float32)
+    step = warmup
+    for s in range(snapshots):
+        patch.run(stride, first_step=step)
+        step += stride
+        h = patch.copy_particles_to_host()
+        out[s, h["id"]] = h["pos"][:, :3]
+    return out, type_ids
+

@@ -58,3 +58,34 @@ def seam_pair_positions(n=20000, L=160.01, seed=21, planted=40):
             pos[k + 1, a] = 0.02
             k += 2
     return pos
+
+
+def canonical_topology(spec, ids):
+    """Bonded/exclusion tuples of the spec (written in global ids) in the patch's
+    current canonical indices: canonical c holds global id ids[c]."""
+    canon_of_id = np.empty(len(ids), np.int64)
+    canon_of_id[ids] = np.arange(len(ids))
+    out = {}
+    for name in ("bonds", "angles", "exclusions", "dihedrals"):
+        arr = getattr(spec, name)
+        out[name] = None if arr is None else canon_of_id[np.asarray(arr)].astype(np.int32)
+    r = spec.restraint_ids
+    if r is None:
+        out["restraint_ids"] = None
+    else:
+        r = np.asarray(r)
+        out["restraint_ids"] = np.where(r >= 0, canon_of_id[np.maximum(r, 0)], -1).astype(np.int32)
+    return out
+
+
+def min_image(d, L):
+    """|d| folded to the minimum image of a periodic length L (per component)."""
+    d = np.abs(np.asarray(d, np.float64))
+    return np.minimum(d, L - d)
+
+
+def host_threads(cap: int = 16) -> int:
+    """CPU threads this process may use (its affinity mask, at most `cap`)."""
+    import os
+
+    return max(1, min(cap, len(os.sched_getaffinity(0))))

@@ -13,7 +13,7 @@ import marsmd
 from marsmd import PeriodicBox, configs
 from marsmd import _capi as api
 from oracle import pyoracle as orc
-from tests.conftest import rel_force_error, sorted_pairs
+from tests.conftest import canonical_topology as _canonical_topology, rel_force_error, sorted_pairs
 
 pytestmark = pytest.mark.gpu
 FORCE_TOL = 1e-5
@@ -199,24 +199,6 @@ def test_bd_and_baoab_launchers_single_step(cuda):
         p = np.mod(p, 150.0)
         assert np.abs(md.cpu().numpy()[i, :3] - m).max() < 1e-3
         assert _min_image(pd.cpu().numpy()[i, :3] - p, 150.0).max() < 1e-4
-
-
-def _canonical_topology(spec, ids):
-    """Bonded/exclusion tuples of the spec (written in global ids) in the patch's
-    current canonical indices: canonical c holds global id ids[c]."""
-    canon_of_id = np.empty(len(ids), np.int64)
-    canon_of_id[ids] = np.arange(len(ids))
-    out = {}
-    for name in ("bonds", "angles", "exclusions", "dihedrals"):
-        arr = getattr(spec, name)
-        out[name] = None if arr is None else canon_of_id[np.asarray(arr)].astype(np.int32)
-    r = spec.restraint_ids
-    if r is None:
-        out["restraint_ids"] = None
-    else:
-        r = np.asarray(r)
-        out["restraint_ids"] = np.where(r >= 0, canon_of_id[np.maximum(r, 0)], -1).astype(np.int32)
-    return out
 
 
 def _check_step(cuda, patch, spec, step, check_pairs=True):
