@@ -1,12 +1,16 @@
 """Throughput of the per-step particle hot path on one MI355X.
 
-Workload (default): 1,000,000 beads of 3 types in a 2,000 A periodic box, 6 tabulated
-WCA+DH pair tables (1,024 knots), a 256^3 FP32 membrane grid sampled tricubically,
-Langevin (BAOAB, gamma = 1/ps), cutoff 30 A + 5 A skin, rebuild period 20.
+Workload (the value): 1,000,000 beads of 3 types in a 2,000 A periodic box, 6
+tabulated WCA+DH pair tables (1,024 knots), a 256^3 FP32 membrane grid sampled
+tricubically, Langevin (BAOAB, gamma = 1/ps), cutoff 30 A + 5 A skin, rebuild period 20.
 
 A step is SimManager's per-step sequence (rebuild policy, forces, PMF, integrator)
 over the whole system. Inputs are resident in HBM before the timed region; timing
-uses HIP events on the stream the library launches on. Prints one JSON line.
+uses HIP events on the stream the library launches on; the value is the median of
+REPEATS timed runs of `--steps` steps. The same JSON line carries the replica batch
+(512 BD replicas, aggregate particle-steps/s, against the same replicas as separate
+Patches), force matching (gradient evaluations/s on 64 BD snapshots x 100k beads), a
+measured copy bandwidth beside the 8 TB/s peak, and the oracle's CPU baseline.
 """
 from __future__ import annotations
 
@@ -23,6 +27,40 @@ sys.path.insert(0, str(ROOT / "mars-md_amd"))
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+REPEATS = 5  # timed runs per figure; the median is reported
+# Algorithmic bytes per unit of the two companion workloads: a batch particle-step
+# (28 B state + (144 B rebuild + 8 B x 9.0 half pairs) / 20 at density 1e-4, the
+# accounting of algorithmic_bytes) and one force-matching gradient evaluation (64
+# snapshots x 100k beads x (16 B positions and types + 12 B target forces + 144 B
+# per-snapshot rebuild + 8 B x 9.0 half pairs)).
+FM_BYTES_PER_EVAL = 64 * 100_000 * (16.0 + 12.0 + 144.0 + 8.0 * 9.0)
+
+
+def median(xs):
+    xs = sorted(xs)
+    k = len(xs)
+    return xs[k // 2] if k % 2 else 0.5 * (xs[k // 2 - 1] + xs[k // 2])
+
+
+def copy_bandwidth(torch, n: int = 1 << 28, reps: int = REPEATS) -> dict:
+    """Measured streaming bandwidth: y = x over n float32 (4n bytes read, 4n written),
+    timed with events, median of `reps` copies."""
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        e0.record()
+        y.copy_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    t = median(times)
+    del x, y
+    return {"GBps": 8.0 * n / t / 1e9, "bytes_per_copy": 8 * n, "peak_GBps": HBM_PEAK / 1e9,
+            "frac_of_peak": 8.0 * n / t / HBM_PEAK, "repeats": reps}
 
 
 def algorithmic_bytes(spec) -> dict:
@@ -92,145 +130,163 @@ def copy_probe(torch, n: int = 1 << 28, reps: int = 3):
                                      "bytes_written_per_launch": 4 * n, "launches": reps}}))
 
 
-def bench_batch(args, torch):
+def timed_runs(torch, run, reps: int = REPEATS, stream=None):
+    """Milliseconds of each of `reps` calls of run(rep), each bracketed by HIP events
+    on `stream` after a device synchronisation."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = []
+    for rep in range(reps):
+        torch.cuda.synchronize()
+        e0.record(stream)
+        run(rep)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out.append(e0.elapsed_time(e1))
+    return out
+
+
+def batch_bytes_per_particle_step(specs) -> float:
+    """Algorithmic bytes per particle-step of the replica batch (the per-replica
+    accounting of algorithmic_bytes, weighted by replica size)."""
+    total = sum(s.n for s in specs)
+    return sum(algorithmic_bytes(s)["b_alg_per_particle_step"] * s.n for s in specs) / total
+
+
+def bench_batch(torch, steps: int, warmup: int, sequential: bool = True) -> dict:
     """Aggregate particle-steps/s of 512 independent BD replicas (N log-uniform in
-    [500, 20000], density 1e-4, config-0 table) stepped together, and of the same
-    replicas stepped one after another as separate Patches."""
+    [500, 20000], density 1e-4, config-0 table) stepped together at the autotuner's
+    launch (median of REPEATS runs of `steps` steps), and of the same replicas stepped
+    one after another as separate Patches."""
     import marsmd
     from marsmd import configs
     from marsmd.batch import Batch
 
     specs = configs.config3()
     total = sum(s.n for s in specs)
-    K = args.steps
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    # the default launch (eager, every workgroup that fits), then the autotuned one on a
-    # fresh batch: the autotuner's steps are the second batch's warmup
+    K = max(steps, 20)
     b = Batch(specs)
-    b.run(args.warmup)
-    torch.cuda.synchronize()
-    e0.record()
-    b.run(K, first_step=args.warmup)
-    e1.record()
-    torch.cuda.synchronize()
-    ms_default = e0.elapsed_time(e1)
-    del b
-    b = Batch(specs)
-    tune_steps = max(args.warmup, 160)
-    choice = b.autotune(tune_steps, cache=False)
-    torch.cuda.synchronize()
-    e0.record()
-    b.run(K, first_step=tune_steps)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
+    # warmup: the cell configuration per size bucket (timed on the walk), then the launch
+    # configuration (timed on real steps), both the autotuners' real steps
+    cell_steps = 40
+    cells = b.autotune_cells(cell_steps, cache=False)
+    tune_steps = max(warmup, 160)
+    choice = b.autotune(tune_steps, first_step=cell_steps, cache=False)
+    step = cell_steps + tune_steps
+
+    def run(rep):
+        nonlocal step
+        b.run(K, first_step=step)
+        step += K
+
+    times = timed_runs(torch, run)
+    ms = median(times)
+    # the same steps with and without hipGraph replay of the runs between host decisions,
+    # at the workgroups per CU kept
+    by_graph = {}
+    for graph in (False, True):
+        b.set_launch(choice["blocks_per_cu"], graph)
+        by_graph[graph] = median(timed_runs(torch, run, reps=3)) / K
+    b.set_launch(choice["blocks_per_cu"], choice["use_graph"])
     # step-kernel timing in a pass of its own (its events add dispatch gaps)
     b.set_kernel_timing(True)
-    b.run(K, first_step=tune_steps + K)
+    b.run(K, first_step=step)
+    step += K
     kms, launches = b.kernel_timing()
     b.set_kernel_timing(False)
-    # the same replicas as separate Patches, one after another on the same stream
-    patches = [marsmd.Patch(s) for s in specs]
-    for p in patches:
-        p.run(args.warmup)
-    torch.cuda.synchronize()
-    Kseq = min(K, 40)
-    e0.record()
-    for p in patches:
-        p.run(Kseq, first_step=args.warmup)
-    e1.record()
-    torch.cuda.synchronize()
-    ms_seq = e0.elapsed_time(e1)
     value = total * K / (ms / 1e3)
-    seq = total * Kseq / (ms_seq / 1e3)
-    print(json.dumps({
-        "metric": "aggregate particle-steps/s, 512 replicas", "value": value, "unit": "particle-steps/s",
-        "n_gpus": 1, "steps": K, "warmup": tune_steps, "ms_per_step": ms / K, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded)",
-        "config": {"workload": "bd-512-replicas-500..20000-beads-density1e-4", "replicas": len(specs),
-                   "beads": total, "integrator": "brownian", "cutoff_A": 30.0, "skin_A": 5.0, "rebuild_period": 20},
-        "launch": {"blocks_per_cu": choice["blocks_per_cu"], "use_graph": choice["use_graph"],
-                   "tuned_ms_per_step": choice["ms_per_step"]},
-        "default_launch_ms_per_step": ms_default / K,
-        "step_kernel_avg_us": kms / max(launches, 1) * 1e3,
-        "sequential_patches": {"value": seq, "steps": Kseq, "ms_per_step": ms_seq / Kseq},
-        "speedup_over_sequential": value / seq}))
+    b_alg = batch_bytes_per_particle_step(specs)
+    out = {"metric": "aggregate particle-steps/s, 512 replicas", "value": value, "unit": "particle-steps/s",
+           "steps": K, "repeats": REPEATS, "ms_per_step": ms / K, "ms_per_step_runs": [t / K for t in times],
+           "warmup": cell_steps + tune_steps,
+           "config": {"workload": "bd-512-replicas-500..20000-beads-density1e-4", "replicas": len(specs),
+                      "beads": total, "integrator": "brownian", "cutoff_A": 30.0, "skin_A": 5.0,
+                      "rebuild_period": 20},
+           "launch": {"blocks_per_cu": choice["blocks_per_cu"], "use_graph": choice["use_graph"],
+                      "tuned_ms_per_step": choice["ms_per_step"]},
+           "ms_per_step_eager": by_graph[False], "ms_per_step_graph": by_graph[True],
+           "cells": {"per_size_bucket": {f"2^{bk}": {"home_target": v[0], "fine_offset": v[1]}
+                                         for bk, v in cells["cells"].items()},
+                     "walk_ms": cells["walk_ms"]},
+           "step_kernel_avg_us": kms / max(launches, 1) * 1e3,
+           "roofline": {"bound": "hbm", "b_alg_per_particle_step": b_alg, "achieved": value * b_alg / 1e9,
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": value * b_alg / HBM_PEAK}}
+    del b
+    if sequential:
+        # the same replicas as separate Patches, one after another on the same stream
+        patches = [marsmd.Patch(s) for s in specs]
+        for p in patches:
+            p.run(warmup)
+        Kseq = min(K, 40)
+        first = [warmup]
+
+        def run_seq(rep):
+            for p in patches:
+                p.run(Kseq, first_step=first[0])
+            first[0] += Kseq
+
+        tseq = median(timed_runs(torch, run_seq, reps=3))
+        seq = total * Kseq / (tseq / 1e3)
+        out["sequential_patches"] = {"value": seq, "steps": Kseq, "repeats": 3, "ms_per_step": tseq / Kseq}
+        out["speedup_over_sequential"] = value / seq
+    return out
 
 
-def bd_frames(n: int, L: float, snapshots: int, seed: int = 42, warmup: int = 100, stride: int = 10):
-    """Brownian-dynamics frames of the 4-type WCA+DH system (the force-matching
-    workload's snapshots), in global-id order, from the GPU Patch."""
-    import numpy as np
-
-    import marsmd
-    from marsmd import configs, rng
-    from marsmd.system import RunParams, SystemSpec
-    from marsmd.core import ParticleType
-    from marsmd import _capi as api
-
-    tables, tm = configs.fm_tables()
-    type_ids = np.minimum((rng.uniform(seed, rng.STREAM_POSITIONS ^ 0x4, n)[:, 0] * 4).astype(np.int32), 3)
-    q = [1.0, -1.0, 0.5, -0.5]
-    spec = SystemSpec(box=marsmd.PeriodicBox((L, L, L)),
-                      types=[ParticleType(mass=1.0, charge=qi, diffusion=(150.0,) * 3) for qi in q],
-                      positions=configs.uniform_positions(n, L, seed), type_ids=type_ids,
-                      params=RunParams(cutoff=30.0, pairlist_cutoff=35.0, rebuild_period=20, timestep=1e-4,
-                                       kT=configs.KT, integrator=api.INTEGRATOR_BROWNIAN),
-                      nb_tables=tables, term_matrix=tm, seed=seed, name="fm-bd")
-    patch = marsmd.Patch(spec)
-    patch.run(warmup)
-    out = np.zeros((snapshots, n, 3), np.float32)
-    step = warmup
-    for s in range(snapshots):
-        patch.run(stride, first_step=step)
-        step += stride
-        h = patch.copy_particles_to_host()
-        out[s, h["id"]] = h["pos"][:, :3]
-    return out, type_ids
-
-
-def bench_fm(args, torch):
-    """Gradient evaluations per second of force matching: forces of 64 snapshots of
-    100k beads for 10 tables x 512 knots, MSE loss, FP64 adjoint, Adam step."""
+def bench_fm(torch, evals: int, warmup: int) -> dict:
+    """Gradient evaluations per second of force matching: forces of 64 BD snapshots of
+    100k beads for 10 tables x 512 knots, MSE loss, FP64 adjoint, Adam step (median of
+    REPEATS runs of `evals` evaluations)."""
     from marsmd import configs
     from marsmd.fm import ForceMatching, force_matching_loss
 
     n, L, S = 100_000, 1000.0, 64
-    frames, type_ids = bd_frames(n, L, S)
+    frames, type_ids = configs.bd_snapshots(n, L, S)
     w = configs.config4(n=n, L=L, snapshots=S, frames=frames)
     assert (w.type_ids == type_ids).all()
     fm = ForceMatching(w.positions, w.type_ids, w.box, w.tables, w.term_matrix, w.cutoff)
     target = fm.forces(fm.knots(w.target_tables))[..., :3].clone()
     knots = fm.knots().clone().requires_grad_(True)
     opt = torch.optim.Adam([knots], lr=0.02)
+    last = [None]
 
     def one():
         opt.zero_grad()
         loss = force_matching_loss(knots, fm, target)
         loss.backward()
         opt.step()
+        last[0] = loss
         return loss
 
     first = one().item()
-    for _ in range(max(args.warmup // 4, 2)):
+    for _ in range(max(warmup, 2)):
         one()
-    K = max(args.steps // 10, 10)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(K):
-        loss = one()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    print(json.dumps({
-        "metric": "force-matching gradient evaluations/s (64 BD snapshots x 100k beads, 10 tables x 512 knots)",
-        "value": K / (ms / 1e3), "unit": "grad evals/s", "n_gpus": 1, "steps": K, "warmup": max(args.warmup // 4, 2),
-        "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-        "data": "synthetic (seeded BD frames)",
-        "config": {"workload": w.name, "snapshots": S, "beads": n, "tables": len(w.tables), "knots_per_table": 512,
-                   "pairs_total": fm.num_pairs()},
-        "loss_first": first, "loss_last": loss.item()}))
+    K = max(evals, 10)
+    times = timed_runs(torch, lambda rep: [one() for _ in range(K)])
+    ms = median(times)
+    value = K / (ms / 1e3)
+    return {"metric": "force-matching gradient evaluations/s (64 BD snapshots x 100k beads, 10 tables x 512 knots)",
+            "value": value, "unit": "grad evals/s", "evals": K, "repeats": REPEATS, "ms_per_eval": ms / K,
+            "ms_per_eval_runs": [t / K for t in times], "dtype": "f32+f64",
+            "config": {"workload": w.name, "snapshots": S, "beads": n, "tables": len(w.tables),
+                       "knots_per_table": 512, "pairs_total": fm.num_pairs()},
+            "roofline": {"bound": "hbm", "bytes_per_eval": FM_BYTES_PER_EVAL, "achieved": value * FM_BYTES_PER_EVAL / 1e9,
+                         "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": value * FM_BYTES_PER_EVAL / HBM_PEAK},
+            "loss_first": first, "loss_last": last[0].item()}
+
+
+def valu_issue(torch, choice, spec):
+    """The fused step kernel's VALU issue share (rocprofv3 SQ counters, profiles/) at the
+    launch shape kept, or None when no profile of that shape is there."""
+    sq = ROOT / "profiles" / "k_step_langevin_1M_grid256_sq.json"
+    if not sq.exists() or choice["split_pmf"]:
+        return None
+    rec = json.loads(sq.read_text())
+    # the persistent step kernel's threads: workgroups per CU (0: all that fit, 4 at its
+    # 128-VGPR bound) x CUs x 256
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    threads = (choice["blocks_per_cu"] or 4) * cus * 256
+    if rec.get("workload") != spec.name or str(threads) not in rec["by_threads"]:
+        return None
+    return {"valu_share": rec["by_threads"][str(threads)]["valu_share"], "source": str(sq.relative_to(ROOT))}
 
 
 def main():
@@ -242,10 +298,11 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=9)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="leave the replica batch out of the line")
+    ap.add_argument("--no-fm", action="store_true", help="leave force matching out of the line")
     ap.add_argument("--workload", choices=["langevin", "batch", "fm"], default="langevin",
-                    help="langevin: the per-step path at 1M beads; batch: 512 BD replicas (500..20000 beads) "
-                         "stepped together; fm: force-matching gradient evaluations (64 BD snapshots x 100k "
-                         "beads, 10 tables x 512 knots)")
+                    help="langevin: the per-step path at 1M beads (with the batch and force matching in the "
+                         "same line); batch: only the 512 BD replicas; fm: only force matching")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a streaming kernel of known size (reads and writes 1 GiB each), the "
                          "scale for hardware byte counters")
@@ -262,15 +319,18 @@ def main():
         copy_probe(torch)
         return
     if args.workload == "fm":
-        bench_fm(args, torch)
+        print(json.dumps(dict(bench_fm(torch, max(args.steps // 10, 10), max(args.warmup // 4, 2)), n_gpus=1,
+                              higher_is_better=True, scaling="weak", vs_baseline=None, data="synthetic (seeded BD frames)")))
         return
     if args.workload == "batch":
-        bench_batch(args, torch)
+        print(json.dumps(dict(bench_batch(torch, args.steps, args.warmup), n_gpus=1, higher_is_better=True,
+                              scaling="weak", vs_baseline=None, dtype="f32", data="synthetic (seeded)")))
         return
 
     import marsmd
     from marsmd import configs
 
+    copy_bw = copy_bandwidth(torch)
     spec = configs.config2(n=args.n, grid_n=args.grid)
     patch = marsmd.Patch(spec)
     stream = torch.cuda.current_stream()
@@ -280,22 +340,23 @@ def main():
     choice = patch.autotune(warmup)
     torch.cuda.synchronize()
 
-    # timed region (the value): `steps` steps in one run() call, bracketed by HIP
-    # events on the stream the library launches on, nothing instrumented inside
-    first = warmup
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    patch.run(args.steps, first_step=first)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    total_ms = e0.elapsed_time(e1)
+    # timed region (the value): REPEATS runs of `steps` steps, each one run() call
+    # bracketed by HIP events on the stream the library launches on, nothing
+    # instrumented inside; the median run is the value
+    step = [warmup]
+
+    def run(rep):
+        patch.run(args.steps, first_step=step[0])
+        step[0] += args.steps
+
+    times = timed_runs(torch, run, stream=stream)
+    total_ms = median(times)
     # the step kernel's duration: as many further steps with the library bracketing
     # every step's kernels (the fused step kernel, and the PMF kernel when split) with
     # HIP events on the same stream (the events add dispatch gaps, which is why this
     # pass is not the value's)
     patch.set_kernel_timing(True)
-    patch.run(args.steps, first_step=first + args.steps)
+    patch.run(args.steps, first_step=step[0])
     kernel_ms, launches = patch.kernel_timing()
     patch.set_kernel_timing(False)
     ms_step = total_ms / args.steps
@@ -310,17 +371,10 @@ def main():
             traffic, traffic_src = rec["hbm_bytes_per_launch"], str(pmc.relative_to(ROOT))
     # the step kernel is issue-bound on VALU work rather than on HBM: quote the measured
     # VALU share of its issue cycles (rocprofv3 SQ counters) for the launch shape kept
-    valu = None
-    sq = ROOT / "profiles" / "k_step_langevin_1M_grid256_sq.json"
-    if sq.exists() and not choice["split_pmf"]:
-        rec = json.loads(sq.read_text())
-        # the persistent step kernel's threads: workgroups per CU (0: all that fit, 4 at its
-        # 128-VGPR bound) x CUs x 256
-        cus = torch.cuda.get_device_properties(0).multi_processor_count
-        threads = (choice["blocks_per_cu"] or 4) * cus * 256
-        if rec.get("workload") == spec.name and threads is not None and str(threads) in rec["by_threads"]:
-            valu = {"valu_share": rec["by_threads"][str(threads)]["valu_share"], "source": str(sq.relative_to(ROOT))}
+    valu = valu_issue(torch, choice, spec)
     achieved = acc["step_kernel_bytes_per_launch"] / t_kernel
+    del patch
+    torch.cuda.synchronize()
     out = {
         "metric": "particle-steps/s at 1M beads",
         "value": value,
@@ -329,6 +383,8 @@ def main():
         "steps": args.steps,
         "warmup": warmup,
         "ms_per_step": ms_step,
+        "repeats": REPEATS,
+        "ms_per_step_runs": [t / args.steps for t in times],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -340,18 +396,29 @@ def main():
                    "rebuild_period": spec.params.rebuild_period},
         "hbm_roofline_frac_whole_step": value * acc["b_alg_per_particle_step"] / HBM_PEAK,
         "b_alg_per_particle_step": acc["b_alg_per_particle_step"],
+        "copy_bandwidth": copy_bw,
         "launch": {"blocks_per_cu": choice["blocks_per_cu"], "split_pmf": choice["split_pmf"],
                    "tuned_ms_per_step": choice["ms_per_step"]},
         "roofline": {"bound": "hbm",
+                     "limiter": "valu",
                      "kernel": ("k_pmf_force + k_step (grid force, then pair force + BAOAB)" if choice["split_pmf"]
                                 else "k_step (fused pair force + PMF + BAOAB)"),
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_unit": "B/launch",
+                     "frac": achieved / HBM_PEAK,
+                     "frac_of_measured_copy": achieved / (copy_bw["GBps"] * 1e9),
+                     "traffic": traffic, "traffic_unit": "B/launch",
                      "traffic_source": traffic_src,
                      "avg_launch_us": t_kernel * 1e6, "launches": launches,
                      "other_ms_per_step": ms_step - kernel_ms / max(launches, 1),
+                     "valu_frac": None if valu is None else valu["valu_share"],
                      "issue": valu},
     }
+    if not args.no_batch:
+        out["batch"] = bench_batch(torch, args.steps, args.warmup)
+        torch.cuda.synchronize()
+    if not args.no_fm:
+        out["fm"] = bench_fm(torch, max(args.steps // 2, 10), max(args.warmup // 4, 2))
+        torch.cuda.synchronize()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(spec, args.cpu_steps)
     else:

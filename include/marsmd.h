@@ -212,6 +212,17 @@ marsmd_status marsmd_zorder_reorder_data(marsmd_zorder* z, const void* in, void*
 /* ZOrderSort::remap_indices (RemapIndicesKernel ZOrderKernels.h:87-101). */
 marsmd_status marsmd_zorder_remap_indices(marsmd_zorder* z, int32_t* indices, uint64_t n,
                                           marsmd_stream stream);
+/* ZOrderSort::update_positions_incremental (ZOrderSort.h:172-184): keep a copy of n
+ * device positions (float4 records) as the reference of compute_max_displacement. */
+marsmd_status marsmd_zorder_update_positions_incremental(marsmd_zorder* z, const float* pos4, uint64_t n,
+                                                         marsmd_stream stream);
+/* ZOrderSort::compute_max_displacement (ZOrderSort.h:160-170): the largest distance a
+ * particle moved since the last update_positions_incremental (same n), minimum image on
+ * axes whose box_len is > 0 (box_len may be NULL: all axes open). Reads the result back
+ * (synchronises the stream); MARSMD_ERR_RUNTIME when no reference was kept. */
+marsmd_status marsmd_zorder_compute_max_displacement(marsmd_zorder* z, const float* pos4, uint64_t n,
+                                                     const float box_len[3], float* max_displacement,
+                                                     marsmd_stream stream);
 /* Stand-alone stable key/index sort used by the above (device keys in, sorted out):
  * orders by bits [0, key_bits) of the keys; the four buffers must not overlap
  * (MARSMD_ERR_VALUE otherwise). */
@@ -221,7 +232,6 @@ marsmd_status marsmd_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* v
                                       marsmd_stream stream);
 /* Scratch bytes marsmd_radix_sort_pairs needs for n keys. */
 uint64_t marsmd_radix_sort_scratch_bytes(uint64_t n);
-
 /* ------------------------------------------------------------------ pairlist
  * Replaces ZOrderPairlist (src/PatchOperation/PairListKernels/ZOrderPairlist.h:27-190,
  * ZOrderPairlist.cpp:102-411; ZOrderNeighbor.h:27-225) behind Pairlist
@@ -439,6 +449,9 @@ marsmd_status marsmd_patch_integrate_motion(marsmd_patch* p, float dt, float kT,
                                             marsmd_stream stream);
 /* Patch::finish_deferred_kick (Patch.cpp:354-373). */
 marsmd_status marsmd_patch_finish_deferred_kick(marsmd_patch* p, float dt, marsmd_stream stream);
+/* Patch::has_deferred_kick (Patch.h:300-304): 1 while a BAOAB closing half-kick is
+ * outstanding (after a Langevin step, until finish_deferred_kick or the next step). */
+marsmd_status marsmd_patch_has_deferred_kick(marsmd_patch* p, int32_t* pending);
 /* Patch::reorder_particles (Patch.cpp:456-480) via ParticleReorderManager. */
 marsmd_status marsmd_patch_reorder_particles(marsmd_patch* p, marsmd_stream stream);
 /* Patch::get_pairlist(). */
@@ -555,6 +568,28 @@ marsmd_status marsmd_batch_get_launch(marsmd_batch* b, marsmd_batch_launch* laun
 marsmd_status marsmd_batch_autotune(marsmd_batch* b, const marsmd_run_params* params, uint64_t first_step,
                                     uint64_t num_steps, marsmd_batch_launch* best, double* best_ms_per_step,
                                     marsmd_stream stream);
+/* Cell configuration of the batch's pairlist walk per replica size bucket (bucket b:
+ * 2^b <= n < 2^(b+1) particles): the home cells (one wave's tile of home particles)
+ * are the finest Morton cells that still hold home_target[b] particles on average,
+ * the candidate cells lie fine_offset[b] Morton levels below them. The pair set does
+ * not depend on it; the order of neighbours in a row, and so the FP32 summation order
+ * of the forces, does. Defaults: 24 and 2. No counterpart in the reference (its cell
+ * grid is ZOrderPairlist::select_cell_grid's, ZOrderPairlist.cpp:24-81). */
+#define MARSMD_BATCH_BUCKETS 32
+typedef struct marsmd_batch_cells {
+    int32_t home_target[MARSMD_BATCH_BUCKETS]; /* 1 .. 4096 */
+    int32_t fine_offset[MARSMD_BATCH_BUCKETS]; /* 1 .. 4    */
+} marsmd_batch_cells;
+marsmd_status marsmd_batch_set_cells(marsmd_batch* b, const marsmd_batch_cells* cells);
+marsmd_status marsmd_batch_get_cells(marsmd_batch* b, marsmd_batch_cells* cells);
+/* Runs first_step+1 .. first_step+num_steps as marsmd_batch_run, after choosing the
+ * cell configuration bucket by bucket (candidates home_target 8/16/24/48/96 x
+ * fine_offset 1/2/3, each timed on the cell tables and walk of the last build's
+ * positions into scratch buffers, the other buckets held); the choice applies from
+ * the next build on. *walk_ms: the chosen configuration's walk time. */
+marsmd_status marsmd_batch_autotune_cells(marsmd_batch* b, const marsmd_run_params* params,
+                                          uint64_t first_step, uint64_t num_steps, double* walk_ms,
+                                          marsmd_stream stream);
 
 /* ------------------------------------------------------------------ force matching
  * No counterpart in the reference, which has no gradient path: the forces of a fixed

@@ -125,7 +125,8 @@ __global__ void k_b_gather(const uint32_t* __restrict__ old_slot, uint64_t total
 __global__ void k_b_cell_ranges(const uint32_t* __restrict__ codes, uint64_t total,
 								const uint32_t* __restrict__ rep_of_slot, const RepGeom* __restrict__ geom,
 								uint32_t* __restrict__ fine_begin, uint32_t* __restrict__ fine_end,
-								uint32_t* __restrict__ home_begin, uint32_t* __restrict__ home_end) {
+								uint32_t* __restrict__ home_begin, uint32_t* __restrict__ home_end,
+								uint32_t* __restrict__ flags) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= total)
 		return;
@@ -133,6 +134,10 @@ __global__ void k_b_cell_ranges(const uint32_t* __restrict__ codes, uint64_t tot
 	const bool first = i == G.slot0;
 	const bool last = i + 1 == uint64_t(G.slot0) + G.n;
 	const uint32_t c = codes[i];
+	// the gather recomputes each code from the position it moves: codes out of order
+	// within a replica (a mismatch with the sort's keys) would give wrong cell ranges
+	if (!first && codes[i - 1] > c && flags)
+		atomicOr(flags, 2u);
 	const int fs = 3 * (10 - G.mf), hs = 3 * (10 - G.h);
 	const uint32_t fc = c >> fs, hc = c >> hs;
 	if (first || (codes[i - 1] >> fs) != fc)
@@ -555,6 +560,9 @@ BatchImpl::~BatchImpl() {
 		(void)hipStreamDestroy(capture_stream);
 	for (hipEvent_t e : timing_events)
 		(void)hipEventDestroy(e);
+	for (hipEvent_t e : t_events)
+		if (e)
+			(void)hipEventDestroy(e);
 	delete tables;
 }
 
@@ -636,15 +644,24 @@ void BatchImpl::copy_to_host(float* pos4, float* mom4, float* force4, int32_t* t
 // Sort every replica (stable by Morton key of its box, ties by canonical index), move
 // the state into the new slot order (relabelling canonical indices when reordering),
 // then cell tables and the home-cell walk of all replicas, with one read-back.
-void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
-	const float rc = pairlist_cutoff;
+int batch_size_bucket(uint64_t n) {
+	int b = 0;
+	while (b + 1 < kBatchBuckets && (uint64_t(1) << (b + 1)) <= n)
+		++b;
+	return b;
+}
+
+// Per-replica geometry of a build with the cell configuration of each replica's size
+// bucket: home cells of the largest level h <= m - 1 holding at least `home_target`
+// particles on average, fine cells `fine_offset` levels below (at most level 10).
+void BatchImpl::plan_geometry(float rc, std::vector<uint8_t>& gh, std::vector<uint32_t>& items,
+							  uint64_t& fine_total, uint64_t& home_total) {
 	const float rc2 = rc * rc;
-	// per-replica geometry
-	geom_host.assign(R * sizeof(RepGeom), 0);
-	RepGeom* G = reinterpret_cast<RepGeom*>(geom_host.data());
-	items_host.assign(R + 1, 0);
-	std::vector<uint32_t>& items = items_host;
-	uint64_t fine_total = 0, home_total = 0;
+	gh.assign(R * sizeof(RepGeom), 0);
+	RepGeom* G = reinterpret_cast<RepGeom*>(gh.data());
+	items.assign(R + 1, 0);
+	fine_total = 0;
+	home_total = 0;
 	for (uint64_t r = 0; r < R; ++r) {
 		RepGeom& g = G[r];
 		const uint64_t n = count[r];
@@ -652,14 +669,16 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 		g.slot0 = uint32_t(off[r]);
 		g.n = uint32_t(n);
 		g.pid = pid[r];
+		const int bucket = batch_size_bucket(n);
+		const uint64_t target = uint64_t(std::max(1, cell_target[size_t(bucket)]));
 		int m = 0;
 		while (m < 10 && (uint64_t(1) << (3 * (m + 1))) <= 2 * std::max<uint64_t>(n, 1))
 			++m;
 		int h = 0;
-		while (h + 1 <= m - 1 && (n >> (3 * (h + 1))) >= 24)
+		while (h + 1 <= m - 1 && (n >> (3 * (h + 1))) >= target)
 			++h;
 		g.h = h;
-		g.mf = std::min(10, h + 2);
+		g.mf = std::min(10, h + std::max(1, cell_fine[size_t(bucket)]));
 		const int fine_cells = 1 << g.mf;
 		const int span = 1 << (g.mf - h);
 		const int k = 1024 / fine_cells;
@@ -715,15 +734,23 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 	}
 	if (fine_total >= (uint64_t(1) << 32))
 		raise(MARSMD_ERR_VALUE, "batch: cell tables exceed 2^32 entries");
-	num_fine = fine_total;
-	num_home = home_total;
-	num_items = items[R];
 	if (stencils_host.size() > stencils_uploaded) { // new geometries (a synchronous upload, rare)
 		stencils.resize(stencils_host.size());
 		MMD_HIP(hipMemcpy(stencils.data(), stencils_host.data(), sizeof(int4) * stencils_host.size(),
 						  hipMemcpyHostToDevice));
 		stencils_uploaded = stencils_host.size();
 	}
+}
+
+void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
+	const float rc = pairlist_cutoff;
+	const float rc2 = rc * rc;
+	std::vector<uint32_t>& items = items_host;
+	uint64_t fine_total = 0, home_total = 0;
+	plan_geometry(rc, geom_host, items, fine_total, home_total);
+	num_fine = fine_total;
+	num_home = home_total;
+	num_items = items[R];
 	geom.resize(R * sizeof(RepGeom));
 	MMD_HIP(hipMemcpyAsync(geom.data(), geom_host.data(), R * sizeof(RepGeom), hipMemcpyHostToDevice, s));
 	item_off.resize(R + 1);
@@ -772,7 +799,7 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 	fill_u32(home_end.data(), 0u, num_home, s);
 	k_b_cell_ranges<<<grid_for(total, 256), 256, 0, s>>>(codes.data(), total, rep_of_slot.data(), gd,
 														 fine_begin.data(), fine_end.data(), home_begin.data(),
-														 home_end.data());
+														 home_end.data(), scalars.data());
 	MMD_LAUNCH_CHECK();
 	// walk with uniform rows of `cap` entries; once more with larger rows if needed
 	const uint64_t tiles = (total + 63) / 64;
@@ -806,8 +833,10 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 		uint32_t host[6] = {0, 0, 0, 0, 0, 0};
 		MMD_HIP(hipMemcpyAsync(host, scalars.data(), 24, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipStreamSynchronize(s));
-		if (host[0] != 0)
+		if (host[0] & 1u)
 			raise(MARSMD_ERR_VALUE, "batch: a particle left its replica's box (positions must stay wrapped)");
+		if (host[0] & 2u)
+			raise(MARSMD_ERR_RUNTIME, "batch: Morton codes after the sort were not in order");
 		max_neighbors = host[2];
 		std::memcpy(&nbr_entries, &host[4], 8);
 		if (max_neighbors <= cap)
@@ -1083,6 +1112,121 @@ void BatchImpl::autotune(const marsmd_run_params& P, uint64_t first_step, uint64
 	if (step < first_step + num_steps)
 		run(P, step, first_step + num_steps - step, s);
 	*best_ms = best->ms < 1e300 ? best->ms : 0.0;
+}
+
+void BatchImpl::set_cells(const marsmd_batch_cells& c) {
+	for (int b = 0; b < kBatchBuckets; ++b) {
+		if (c.home_target[b] < 1 || c.home_target[b] > 4096)
+			raise(MARSMD_ERR_VALUE, "batch: home_target must be in [1, 4096]");
+		if (c.fine_offset[b] < 1 || c.fine_offset[b] > 4)
+			raise(MARSMD_ERR_VALUE, "batch: fine_offset must be in [1, 4]");
+	}
+	for (int b = 0; b < kBatchBuckets; ++b) {
+		cell_target[size_t(b)] = c.home_target[b];
+		cell_fine[size_t(b)] = c.fine_offset[b];
+	}
+}
+
+void BatchImpl::get_cells(marsmd_batch_cells* c) const {
+	for (int b = 0; b < kBatchBuckets; ++b) {
+		c->home_target[b] = cell_target[size_t(b)];
+		c->fine_offset[b] = cell_fine[size_t(b)];
+	}
+}
+
+double BatchImpl::walk_ms(hipStream_t s) {
+	std::vector<uint8_t> gh;
+	std::vector<uint32_t> items;
+	uint64_t ft = 0, ht = 0;
+	plan_geometry(built_cutoff, gh, items, ft, ht);
+	t_geom.resize(R * sizeof(RepGeom));
+	MMD_HIP(hipMemcpyAsync(t_geom.data(), gh.data(), R * sizeof(RepGeom), hipMemcpyHostToDevice, s));
+	t_items.resize(R + 1);
+	MMD_HIP(hipMemcpyAsync(t_items.data(), items.data(), 4 * (R + 1), hipMemcpyHostToDevice, s));
+	t_fine_begin.resize(ft);
+	t_fine_end.resize(ft);
+	t_home_begin.resize(ht);
+	t_home_end.resize(ht);
+	t_counts.resize(total);
+	const uint64_t storage = (total + 63) / 64 * 64 * uint64_t(cap);
+	t_nbr.resize(std::max<uint64_t>(storage, 1));
+	if (!t_events[0]) {
+		MMD_HIP(hipEventCreate(&t_events[0]));
+		MMD_HIP(hipEventCreate(&t_events[1]));
+	}
+	const RepGeom* gd = reinterpret_cast<const RepGeom*>(t_geom.data());
+	const uint64_t nitems = items[R];
+	const unsigned groups = unsigned((nitems + kHomeWaves - 1) / kHomeWaves);
+	const unsigned grid = std::max(8u, (groups + 7u) / 8u * 8u);
+	MMD_HIP(hipEventRecord(t_events[0], s));
+	fill_u32(t_fine_begin.data(), 0u, ft, s);
+	fill_u32(t_fine_end.data(), 0u, ft, s);
+	fill_u32(t_home_begin.data(), 0u, ht, s);
+	fill_u32(t_home_end.data(), 0u, ht, s);
+	k_b_cell_ranges<<<grid_for(total, 256), 256, 0, s>>>(codes.data(), total, rep_of_slot.data(), gd,
+														 t_fine_begin.data(), t_fine_end.data(), t_home_begin.data(),
+														 t_home_end.data(), nullptr);
+	MMD_LAUNCH_CHECK();
+	k_b_walk<<<grid, 256, 0, s>>>(gd, t_items.data(), uint32_t(R), uint32_t(nitems), snap.data(), t_home_begin.data(),
+								  t_home_end.data(), t_fine_begin.data(), t_fine_end.data(),
+								  built_cutoff * built_cutoff, cap, stencils.data(), t_counts.data(), t_nbr.data());
+	MMD_LAUNCH_CHECK();
+	MMD_HIP(hipEventRecord(t_events[1], s));
+	MMD_HIP(hipEventSynchronize(t_events[1]));
+	float ms = 0.0f;
+	MMD_HIP(hipEventElapsedTime(&ms, t_events[0], t_events[1]));
+	return double(ms);
+}
+
+void BatchImpl::autotune_cells(const marsmd_run_params& P, uint64_t first_step, uint64_t num_steps, hipStream_t s,
+							   double* walk_out) {
+	uint64_t step = first_step;
+	if (!built && num_steps > 0) { // the first build
+		run(P, step, 1, s);
+		++step;
+	}
+	if (!built)
+		raise(MARSMD_ERR_RUNTIME, "batch: the cell autotuner needs a first build (num_steps >= 1)");
+	std::vector<int> present;
+	for (uint64_t r = 0; r < R; ++r) {
+		const int b = batch_size_bucket(count[r]);
+		if (std::find(present.begin(), present.end(), b) == present.end())
+			present.push_back(b);
+	}
+	std::sort(present.begin(), present.end());
+	// the fastest of three timings of one configuration
+	const auto measure = [&]() {
+		double best = 1e300;
+		for (int k = 0; k < 3; ++k)
+			best = std::min(best, walk_ms(s));
+		return best;
+	};
+	(void)walk_ms(s); // warm the scratch buffers and stencils
+	double best = measure();
+	const int targets[] = {8, 16, 24, 48, 96};
+	const int fines[] = {1, 2, 3};
+	// one bucket at a time, the others held; a candidate must beat the incumbent by 1%
+	for (int b : present) {
+		int bt = cell_target[size_t(b)], bf = cell_fine[size_t(b)];
+		for (int t : targets)
+			for (int f : fines) {
+				if (t == bt && f == bf)
+					continue;
+				cell_target[size_t(b)] = t;
+				cell_fine[size_t(b)] = f;
+				const double ms = measure();
+				if (ms < 0.99 * best) {
+					best = ms;
+					bt = t;
+					bf = f;
+				}
+			}
+		cell_target[size_t(b)] = bt;
+		cell_fine[size_t(b)] = bf;
+	}
+	*walk_out = best < 1e300 ? best : 0.0;
+	if (step < first_step + num_steps)
+		run(P, step, first_step + num_steps - step, s);
 }
 
 void BatchImpl::finish_deferred_kick(float dt, hipStream_t s) {

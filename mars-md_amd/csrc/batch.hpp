@@ -11,6 +11,11 @@ namespace mmd {
 struct NbTables;
 struct StepParams;
 
+// Size buckets of the per-bucket cell configuration: bucket b holds the replicas with
+// 2^b <= n < 2^(b+1) particles.
+constexpr int kBatchBuckets = MARSMD_BATCH_BUCKETS;
+int batch_size_bucket(uint64_t n);
+
 struct BatchImpl {
 	uint64_t R = 0, total = 0, base_seed = 0;
 	std::vector<uint64_t> off, count, pid;
@@ -66,6 +71,14 @@ struct BatchImpl {
 	std::vector<SegmentGraph> graphs;
 	hipStream_t capture_stream = nullptr;
 	DevArray<uint32_t> step_word;
+	// cell configuration per size bucket: the walk's home cells hold at least
+	// cell_target particles on average; its fine cells lie cell_fine levels below
+	std::vector<int> cell_target = std::vector<int>(size_t(kBatchBuckets), 24);
+	std::vector<int> cell_fine = std::vector<int>(size_t(kBatchBuckets), 2);
+	// scratch of the walk timing (the live list is untouched)
+	DevArray<uint8_t> t_geom;
+	DevArray<uint32_t> t_items, t_fine_begin, t_fine_end, t_home_begin, t_home_end, t_counts, t_nbr;
+	hipEvent_t t_events[2] = {nullptr, nullptr};
 
 	explicit BatchImpl(const marsmd_batch_desc& d);
 	~BatchImpl();
@@ -85,8 +98,18 @@ struct BatchImpl {
 	// of steps, and keeps the fastest (its ms per step in *best_ms).
 	void autotune(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s,
 				  double* best_ms);
+	void set_cells(const marsmd_batch_cells& c);
+	void get_cells(marsmd_batch_cells* c) const;
+	// Steps num_steps real steps after choosing, bucket by bucket, the cell
+	// configuration whose cell tables and walk over the last build's positions take
+	// least time (*walk_ms: that time); the choice applies from the next build on.
+	void autotune_cells(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s,
+						double* walk_ms);
 
   private:
+	void plan_geometry(float rc, std::vector<uint8_t>& gh, std::vector<uint32_t>& items, uint64_t& fine_total,
+					   uint64_t& home_total);
+	double walk_ms(hipStream_t s);
 	void build(float pairlist_cutoff, bool relabel, hipStream_t s);
 	bool needs_rebuild(float skin, hipStream_t s);
 	unsigned step_grid(const void* kernel);

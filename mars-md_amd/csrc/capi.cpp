@@ -104,6 +104,27 @@ marsmd_status marsmd_zorder_remap_indices(marsmd_zorder* z, int32_t* idx, uint64
 		z->impl.remap_indices(idx, n, st(stream));
 	});
 }
+marsmd_status marsmd_zorder_update_positions_incremental(marsmd_zorder* z, const float* pos4, uint64_t n,
+														 marsmd_stream stream) {
+	return guard([&] {
+		need(z, "sorter");
+		if (n)
+			need(pos4, "positions");
+		z->impl.update_positions_incremental(reinterpret_cast<const float4*>(pos4), n, st(stream));
+	});
+}
+marsmd_status marsmd_zorder_compute_max_displacement(marsmd_zorder* z, const float* pos4, uint64_t n,
+													 const float box_len[3], float* max_displacement,
+													 marsmd_stream stream) {
+	return guard([&] {
+		need(z, "sorter");
+		need(max_displacement, "max_displacement");
+		if (n)
+			need(pos4, "positions");
+		*max_displacement =
+			z->impl.compute_max_displacement(reinterpret_cast<const float4*>(pos4), n, box_len, st(stream));
+	});
+}
 marsmd_status marsmd_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
 									  uint32_t* vals_out, uint64_t n, int32_t key_bits, void* scratch,
 									  uint64_t scratch_bytes, marsmd_stream stream) {
@@ -439,6 +460,13 @@ marsmd_status marsmd_patch_finish_deferred_kick(marsmd_patch* p, float dt, marsm
 		p->impl.finish_deferred_kick(dt, st(stream));
 	});
 }
+marsmd_status marsmd_patch_has_deferred_kick(marsmd_patch* p, int32_t* pending) {
+	return guard([&] {
+		need(p, "patch");
+		need(pending, "pending");
+		*pending = p->impl.deferred_kick ? 1 : 0;
+	});
+}
 marsmd_status marsmd_patch_reorder_particles(marsmd_patch* p, marsmd_stream stream) {
 	return guard([&] {
 		need(p, "patch");
@@ -622,6 +650,29 @@ marsmd_status marsmd_batch_autotune(marsmd_batch* b, const marsmd_run_params* pa
 		b->impl.autotune(*params, first_step, num_steps, st(stream), best_ms_per_step);
 		best->blocks_per_cu = b->impl.blocks_per_cu;
 		best->use_graph = b->impl.use_graph ? 1 : 0;
+	});
+}
+marsmd_status marsmd_batch_set_cells(marsmd_batch* b, const marsmd_batch_cells* cells) {
+	return guard([&] {
+		need(b, "batch");
+		need(cells, "cells");
+		b->impl.set_cells(*cells);
+	});
+}
+marsmd_status marsmd_batch_get_cells(marsmd_batch* b, marsmd_batch_cells* cells) {
+	return guard([&] {
+		need(b, "batch");
+		need(cells, "cells");
+		b->impl.get_cells(cells);
+	});
+}
+marsmd_status marsmd_batch_autotune_cells(marsmd_batch* b, const marsmd_run_params* params, uint64_t first_step,
+										  uint64_t num_steps, double* walk_ms, marsmd_stream stream) {
+	return guard([&] {
+		need(b, "batch");
+		need(params, "params");
+		need(walk_ms, "walk_ms");
+		b->impl.autotune_cells(*params, first_step, num_steps, st(stream), walk_ms);
 	});
 }
 

@@ -980,6 +980,41 @@ float PairlistImpl::max_displacement(const float4* pos, const uint32_t* sorted_o
 	return std::sqrt(d2);
 }
 
+void ZSort::update_positions_incremental(const float4* pos, uint64_t nn, hipStream_t s) {
+	if (nn > max_particles)
+		raise(MARSMD_ERR_VALUE, "ZOrderSort: " + std::to_string(nn) + " positions exceed the capacity of " +
+									std::to_string(max_particles));
+	disp_ref.resize(std::max<uint64_t>(nn, 1));
+	if (nn)
+		MMD_HIP(hipMemcpyAsync(disp_ref.data(), pos, 16 * nn, hipMemcpyDeviceToDevice, s));
+	disp_n = nn;
+	disp_valid = true;
+}
+
+float ZSort::compute_max_displacement(const float4* pos, uint64_t nn, const float box_len[3], hipStream_t s) {
+	if (!disp_valid)
+		raise(MARSMD_ERR_RUNTIME, "ZOrderSort: compute_max_displacement before update_positions_incremental");
+	if (nn != disp_n)
+		raise(MARSMD_ERR_VALUE, "ZOrderSort: " + std::to_string(nn) + " positions against a reference of " +
+									std::to_string(disp_n));
+	if (nn == 0)
+		return 0.0f;
+	disp_bits.resize(1);
+	fill_u32(disp_bits.data(), 0u, 1, s);
+	const float3 plen = make_float3(box_len && box_len[0] > 0.0f ? box_len[0] : 0.0f,
+									box_len && box_len[1] > 0.0f ? box_len[1] : 0.0f,
+									box_len && box_len[2] > 0.0f ? box_len[2] : 0.0f);
+	k_max_disp<<<std::min<unsigned>(grid_for(nn, 256), 1024), 256, 0, s>>>(pos, disp_ref.data(), nullptr, nn, plen,
+																			 disp_bits.data());
+	MMD_LAUNCH_CHECK();
+	uint32_t bits = 0;
+	MMD_HIP(hipMemcpyAsync(&bits, disp_bits.data(), 4, hipMemcpyDeviceToHost, s));
+	MMD_HIP(hipStreamSynchronize(s));
+	float d2;
+	std::memcpy(&d2, &bits, 4);
+	return std::sqrt(d2);
+}
+
 const int32_t* PairlistImpl::neighbor_pairs(hipStream_t s) {
 	if (pairs_valid)
 		return pairs.data();

@@ -39,6 +39,17 @@ struct ZSort {
 	void reorder_data(const void* in, void* out, uint64_t n, uint32_t elem, hipStream_t s) const;
 	// RemapIndicesKernel (ZOrderKernels.h:87-101).
 	void remap_indices(int32_t* idx, uint64_t count, hipStream_t s) const;
+	// ZOrderSort::update_positions_incremental (ZOrderSort.h:172-184): keep a copy of
+	// the positions as the reference of compute_max_displacement.
+	void update_positions_incremental(const float4* pos, uint64_t n, hipStream_t s);
+	// ZOrderSort::compute_max_displacement (ZOrderSort.h:160-170): the largest distance
+	// any particle moved since that copy, minimum image on axes with box_len > 0.
+	// Reads the result back (synchronises the stream).
+	float compute_max_displacement(const float4* pos, uint64_t n, const float box_len[3], hipStream_t s);
+	DevArray<float4> disp_ref;
+	DevArray<uint32_t> disp_bits;
+	uint64_t disp_n = 0;
+	bool disp_valid = false;
 };
 
 // Full neighbour list in wave tiles. Slots 64w .. 64w+63 form tile w, whose rows hold

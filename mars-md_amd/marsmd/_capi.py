@@ -188,6 +188,13 @@ class BatchLaunch(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int32), ("use_graph", C.c_int32)]
 
 
+BATCH_BUCKETS = 32
+
+
+class BatchCells(C.Structure):
+    _fields_ = [("home_target", C.c_int32 * BATCH_BUCKETS), ("fine_offset", C.c_int32 * BATCH_BUCKETS)]
+
+
 class PatchLaunch(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int32), ("split_pmf", C.c_int32)]
 
@@ -241,6 +248,8 @@ PROTOTYPES = {
     "marsmd_zorder_get_buffers": (_I32, [_P, _PP, _PP, _PP]),
     "marsmd_zorder_reorder_data": (_I32, [_P, _P, _P, _U64, _U32, _P]),
     "marsmd_zorder_remap_indices": (_I32, [_P, _P, _U64, _P]),
+    "marsmd_zorder_update_positions_incremental": (_I32, [_P, _P, _U64, _P]),
+    "marsmd_zorder_compute_max_displacement": (_I32, [_P, _P, _U64, _P, _P, _P]),
     "marsmd_radix_sort_pairs": (_I32, [_P, _P, _P, _P, _U64, _I32, _P, _U64, _P]),
     "marsmd_radix_sort_scratch_bytes": (_U64, [_U64]),
     "marsmd_pairlist_create": (_I32, [_U64, _U64, _PP]),
@@ -282,6 +291,7 @@ PROTOTYPES = {
     "marsmd_patch_calculate_bonded_forces": (_I32, [_P, _I32, _P]),
     "marsmd_patch_integrate_motion": (_I32, [_P, _F, _F, _I32, _U64, _P]),
     "marsmd_patch_finish_deferred_kick": (_I32, [_P, _F, _P]),
+    "marsmd_patch_has_deferred_kick": (_I32, [_P, _P]),
     "marsmd_patch_reorder_particles": (_I32, [_P, _P]),
     "marsmd_patch_get_pairlist": (_I32, [_P, _PP]),
     "marsmd_patch_run": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, _P]),
@@ -305,6 +315,9 @@ PROTOTYPES = {
     "marsmd_batch_get_launch": (_I32, [_P, C.POINTER(BatchLaunch)]),
     "marsmd_batch_autotune": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, C.POINTER(BatchLaunch),
                                      C.POINTER(C.c_double), _P]),
+    "marsmd_batch_set_cells": (_I32, [_P, C.POINTER(BatchCells)]),
+    "marsmd_batch_get_cells": (_I32, [_P, C.POINTER(BatchCells)]),
+    "marsmd_batch_autotune_cells": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, C.POINTER(C.c_double), _P]),
     "marsmd_fm_create": (_I32, [C.POINTER(FMDesc), C.POINTER(_P)]),
     "marsmd_fm_destroy": (_I32, [_P]),
     "marsmd_fm_num_pairs": (_I32, [_P, C.POINTER(_U64)]),

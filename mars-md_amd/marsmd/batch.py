@@ -155,6 +155,64 @@ class Batch:
         env = os.environ.get("MARSMD_AUTOTUNE_CACHE")
         return Path(env) if env else Path.home() / ".cache" / "marsmd" / "autotune.json"
 
+    # ------------------------------------------------------------------ cell configuration
+    @staticmethod
+    def size_bucket(n: int) -> int:
+        """The size bucket of a replica of n particles: 2^b <= n < 2^(b+1)."""
+        return max(0, min(api.BATCH_BUCKETS - 1, int(n).bit_length() - 1))
+
+    def set_cells(self, cells: dict):
+        """Cell configuration per size bucket, {bucket: (home_target, fine_offset)}; the
+        buckets not named keep theirs. Applies from the next build."""
+        cur = self.get_cells(all_buckets=True)
+        cur.update({int(b): tuple(v) for b, v in cells.items()})
+        c = api.BatchCells()
+        for b in range(api.BATCH_BUCKETS):
+            c.home_target[b], c.fine_offset[b] = cur[b]
+        api.call("marsmd_batch_set_cells", self._h, C.byref(c))
+
+    def get_cells(self, all_buckets: bool = False) -> dict:
+        """{bucket: (home_target, fine_offset)} for the buckets this batch's replicas fall
+        in (every bucket with all_buckets)."""
+        c = api.BatchCells()
+        api.call("marsmd_batch_get_cells", self._h, C.byref(c))
+        buckets = range(api.BATCH_BUCKETS) if all_buckets else sorted({self.size_bucket(n) for n in self.counts})
+        return {b: (c.home_target[b], c.fine_offset[b]) for b in buckets}
+
+    def autotune_cells(self, num_steps: int, first_step: int = 0, params: Optional[RunParams] = None, stream=None,
+                       cache=None, use_cache: bool = True) -> dict:
+        """Steps first_step+1 .. first_step+num_steps after choosing the walk's cell
+        configuration per size bucket (marsmd_batch_autotune_cells), or with the cached
+        choice for this workload. Returns {"cells", "walk_ms", "cached"}."""
+        key = self.workload_key(params) + "|cells"
+        path = None if cache is False else self._cache_path(cache)
+        table = {}
+        if path is not None and path.exists():
+            try:
+                table = json.loads(path.read_text())
+            except (OSError, ValueError):
+                table = {}
+        if use_cache and key in table:
+            hit = table[key]
+            self.set_cells({int(b): tuple(v) for b, v in hit["cells"].items()})
+            self.run(num_steps, first_step, params, stream)
+            return {"cells": self.get_cells(), "walk_ms": hit["walk_ms"], "cached": True}
+        p = (params or self.specs[0].params).to_c()
+        ms = C.c_double(0.0)
+        api.call("marsmd_batch_autotune_cells", self._h, C.byref(p), first_step, num_steps, C.byref(ms),
+                 _stream(stream))
+        cells = self.get_cells()
+        if ms.value > 0.0 and path is not None:
+            table[key] = {"cells": {str(b): list(v) for b, v in cells.items()}, "walk_ms": ms.value}
+            try:
+                path.parent.mkdir(parents=True, exist_ok=True)
+                tmp = path.with_suffix(".tmp")
+                tmp.write_text(json.dumps(table, indent=1, sort_keys=True))
+                tmp.replace(path)
+            except OSError:
+                pass  # an unwritable cache only costs the next batch a tuning run
+        return {"cells": cells, "walk_ms": ms.value, "cached": False}
+
     def autotune(self, num_steps: int, first_step: int = 0, params: Optional[RunParams] = None, stream=None,
                  cache=None, use_cache: bool = True) -> dict:
         """Steps first_step+1 .. first_step+num_steps (real steps, as run) while timing

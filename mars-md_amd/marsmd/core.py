@@ -274,6 +274,20 @@ class ZOrderSort:
     def remap_indices(self, indices, stream=None):
         api.call("marsmd_zorder_remap_indices", self._h, _ptr(indices), indices.numel(), _stream(stream))
 
+    def update_positions_incremental(self, positions, num_particles: int, stream=None):
+        """Keep these device positions (float4 records) as the displacement reference."""
+        api.call("marsmd_zorder_update_positions_incremental", self._h, _ptr(positions), num_particles,
+                 _stream(stream))
+
+    def compute_max_displacement(self, positions, num_particles: int, box_len=(0.0, 0.0, 0.0), stream=None) -> float:
+        """Largest displacement since update_positions_incremental; minimum image on
+        axes with box_len > 0."""
+        bl = (C.c_float * 3)(*map(float, box_len))
+        out = C.c_float(0.0)
+        api.call("marsmd_zorder_compute_max_displacement", self._h, _ptr(positions), num_particles, bl, C.byref(out),
+                 _stream(stream))
+        return out.value
+
 
 # ---------------------------------------------------------------------------- pairlist
 @dataclass

@@ -71,6 +71,12 @@ class Patch:
     def finish_deferred_kick(self, dt: float, stream=None):
         api.call("marsmd_patch_finish_deferred_kick", self._h, float(dt), _stream(stream))
 
+    def has_deferred_kick(self) -> bool:
+        """Patch::has_deferred_kick (Patch.h:300-304)."""
+        v = C.c_int32(0)
+        api.call("marsmd_patch_has_deferred_kick", self._h, C.byref(v))
+        return bool(v.value)
+
     def reorder_particles(self, stream=None):
         api.call("marsmd_patch_reorder_particles", self._h, _stream(stream))
 
@@ -82,6 +88,8 @@ class Patch:
     def run(self, num_steps: int, first_step: int = 0, params: Optional[RunParams] = None, stream=None):
         p = (params or self.spec.params).to_c()
         api.call("marsmd_patch_run", self._h, C.byref(p), first_step, num_steps, _stream(stream))
+        if num_steps > 0:
+            self._prepared = True  # the first step prepared the bonded topology
 
     def set_kernel_timing(self, enable: bool = True):
         """Record HIP events around each fused step kernel launched by run()."""

@@ -67,6 +67,7 @@ STRUCTS = {
     "marsmd_fm_desc": api.FMDesc,
     "marsmd_batch_desc": api.BatchDesc,
     "marsmd_batch_launch": api.BatchLaunch,
+    "marsmd_batch_cells": api.BatchCells,
     "marsmd_patch_launch": api.PatchLaunch,
     "marsmd_pair_params": api.PairParams,
 }
@@ -118,3 +119,4 @@ def test_batch_rejects_replicas_with_different_seeds():
     b = configs.config0(n=120, L=60.0, seed=2)
     with pytest.raises(ValueError, match="base seed"):
         Batch([a, b])
+

@@ -13,7 +13,10 @@ that replica's patch id.
   same bits as the default eager launch, checked after each of several runs with host
   work and synchronisation between the replays, for BD and Langevin; the autotuner
   steps the same trajectory and its cached choice is reused;
-* particles outside their box are rejected.
+* particles outside their box are rejected;
+* the walk's cell configuration per replica size bucket keeps every pair set (forces
+  within 1e-5: only their FP32 summation order changes), and its autotuner picks one
+  per bucket present, caches it, and a first build with it gives the oracle's pair sets.
 """
 import numpy as np
 import pytest
@@ -163,3 +166,46 @@ def test_particles_outside_their_box_are_rejected(cuda):
     specs[1].positions[3, 0] = specs[1].box.size[0] + 5.0
     with pytest.raises(marsmd.MarsError):
         Batch(specs)
+
+
+def test_cell_configuration_per_size_bucket(cuda, tmp_path):
+    # the walk's cells per size bucket change the order of neighbours in a row (so the
+    # FP32 summation order) but not the pair set
+    specs = _specs()
+    ref = Batch(specs)
+    ref.run(1)
+    b = Batch(specs)
+    buckets = b.get_cells()
+    assert set(buckets) == {Batch.size_bucket(s.n) for s in specs}
+    assert all(v == (24, 2) for v in buckets.values())
+    b.set_cells({bk: (8, 3) for bk in buckets})
+    assert all(v == (8, 3) for v in b.get_cells().values())
+    b.run(1)
+    cuda.cuda.synchronize()
+    got, want = b.copy_particles_to_host(), ref.copy_particles_to_host()
+    for r, s in enumerate(specs):
+        print(f"case: cells (8, 3), replica {r} n={s.n}", flush=True)
+        assert np.array_equal(sorted_pairs(b.get_neighbor_pairs(r)), sorted_pairs(ref.get_neighbor_pairs(r)))
+        f, fr = got[r]["force"][:, :3].astype(np.float64), want[r]["force"][:, :3].astype(np.float64)
+        assert np.abs(f - fr).max() <= 1e-5 * np.abs(fr).max(), f"replica {r}"
+    with pytest.raises(marsmd.MarsError):
+        b.set_cells({min(buckets): (0, 2)})
+    # the autotuner: a choice per bucket present, cached per workload
+    cache = tmp_path / "cells.json"
+    c = Batch(specs)
+    choice = c.autotune_cells(45, cache=cache)
+    cuda.cuda.synchronize()
+    print(f"case: tuned cells {choice}", flush=True)
+    assert not choice["cached"] and choice["walk_ms"] > 0.0
+    assert set(choice["cells"]) == set(buckets)
+    assert c.get_cells() == choice["cells"]
+    again = Batch(specs).autotune_cells(45, cache=cache)
+    assert again["cached"] and again["cells"] == choice["cells"]
+    # a first build with the chosen cells gives the reference's pair sets
+    d = Batch(specs)
+    d.set_cells(choice["cells"])
+    d.run(1)
+    cuda.cuda.synchronize()
+    for r, s in enumerate(specs):
+        ref_pairs = orc.build_pairlist(s.pos4(), s.box, s.params.pairlist_cutoff)["pairs"].astype(np.int64)
+        assert np.array_equal(sorted_pairs(d.get_neighbor_pairs(r)), ref_pairs), f"replica {r}"
