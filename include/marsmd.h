@@ -232,6 +232,21 @@ marsmd_status marsmd_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* v
                                       marsmd_stream stream);
 /* Scratch bytes marsmd_radix_sort_pairs needs for n keys. */
 uint64_t marsmd_radix_sort_scratch_bytes(uint64_t n);
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): marsmd_radix_sort_pairs with the library's own fixed workspace, a device
+ * array of marsmd_radix_sort_static_scratch_bytes() bytes that every GPU loading the
+ * library holds until the process ends. Same results and buffer rules as
+ * marsmd_radix_sort_pairs; at most MARSMD_STATIC_SORT_MAX_KEYS keys (MARSMD_ERR_VALUE
+ * beyond). It allocates nothing; the first call on a device looks the workspace's
+ * address up. Every call on one GPU shares that workspace: calls that overlap in time
+ * on one GPU (other streams, other threads) race. */
+#define MARSMD_STATIC_SORT_MAX_KEYS (1u << 21)
+marsmd_status marsmd_radix_sort_pairs_static(const uint32_t* keys_in, const uint32_t* vals_in,
+                                             uint32_t* keys_out, uint32_t* vals_out, uint64_t n,
+                                             int32_t key_bits, marsmd_stream stream);
+uint64_t marsmd_radix_sort_static_scratch_bytes(void);
+
 /* ------------------------------------------------------------------ pairlist
  * Replaces ZOrderPairlist (src/PatchOperation/PairListKernels/ZOrderPairlist.h:27-190,
  * ZOrderPairlist.cpp:102-411; ZOrderNeighbor.h:27-225) behind Pairlist

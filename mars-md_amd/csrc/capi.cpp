@@ -133,6 +133,14 @@ marsmd_status marsmd_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* v
 	});
 }
 uint64_t marsmd_radix_sort_scratch_bytes(uint64_t n) { return mmd::radix_sort_scratch_bytes(n); }
+marsmd_status marsmd_radix_sort_pairs_static(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+											 uint32_t* vals_out, uint64_t n, int32_t key_bits, marsmd_stream stream) {
+	static_assert(MARSMD_STATIC_SORT_MAX_KEYS == mmd::kStaticSortMaxKeys, "static sort limit");
+	return guard([&] {
+		mmd::radix_sort_pairs_static(keys_in, vals_in, keys_out, vals_out, n, key_bits, st(stream));
+	});
+}
+uint64_t marsmd_radix_sort_static_scratch_bytes(void) { return mmd::static_sort_scratch_bytes(); }
 
 // ---------------------------------------------------------------- pairlist
 marsmd_status marsmd_pairlist_create(uint64_t max_particles, uint64_t max_pairs, marsmd_pairlist** out) {

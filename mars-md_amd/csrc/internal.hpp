@@ -107,6 +107,13 @@ uint64_t radix_sort_scratch_bytes(uint64_t n);
 void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
 					  uint32_t* vals_out, uint64_t n, int key_bits, void* scratch,
 					  uint64_t scratch_bytes, hipStream_t s);
+// The same sort with the library's own fixed device workspace instead of a caller's:
+// no allocation, at most kStaticSortMaxKeys keys (MARSMD_ERR_VALUE beyond). All calls
+// on one GPU share that workspace, so calls that overlap in time on one GPU race.
+constexpr uint64_t kStaticSortMaxKeys = uint64_t(1) << 21;
+void radix_sort_pairs_static(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+							 uint32_t* vals_out, uint64_t n, int key_bits, hipStream_t s);
+uint64_t static_sort_scratch_bytes();
 uint64_t scan_scratch_bytes(uint64_t n);
 // Exclusive scan of n counts into out[0..n]; out[n] holds the total.
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* scratch,

@@ -17,6 +17,8 @@
 // ranked tile in LDS and writes it out in digit order.
 #include "internal.hpp"
 
+#include <mutex>
+
 namespace mmd {
 
 namespace {
@@ -339,6 +341,62 @@ void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
 uint64_t scan_scratch_bytes(uint64_t n) {
 	const uint64_t blocks = (n + kScanTile - 1) / kScanTile;
 	return align256((blocks + 1) * 4);
+}
+
+// ------------------------------------------------- sort without allocation
+// A workspace for radix_sort_pairs_static, fixed at build time: the same layout as
+// radix_sort_scratch_bytes(kStaticSortMaxKeys) (two key/value staging arrays, the
+// [digit][tile] histogram with its total, the scan's block sums). Every GPU that loads
+// the library holds one copy until the process ends. Nothing in it has to start at
+// zero: each pass writes every histogram entry and block sum before reading it.
+namespace {
+constexpr uint64_t cx_align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
+constexpr uint64_t cx_sort_scratch_bytes(uint64_t n) {
+	return cx_align256(n * 4) * 2 +
+		   cx_align256((n + kRsTile - 1) / kRsTile * kRsMaxBuckets * 4 + 4) +
+		   cx_align256(((n + kRsTile - 1) / kRsTile * kRsMaxBuckets + kScanTile - 1) / kScanTile * 4 + 4);
+}
+constexpr uint64_t kStaticSortScratchBytes = cx_sort_scratch_bytes(kStaticSortMaxKeys);
+
+static __device__ __align__(256) unsigned char scratch[kStaticSortScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device
+// (each GPU holds the array at its own address; its host-side name is not one).
+void* static_sort_scratch() {
+	int dev = 0;
+	MMD_HIP(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices)
+		raise(MARSMD_ERR_VALUE, "radix sort (static workspace): device ordinal " + std::to_string(dev) +
+									" beyond the " + std::to_string(kMaxDevices) + " supported");
+	std::lock_guard<std::mutex> lock(scratch_mu);
+	if (!scratch_addr[dev]) {
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+			raise(MARSMD_ERR_DEVICE, std::string("radix sort (static workspace): hipGetSymbolAddress failed: ") +
+									  hipGetErrorString(e));
+		scratch_addr[dev] = p;
+	}
+	return scratch_addr[dev];
+}
+} // namespace
+
+uint64_t static_sort_scratch_bytes() { return kStaticSortScratchBytes; }
+
+void radix_sort_pairs_static(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+							 uint32_t* vals_out, uint64_t n, int key_bits, hipStream_t s) {
+	if (n > kStaticSortMaxKeys)
+		raise(MARSMD_ERR_VALUE, "radix sort (static workspace): " + std::to_string(n) + " keys exceed the maximum of " +
+									std::to_string(kStaticSortMaxKeys));
+	static_assert(cx_sort_scratch_bytes(kStaticSortMaxKeys) >= 2 * kStaticSortMaxKeys * 4, "workspace layout");
+	void* ws = static_sort_scratch();
+	if (radix_sort_scratch_bytes(n) > kStaticSortScratchBytes)
+		raise(MARSMD_ERR_RUNTIME, "radix sort (static workspace): layout mismatch");
+	radix_sort_pairs(keys_in, vals_in, keys_out, vals_out, n, key_bits, ws, kStaticSortScratchBytes, s);
 }
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* scratch, hipStream_t s) {

@@ -252,6 +252,8 @@ PROTOTYPES = {
     "marsmd_zorder_compute_max_displacement": (_I32, [_P, _P, _U64, _P, _P, _P]),
     "marsmd_radix_sort_pairs": (_I32, [_P, _P, _P, _P, _U64, _I32, _P, _U64, _P]),
     "marsmd_radix_sort_scratch_bytes": (_U64, [_U64]),
+    "marsmd_radix_sort_pairs_static": (_I32, [_P, _P, _P, _P, _U64, _I32, _P]),
+    "marsmd_radix_sort_static_scratch_bytes": (_U64, []),
     "marsmd_pairlist_create": (_I32, [_U64, _U64, _PP]),
     "marsmd_pairlist_destroy": (_I32, [_P]),
     "marsmd_pairlist_set_periodic_box": (_I32, [_P, C.POINTER(Box)]),

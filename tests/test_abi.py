@@ -120,3 +120,13 @@ def test_batch_rejects_replicas_with_different_seeds():
     with pytest.raises(ValueError, match="base seed"):
         Batch([a, b])
 
+
+def test_static_workspace_sort_limits_without_touching_the_device():
+    lib = api.lib()
+    limit = 1 << 21
+    # the workspace holds the allocating entry point's scratch for the largest sort
+    assert lib.marsmd_radix_sort_static_scratch_bytes() >= lib.marsmd_radix_sort_scratch_bytes(limit)
+    # more keys than the workspace serves: rejected on the host before any device call
+    a, b, c, d = (0x10000000 * k for k in range(1, 5))
+    with pytest.raises(api.MarsError, match="maximum"):
+        api.call("marsmd_radix_sort_pairs_static", a, b, c, d, limit + 1, 30, None)
