@@ -640,6 +640,14 @@ marsmd_status marsmd_fm_forces(marsmd_fm* fm, const float* knots, float* force_e
  * force_energy4; .w ignored). Overwrites grad_knots. */
 marsmd_status marsmd_fm_adjoint(marsmd_fm* fm, const float* knots, const float* grad_force4, double* grad_knots,
                                 marsmd_stream stream);
+/* The mean squared force error over every snapshot, particle and component, and its
+ * gradient, in one pass: *loss (device FP64) = sum |F - target|^2 / (3 S n) with the
+ * differences in FP32 and the squares summed in FP64 in a fixed order; grad_force4
+ * (S x n float4) = (F - target) * (FP32 value of 2 / (3 S n)), .w = 0. force_energy4 as
+ * marsmd_fm_forces writes it; target3: S x n x 3 floats in the same order. No
+ * counterpart in the reference (it has no gradient path). */
+marsmd_status marsmd_fm_mse(marsmd_fm* fm, const float* force_energy4, const float* target3, float* grad_force4,
+                            double* loss, marsmd_stream stream);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -720,6 +720,17 @@ marsmd_status marsmd_fm_forces(marsmd_fm* fm, const float* knots, float* force_e
 	});
 }
 
+marsmd_status marsmd_fm_mse(marsmd_fm* fm, const float* force_energy4, const float* target3, float* grad_force4,
+							double* loss, marsmd_stream stream) {
+	return guard([&] {
+		need(fm, "force matching");
+		need(force_energy4, "forces");
+		need(target3, "target");
+		need(grad_force4, "grad_force4");
+		need(loss, "loss");
+		fm->impl.mse(force_energy4, target3, grad_force4, loss, st(stream));
+	});
+}
 marsmd_status marsmd_fm_adjoint(marsmd_fm* fm, const float* knots, const float* grad_force4, double* grad_knots,
 								marsmd_stream stream) {
 	return guard([&] {

@@ -12,6 +12,14 @@
 //   dL/dfm = (G_y - G_x) . u,   dL/dU[h] += dL/dfm step_inv,   dL/dU[h+1] -= dL/dfm step_inv;
 // a distance past the last knot of a (non-periodic) table carries no force and no
 // gradient. Each pair is visited once, from its lower canonical end.
+//
+// The snapshots are fixed, so everything the adjoint needs of a pair but the gradients
+// is fixed too: at creation each snapshot's pairs are reduced, once, to records of the
+// pairs that carry a gradient (lower canonical end, tabulated, 1e-6 <= d <= cutoff,
+// inside the table) holding the higher end's canonical index, the knot index of the
+// pair's bin and {u, step_inv} (u = r / d), computed with the operations the
+// evaluation used to repeat on every call. The adjoint then streams records instead of
+// the full neighbour rows and positions (half the entries, no geometry).
 #include "fm.hpp"
 
 #include "nbforce.hpp"
@@ -35,6 +43,11 @@ struct FmSnapView {
 	const uint32_t* counts;
 	const uint32_t* tile_base;
 	const uint32_t* nbr;
+	// adjoint records: record k of slot t at nbr_slot_index(tile_base[t / 64], t, k)
+	const uint32_t* rec_count;
+	const uint32_t* rec_y;  // canonical index of the pair's higher end
+	const int32_t* rec_k;   // knot index of the pair's bin (table offset + bin)
+	const float4* rec_u;    // {u, step_inv}
 };
 
 namespace {
@@ -62,22 +75,75 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const FmSnapView* __rest
 	}
 }
 
-// dL/dknot of every snapshot in one persistent launch (work items as in k_fm_forces):
-// per-workgroup FP64 sums in LDS written to partial[blockIdx.x][knot], or global FP64
-// atomics into partial[0] when the tables exceed the LDS budget. Each workgroup visits
-// a fixed sequence of work items; the order of the atomic additions within it is not
-// fixed, which FP64 keeps at the level of its rounding (about 1e-16 relative).
-template<bool LDS>
-__global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const FmSnapView* __restrict__ snaps, uint64_t n,
-														 uint32_t tiles, uint32_t items, NbTables T, DevBox box,
-														 float cutoff2, const float4* __restrict__ grad_all,
-														 double* __restrict__ partial) {
-	extern __shared__ double lds_grad[];
+// The adjoint's records of one snapshot (see the top of this file): slot t's row is
+// scanned in order and every pair that carries a gradient is written as the next
+// record of t, with the operations k_fm_adjoint used on the pair.
+__global__ __launch_bounds__(kFmBlock) void k_fm_records(const float4* __restrict__ spos, uint64_t n,
+														 NbrList L, NbTables T, DevBox box, float cutoff2,
+														 uint32_t* __restrict__ rec_count, uint32_t* __restrict__ rec_y,
+														 int32_t* __restrict__ rec_k, float4* __restrict__ rec_u) {
 	__shared__ NbLds lds_nb;
 	stage_params(T, lds_nb);
+	__syncthreads();
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const float4 p = spos[t];
+	const uint32_t mt = __float_as_uint(p.w);
+	const uint32_t ct = meta_canon(mt);
+	const uint32_t tt = meta_type(mt);
+	const uint64_t base = L.tile_base[t >> 6];
+	const uint32_t cnt = L.count[t];
+	uint32_t m = 0;
+	for (uint32_t k = 0; k < cnt; ++k) {
+		const float4 q = spos[L.nbr[nbr_slot_index(base, uint32_t(t), k)]];
+		const uint32_t mj = __float_as_uint(q.w);
+		const uint32_t cj = meta_canon(mj);
+		if (!(ct < cj))
+			continue; // the pair is taken from its lower canonical end
+		const uint32_t tag = lds_nb.tag[(tt << 4) | meta_type(mj)];
+		if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
+			continue;
+		const float3 r = wrap_diff(box, sub3(q, p));
+		const float d = sqrtf(len2(r));
+		if (d < 1e-6f)
+			continue;
+		if (cutoff2 > 0.0f && d * d > cutoff2)
+			continue;
+		const float inv = 1.0f / d;
+		const float4 tp = lds_nb.tab[tag >> MARSMD_PAIR_TABLE_SHIFT]; // {start, step_inv, size, offset}
+		const float w = (d - tp.x) * tp.y;
+		int h = int(floorf(w));
+		if (h >= __float_as_int(tp.z) - 1)
+			continue;
+		if (h < 0)
+			h = 0;
+		const uint64_t at = nbr_slot_index(base, uint32_t(t), m);
+		rec_y[at] = cj;
+		rec_k[at] = __float_as_int(tp.w) + h;
+		rec_u[at] = make_float4(inv * r.x, inv * r.y, inv * r.z, tp.y);
+		++m;
+	}
+	rec_count[t] = m;
+}
+
+// dL/dknot of every snapshot in one persistent launch (work items as in k_fm_forces),
+// from the records: slot t (canonical x) adds, for each record (y, k, u),
+// g = ((G_y - G_x) . u) step_inv to knot k and -g to knot k + 1, in FP64; records are
+// read four at a time. Per-workgroup FP64 sums in LDS are written to
+// partial[blockIdx.x][knot], or global FP64 atomics go into partial[0] when the tables
+// exceed the LDS budget. Each workgroup visits a fixed sequence of work items, but the
+// order of the atomic additions within it is not fixed: the sums are deterministic up
+// to FP64 rounding (about 1e-16 relative), not bit for bit.
+template<bool LDS>
+__global__ __launch_bounds__(kFmBlock) void k_fm_adjoint_rec(const FmSnapView* __restrict__ snaps, uint64_t n,
+															 uint32_t tiles, uint32_t items, uint32_t total_knots,
+															 const float4* __restrict__ grad_all,
+															 double* __restrict__ partial) {
+	extern __shared__ double lds_grad[];
 	double* acc = LDS ? lds_grad : partial;
 	if (LDS)
-		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
+		for (uint32_t i = threadIdx.x; i < total_knots; i += blockDim.x)
 			lds_grad[i] = 0.0;
 	__syncthreads();
 	const TileRange R = xcd_tiles(items);
@@ -87,66 +153,45 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint(const FmSnapView* __res
 		if (t >= n)
 			continue;
 		const FmSnapView V = snaps[snap];
-		const float4* __restrict__ spos = V.spos;
-		const NbrList L{V.counts, V.tile_base, V.nbr};
 		const float4* __restrict__ grad = grad_all + uint64_t(snap) * n;
-		const float4 p = spos[t];
-		const uint32_t mt = __float_as_uint(p.w);
-		const uint32_t ct = meta_canon(mt);
-		const uint32_t tt = meta_type(mt);
+		const uint32_t cnt = V.rec_count[t];
+		if (cnt == 0)
+			continue;
+		const uint32_t ct = meta_canon(__float_as_uint(V.spos[t].w));
 		const float4 gx = grad[ct];
-		// the row in groups of 4 entries (one 16-byte index load, 4 independent position
-		// gathers); the pads of the last group are t itself, skipped as ct == cj
-		const uint32_t ng = (L.count[t] + 3u) >> 2;
-		const uint4* __restrict__ row = reinterpret_cast<const uint4*>(L.nbr + L.tile_base[t >> 6]) + (t & 63u);
-		for (uint32_t gi = 0; gi < ng; ++gi) {
-			const uint4 jj = row[uint64_t(gi) * 64u];
-			const uint32_t js[4] = {jj.x, jj.y, jj.z, jj.w};
-			float4 qs[4];
-#pragma unroll
-			for (int e = 0; e < 4; ++e)
-				qs[e] = spos[js[e]];
+		const uint64_t row = V.tile_base[t >> 6] + uint64_t(t & 63u) * 4u;
+		for (uint32_t g0 = 0; g0 < cnt; g0 += 4) {
+			const uint64_t at = row + uint64_t(g0 >> 2) * 256u;
+			const uint4 ys = *reinterpret_cast<const uint4*>(V.rec_y + at);
+			const int4 ks = *reinterpret_cast<const int4*>(V.rec_k + at);
+			const uint32_t yv[4] = {ys.x, ys.y, ys.z, ys.w};
+			const int kv[4] = {ks.x, ks.y, ks.z, ks.w};
+			float4 uv[4], gy[4];
 #pragma unroll
 			for (int e = 0; e < 4; ++e) {
-				const float4 q = qs[e];
-				const uint32_t mj = __float_as_uint(q.w);
-				const uint32_t cj = meta_canon(mj);
-				if (!(ct < cj))
-					continue; // the pair is taken from its lower canonical end
-				const uint32_t tag = lds_nb.tag[(tt << 4) | meta_type(mj)];
-				if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
-					continue;
-				const float3 r = wrap_diff(box, sub3(q, p));
-				const float d = sqrtf(len2(r));
-				if (d < 1e-6f)
-					continue;
-				if (cutoff2 > 0.0f && d * d > cutoff2)
-					continue;
-				const float inv = 1.0f / d;
-				const float ux = inv * r.x, uy = inv * r.y, uz = inv * r.z;
-				const float4 tp = lds_nb.tab[tag >> MARSMD_PAIR_TABLE_SHIFT]; // {start, step_inv, size, offset}
-				const float w = (d - tp.x) * tp.y;
-				int h = int(floorf(w));
-				if (h >= __float_as_int(tp.z) - 1)
-					continue;
-				if (h < 0)
-					h = 0;
-				const float4 gy = grad[cj];
-				const double dfm = (double(gy.x) - double(gx.x)) * double(ux) +
-								   (double(gy.y) - double(gx.y)) * double(uy) +
-								   (double(gy.z) - double(gx.z)) * double(uz);
-				const double g = dfm * double(tp.y);
-				const int k0 = __float_as_int(tp.w) + h;
-				atomicAdd(&acc[k0], g);
-				atomicAdd(&acc[k0 + 1], -g);
+				const bool live = g0 + uint32_t(e) < cnt;
+				uv[e] = live ? V.rec_u[at + uint64_t(e)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				gy[e] = live ? grad[yv[e]] : gx;
+			}
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				if (g0 + uint32_t(e) >= cnt)
+					break;
+				const float4 u = uv[e];
+				const double dfm = (double(gy[e].x) - double(gx.x)) * double(u.x) +
+								   (double(gy[e].y) - double(gx.y)) * double(u.y) +
+								   (double(gy[e].z) - double(gx.z)) * double(u.z);
+				const double g = dfm * double(u.w);
+				atomicAdd(&acc[kv[e]], g);
+				atomicAdd(&acc[kv[e] + 1], -g);
 			}
 		}
 	}
 	if (LDS) {
 		__syncthreads();
-		double* __restrict__ row = partial + uint64_t(blockIdx.x) * uint64_t(T.total_knots);
-		for (int i = threadIdx.x; i < T.total_knots; i += blockDim.x)
-			row[i] = lds_grad[i];
+		double* __restrict__ out = partial + uint64_t(blockIdx.x) * uint64_t(total_knots);
+		for (uint32_t i = threadIdx.x; i < total_knots; i += blockDim.x)
+			out[i] = lds_grad[i];
 	}
 }
 
@@ -176,6 +221,43 @@ __global__ void k_fm_reduce_segs(const double* __restrict__ seg, uint32_t total,
 	out[k] = s;
 }
 
+// The mean squared force error and its gradient in one pass over S x n records:
+// d = F - target (FP32), G = d * scale (scale = 2 / (3 S n), FP32, the .w lane 0),
+// and per-block FP64 sums of d . d (a fixed tree inside the block).
+constexpr int kMseBlock = 256;
+__global__ __launch_bounds__(kMseBlock) void k_fm_mse(const float4* __restrict__ f, const float* __restrict__ target3,
+													  uint64_t count, float scale, float4* __restrict__ grad,
+													  double* __restrict__ block_sums) {
+	__shared__ double part[kMseBlock];
+	double acc = 0.0;
+	for (uint64_t i = uint64_t(blockIdx.x) * kMseBlock + threadIdx.x; i < count;
+		 i += uint64_t(gridDim.x) * kMseBlock) {
+		const float4 v = f[i];
+		const float dx = v.x - target3[3 * i], dy = v.y - target3[3 * i + 1], dz = v.z - target3[3 * i + 2];
+		grad[i] = make_float4(dx * scale, dy * scale, dz * scale, 0.0f);
+		acc += double(dx) * double(dx) + double(dy) * double(dy) + double(dz) * double(dz);
+	}
+	part[threadIdx.x] = acc;
+	__syncthreads();
+	for (int o = kMseBlock / 2; o > 0; o >>= 1) {
+		if (int(threadIdx.x) < o)
+			part[threadIdx.x] += part[threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		block_sums[blockIdx.x] = part[0];
+}
+// loss = (sum of the block sums, in block order) / (3 S n), one thread.
+__global__ void k_fm_mse_total(const double* __restrict__ block_sums, uint32_t blocks, double inv_count,
+							   double* __restrict__ loss) {
+	if (threadIdx.x != 0 || blockIdx.x != 0)
+		return;
+	double s = 0.0;
+	for (uint32_t b = 0; b < blocks; ++b)
+		s += block_sums[b];
+	*loss = s * inv_count;
+}
+
 __global__ void k_fill_f64(double* p, uint64_t n) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i < n)
@@ -187,6 +269,9 @@ __global__ void k_fill_f64(double* p, uint64_t n) {
 struct FMImpl::Snap {
 	DevArray<float4> spos;
 	DevArray<uint32_t> sorted_idx, counts, tile_base, nbr;
+	DevArray<uint32_t> rec_count, rec_y;
+	DevArray<int32_t> rec_k;
+	DevArray<float4> rec_u;
 };
 
 FMImpl::FMImpl(const marsmd_fm_desc& d) : tables(new NbTables{}) {
@@ -248,11 +333,22 @@ FMImpl::FMImpl(const marsmd_fm_desc& d) : tables(new NbTables{}) {
 		if (pl.nbr_storage)
 			MMD_HIP(hipMemcpyAsync(sn->nbr.data(), pl.nbr.data(), 4 * pl.nbr_storage, hipMemcpyDeviceToDevice, nullptr));
 		num_pairs += pl.num_pairs();
+		// the adjoint's records, laid out like the rows (at most as many as row entries)
+		sn->rec_count.resize(n);
+		sn->rec_y.resize(std::max<uint64_t>(pl.nbr_storage, 4));
+		sn->rec_k.resize(std::max<uint64_t>(pl.nbr_storage, 4));
+		sn->rec_u.resize(std::max<uint64_t>(pl.nbr_storage, 4));
+		const NbrList L{sn->counts.data(), sn->tile_base.data(), sn->nbr.data()};
+		k_fm_records<<<grid_for(n, kFmBlock), kFmBlock, 0, nullptr>>>(sn->spos.data(), n, L, T, make_devbox(box),
+																	  cutoff2, sn->rec_count.data(), sn->rec_y.data(),
+																	  sn->rec_k.data(), sn->rec_u.data());
+		MMD_LAUNCH_CHECK();
 	}
 	std::vector<FmSnapView> views(S);
 	for (uint64_t k = 0; k < S; ++k)
-		views[k] = FmSnapView{snaps[k]->spos.data(), snaps[k]->sorted_idx.data(), snaps[k]->counts.data(),
-							  snaps[k]->tile_base.data(), snaps[k]->nbr.data()};
+		views[k] = FmSnapView{snaps[k]->spos.data(),   snaps[k]->sorted_idx.data(), snaps[k]->counts.data(),
+							  snaps[k]->tile_base.data(), snaps[k]->nbr.data(),  snaps[k]->rec_count.data(),
+							  snaps[k]->rec_y.data(),  snaps[k]->rec_k.data(),  snaps[k]->rec_u.data()};
 	snap_views.resize(S);
 	MMD_HIP(hipMemcpy(snap_views.data(), views.data(), sizeof(FmSnapView) * S, hipMemcpyHostToDevice));
 	MMD_HIP(hipDeviceSynchronize());
@@ -286,6 +382,19 @@ void FMImpl::forces(const float* knots, float* out4, hipStream_t s) {
 	MMD_LAUNCH_CHECK();
 }
 
+void FMImpl::mse(const float* force4, const float* target3, float* grad4, double* loss, hipStream_t s) {
+	const uint64_t count = S * n;
+	const uint32_t blocks = uint32_t(std::min<uint64_t>((count + kMseBlock - 1) / kMseBlock, 2048));
+	mse_sums.resize(blocks);
+	// the gradient's scale as the FP32 value of 2 / count (FP64 division)
+	const float scale = float(2.0 / (3.0 * double(count)));
+	k_fm_mse<<<blocks, kMseBlock, 0, s>>>(reinterpret_cast<const float4*>(force4), target3, count, scale,
+										  reinterpret_cast<float4*>(grad4), mse_sums.data());
+	MMD_LAUNCH_CHECK();
+	k_fm_mse_total<<<1, 64, 0, s>>>(mse_sums.data(), blocks, 1.0 / (3.0 * double(count)), loss);
+	MMD_LAUNCH_CHECK();
+}
+
 void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStream_t s) {
 	(void)knots; // the gradient of a linear table force does not depend on the knots
 	const uint32_t total = uint32_t(tables->total_knots);
@@ -295,18 +404,18 @@ void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStr
 	const float4* g = reinterpret_cast<const float4*>(grad4);
 	uint32_t rows = 1;
 	if (lds) {
-		const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_adjoint<true>), total * 8u, items);
+		const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_adjoint_rec<true>), total * 8u, items);
 		rows = grid;
 		partial.resize(uint64_t(rows) * total);
-		k_fm_adjoint<true><<<grid, kFmBlock, total * 8u, s>>>(snap_views.data(), n, tiles, uint32_t(items), *tables,
-															 make_devbox(box), cutoff2, g, partial.data());
+		k_fm_adjoint_rec<true><<<grid, kFmBlock, total * 8u, s>>>(snap_views.data(), n, tiles, uint32_t(items), total,
+																 g, partial.data());
 	} else {
-		const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_adjoint<false>), 0u, items);
+		const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_adjoint_rec<false>), 0u, items);
 		partial.resize(total);
 		k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), total);
 		MMD_LAUNCH_CHECK();
-		k_fm_adjoint<false><<<grid, kFmBlock, 0, s>>>(snap_views.data(), n, tiles, uint32_t(items), *tables,
-													  make_devbox(box), cutoff2, g, partial.data());
+		k_fm_adjoint_rec<false><<<grid, kFmBlock, 0, s>>>(snap_views.data(), n, tiles, uint32_t(items), total, g,
+														  partial.data());
 	}
 	MMD_LAUNCH_CHECK();
 	seg_sums.resize(uint64_t(kFmSegs) * total);

@@ -17,7 +17,7 @@ struct FMImpl {
 	float cutoff2 = 0.0f;
 	NbTables* tables = nullptr;
 	std::vector<Snap*> snaps;
-	DevArray<double> partial, seg_sums;
+	DevArray<double> partial, seg_sums, mse_sums;
 	DevArray<FmSnapView> snap_views; // per snapshot: its device arrays
 
 	explicit FMImpl(const marsmd_fm_desc& d);
@@ -28,6 +28,10 @@ struct FMImpl {
 	// Forces of every snapshot for the given knots (device FP32, tables concatenated):
 	// out4 is S x n float4 records {F, energy} in each snapshot's particle order.
 	void forces(const float* knots, float* out4, hipStream_t s);
+	// Mean squared error of the forces (force4: S x n float4 as forces() writes them)
+	// against target3 (S x n x 3 FP32): *loss (device FP64) = sum |F - target|^2 / (3 S n),
+	// and grad4 (S x n float4) = (F - target) 2 / (3 S n), the loss's dL/dF.
+	void mse(const float* force4, const float* target3, float* grad4, double* loss, hipStream_t s);
 	// dL/dknot (device FP64) for dL/dF = grad4 (S x n float4, .w ignored).
 	void adjoint(const float* knots, const float* grad4, double* out, hipStream_t s);
 };

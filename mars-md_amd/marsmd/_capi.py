@@ -324,6 +324,7 @@ PROTOTYPES = {
     "marsmd_fm_destroy": (_I32, [_P]),
     "marsmd_fm_num_pairs": (_I32, [_P, C.POINTER(_U64)]),
     "marsmd_fm_forces": (_I32, [_P, _P, _P, _P]),
+    "marsmd_fm_mse": (_I32, [_P, _P, _P, _P, _P, _P]),
     "marsmd_fm_adjoint": (_I32, [_P, _P, _P, _P, _P]),
 }
 

@@ -82,6 +82,25 @@ class ForceMatching:
         api.call("marsmd_fm_forces", self._h, k.data_ptr(), out.data_ptr(), _stream(stream))
         return out
 
+    def mse(self, forces, target, stream=None):
+        """(loss, dL/dF) of the mean squared force error in one pass: loss a device FP64
+        scalar, sum |F - target|^2 / (3 S n) (FP32 differences, FP64 squares in a fixed
+        order); dL/dF (S, n, 4) float32 = (F - target) * 2 / (3 S n). forces as forces()
+        returns them; target (S, n, 3)."""
+        torch = _torch()
+        f = forces.detach()
+        if f.shape != (self.S, self.n, 4) or f.dtype != torch.float32:
+            raise ValueError("forces must be the (S, n, 4) float32 output of forces()")
+        t = target.detach().to(device="cuda", dtype=torch.float32).contiguous()
+        if t.shape != (self.S, self.n, 3):
+            raise ValueError(f"target must be ({self.S}, {self.n}, 3)")
+        f = f.contiguous()
+        grad = torch.empty_like(f)
+        loss = torch.empty((), dtype=torch.float64, device="cuda")
+        api.call("marsmd_fm_mse", self._h, f.data_ptr(), t.data_ptr(), grad.data_ptr(), loss.data_ptr(),
+                 _stream(stream))
+        return loss, grad
+
     def adjoint(self, knots, grad_forces, stream=None):
         """dL/dknot (float64, total knots) for dL/dF = grad_forces (S, n, 3 or 4)."""
         torch = _torch()
@@ -151,10 +170,43 @@ def _mse_function():
     return ForceMSE
 
 
+_LOSS = None
+
+
+def _loss_function():
+    torch = _torch()
+
+    class ForceMatchingLoss(torch.autograd.Function):
+        """mean((F(knots) - target)^2) as two library calls (forces, then the loss and
+        its dL/dF in one pass); backward is the FP64 adjoint of that dL/dF."""
+
+        @staticmethod
+        def forward(ctx, knots, fm: ForceMatching, target):
+            f = fm.forces(knots)
+            loss, grad_f = fm.mse(f, target)
+            ctx.fm = fm
+            ctx.save_for_backward(knots, grad_f)
+            return loss
+
+        @staticmethod
+        def backward(ctx, g):
+            knots, grad_f = ctx.saved_tensors
+            gk = ctx.fm.adjoint(knots, grad_f)
+            return (gk * g.to(gk.dtype)).to(knots.dtype), None, None
+
+    return ForceMatchingLoss
+
+
 def force_matching_loss(knots, fm: ForceMatching, target):
-    """Mean squared force error over snapshots, particles and components (FP64 sum)."""
-    global _MSE
-    if _MSE is None:
-        _MSE = _mse_function()
-    f = tabulated_pair_forces(knots, fm)
-    return _MSE.apply(f, target)
+    """Mean squared force error over snapshots, particles and components (FP64 sum),
+    differentiable in the knots. A target that itself requires a gradient goes through
+    the forces' autograd function and an elementwise loss instead."""
+    global _MSE, _LOSS
+    if getattr(target, "requires_grad", False):
+        if _MSE is None:
+            _MSE = _mse_function()
+        f = tabulated_pair_forces(knots, fm)
+        return _MSE.apply(f, target)
+    if _LOSS is None:
+        _LOSS = _loss_function()
+    return _LOSS.apply(knots, fm, target)

@@ -77,3 +77,24 @@ def test_autograd_is_the_adjoint_and_adam_reduces_the_loss(cuda):
         opt.step()
         first = loss.item() if first is None else first
     assert loss.item() < 0.2 * first, (first, loss.item())
+
+
+def test_fused_loss_matches_the_elementwise_loss(cuda):
+    torch = cuda
+    w, fm = _setup(n=2000, L=260.0, S=3, knots=256, seed=4)
+    target = fm.forces(fm.knots(w.target_tables))[..., :3].clone()
+    knots = fm.knots()
+    f = fm.forces(knots)
+    loss, grad = fm.mse(f, target)
+    d = f[..., :3] - target
+    ref = (d.double() ** 2).sum() / d.numel()
+    assert abs(loss.item() - ref.item()) <= 1e-12 * ref.item()
+    scale = torch.tensor(2.0 / d.numel(), dtype=torch.float64).to(torch.float32)
+    assert torch.equal(grad[..., :3], d * scale) and torch.all(grad[..., 3] == 0)
+    # the fused autograd path and the forces' own autograd function give one gradient
+    k1 = knots.clone().requires_grad_(True)
+    force_matching_loss(k1, fm, target).backward()
+    k2 = knots.clone().requires_grad_(True)
+    f2 = tabulated_pair_forces(k2, fm)
+    ((f2 - target) ** 2).sum().div(f2.numel()).backward()
+    assert torch.allclose(k1.grad.double(), k2.grad.double(), rtol=1e-5, atol=1e-12 * k1.grad.abs().max().item())
