@@ -334,10 +334,13 @@ def main():
     spec = configs.config2(n=args.n, grid_n=args.grid)
     patch = marsmd.Patch(spec)
     stream = torch.cuda.current_stream()
-    # warmup: the autotuner's real steps (launch candidates timed on the step kernels),
-    # then the configuration it keeps
+    # warmup: the autotuners' real steps (the walk's cell configuration timed on
+    # re-walks of the first build's positions, then launch candidates timed on the step
+    # kernels), then the configuration they keep
     warmup = max(args.warmup, 60)
-    choice = patch.autotune(warmup)
+    cell_steps = 10
+    cells = patch.autotune_cells(cell_steps)
+    choice = patch.autotune(warmup - cell_steps, first_step=cell_steps)
     torch.cuda.synchronize()
 
     # timed region (the value): REPEATS runs of `steps` steps, each one run() call
@@ -399,6 +402,7 @@ def main():
         "copy_bandwidth": copy_bw,
         "launch": {"blocks_per_cu": choice["blocks_per_cu"], "split_pmf": choice["split_pmf"],
                    "tuned_ms_per_step": choice["ms_per_step"]},
+        "cells": cells,
         "roofline": {"bound": "hbm",
                      "limiter": "valu",
                      "kernel": ("k_pmf_force + k_step (grid force, then pair force + BAOAB)" if choice["split_pmf"]

@@ -518,6 +518,22 @@ marsmd_status marsmd_patch_get_launch(marsmd_patch* p, marsmd_patch_launch* laun
 marsmd_status marsmd_patch_autotune(marsmd_patch* p, const marsmd_run_params* params, uint64_t first_step,
                                     uint64_t num_steps, marsmd_patch_launch* best, double* best_ms_per_step,
                                     marsmd_stream stream);
+/* Cell configuration of the Patch's pairlist walk: home cells (one wave's tile of home
+ * particles) of the finest Morton level that holds home_target particles on average
+ * (1 .. 4096, default 24), candidate cells fine_offset levels below (1 .. 4, default
+ * 2). The pair set does not depend on it; the order of neighbours in a row, and so the
+ * FP32 summation order of the forces, does. Applies from the next build. No
+ * counterpart in the reference (its cell grid is select_cell_grid's,
+ * ZOrderPairlist.cpp:24-81). */
+marsmd_status marsmd_patch_set_cells(marsmd_patch* p, int32_t home_target, int32_t fine_offset);
+marsmd_status marsmd_patch_get_cells(marsmd_patch* p, int32_t* home_target, int32_t* fine_offset);
+/* Runs first_step+1 .. first_step+num_steps as marsmd_patch_run, after choosing the cell
+ * configuration (home_target 8/12/16/24/48 x fine_offset 1/2/3), each timed re-walking
+ * the last build's positions; the live list is then walked with the choice.
+ * *walk_ms: the choice's walk time (0 when the home walk does not apply). */
+marsmd_status marsmd_patch_autotune_cells(marsmd_patch* p, const marsmd_run_params* params,
+                                          uint64_t first_step, uint64_t num_steps, double* walk_ms,
+                                          marsmd_stream stream);
 
 /* ------------------------------------------------------------------ replica batch
  * Many independent replicas of a periodic nonbonded system, each one the reference's

@@ -56,6 +56,15 @@ struct RepGeom {
 	uint32_t stencil_off, stencil_n;
 };
 
+// A wrapped coordinate lies in [lo, hi] up to the rounding of PeriodicBox::wrap_scalar
+// (x - floor((x - o) inv_L) L can end an ulp or two below lo or above hi; the Morton
+// quantisation clamps such a coordinate, as the reference's does). One more than 2^-20
+// of the box length outside, or NaN, has left the box.
+__device__ __forceinline__ bool beyond_box(float x, float lo, float hi) {
+	const float tol = (hi - lo) * 0x1p-20f;
+	return !(x >= lo - tol && x <= hi + tol);
+}
+
 // keys / vals in canonical position: global canonical g = off_r + canon.
 __global__ void k_b_encode(const float4* __restrict__ pos, uint64_t total, const uint32_t* __restrict__ rep_of_slot,
 						   const RepGeom* __restrict__ geom, uint32_t* __restrict__ keys,
@@ -74,7 +83,7 @@ __global__ void k_b_encode(const float4* __restrict__ pos, uint64_t total, const
 	// the particle's current slot, which the gather reads from
 	keys[g] = split_by3(qz) | (split_by3(qy) << 1) | (split_by3(qx) << 2);
 	vals[g] = uint32_t(s);
-	if (out)
+	if (beyond_box(p.x, G.lo[0], G.hi[0]) || beyond_box(p.y, G.lo[1], G.hi[1]) || beyond_box(p.z, G.lo[2], G.hi[2]))
 		atomicOr(outside, 1u);
 }
 

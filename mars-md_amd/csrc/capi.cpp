@@ -530,6 +530,30 @@ marsmd_status marsmd_patch_get_launch(marsmd_patch* p, marsmd_patch_launch* laun
 	});
 }
 
+marsmd_status marsmd_patch_set_cells(marsmd_patch* p, int32_t home_target, int32_t fine_offset) {
+	return guard([&] {
+		need(p, "patch");
+		p->impl.set_cells(home_target, fine_offset);
+	});
+}
+marsmd_status marsmd_patch_get_cells(marsmd_patch* p, int32_t* home_target, int32_t* fine_offset) {
+	return guard([&] {
+		need(p, "patch");
+		need(home_target, "home_target");
+		need(fine_offset, "fine_offset");
+		*home_target = p->impl.pl->home_target;
+		*fine_offset = p->impl.pl->fine_offset;
+	});
+}
+marsmd_status marsmd_patch_autotune_cells(marsmd_patch* p, const marsmd_run_params* params, uint64_t first_step,
+										  uint64_t num_steps, double* walk_ms, marsmd_stream stream) {
+	return guard([&] {
+		need(p, "patch");
+		need(params, "params");
+		need(walk_ms, "walk_ms");
+		p->impl.autotune_cells(*params, first_step, num_steps, st(stream), walk_ms);
+	});
+}
 marsmd_status marsmd_patch_autotune(marsmd_patch* p, const marsmd_run_params* params, uint64_t first_step,
 									uint64_t num_steps, marsmd_patch_launch* best, double* best_ms_per_step,
 									marsmd_stream stream) {

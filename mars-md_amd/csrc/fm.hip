@@ -8,18 +8,21 @@
 // pairlist (ZOrderPairlist semantics), i.e. calculate_nonbonded_forces on that
 // configuration. A table force is linear in its knots,
 //   fm = -(U[h+1] - U[h]) step_inv      (h = the bin of the pair's distance),
-// so with G = dL/dF each pair (x = lower canonical index, y = higher) contributes
+// so with G = dL/dF each pair (x = lower canonical index, y = higher, u = r / d with
+// r = wrap(pos_y - pos_x), the forward force's orientation) contributes
 //   dL/dfm = (G_y - G_x) . u,   dL/dU[h] += dL/dfm step_inv,   dL/dU[h+1] -= dL/dfm step_inv;
 // a distance past the last knot of a (non-periodic) table carries no force and no
-// gradient. Each pair is visited once, from its lower canonical end.
+// gradient. The expression is symmetric in the two ends (swapping them negates both
+// factors), so each pair is visited once, from the end fm_owns picks.
 //
 // The snapshots are fixed, so everything the adjoint needs of a pair but the gradients
 // is fixed too: at creation each snapshot's pairs are reduced, once, to records of the
-// pairs that carry a gradient (lower canonical end, tabulated, 1e-6 <= d <= cutoff,
-// inside the table) holding the higher end's canonical index, the knot index of the
-// pair's bin and {u, step_inv} (u = r / d), computed with the operations the
-// evaluation used to repeat on every call. The adjoint then streams records instead of
-// the full neighbour rows and positions (half the entries, no geometry).
+// pairs that carry a gradient (tabulated, 1e-6 <= d <= cutoff, inside the table) at
+// their owning end t, holding the other end's canonical index o, the knot index of the
+// pair's bin and {v, step_inv}, v = u taken from t towards o (-u when t is the higher
+// end: the same r, negated exactly), so that dL/dfm = (G_o - G_t) . v. The adjoint
+// then streams records instead of the full neighbour rows and positions (half the
+// entries, no geometry).
 #include "fm.hpp"
 
 #include "nbforce.hpp"
@@ -75,9 +78,16 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const FmSnapView* __rest
 	}
 }
 
+// The end of pair (a, b) that holds its record: the lower index when a and b have the
+// same parity, the higher one otherwise. Exactly one of fm_owns(a, b), fm_owns(b, a)
+// holds, and a slot owns about half of its pairs whatever its index (with the lower
+// end always owning, slot counts would run from the whole row down to none, and a
+// wavefront's loop would take its longest lane's count).
+__device__ __forceinline__ bool fm_owns(uint32_t a, uint32_t b) { return ((a ^ b) & 1u) ? a > b : a < b; }
+
 // The adjoint's records of one snapshot (see the top of this file): slot t's row is
-// scanned in order and every pair that carries a gradient is written as the next
-// record of t, with the operations k_fm_adjoint used on the pair.
+// scanned in order and every pair that t owns and that carries a gradient is written as
+// the next record of t.
 __global__ __launch_bounds__(kFmBlock) void k_fm_records(const float4* __restrict__ spos, uint64_t n,
 														 NbrList L, NbTables T, DevBox box, float cutoff2,
 														 uint32_t* __restrict__ rec_count, uint32_t* __restrict__ rec_y,
@@ -99,18 +109,23 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_records(const float4* __restric
 		const float4 q = spos[L.nbr[nbr_slot_index(base, uint32_t(t), k)]];
 		const uint32_t mj = __float_as_uint(q.w);
 		const uint32_t cj = meta_canon(mj);
-		if (!(ct < cj))
-			continue; // the pair is taken from its lower canonical end
-		const uint32_t tag = lds_nb.tag[(tt << 4) | meta_type(mj)];
+		if (!fm_owns(ct, cj))
+			continue;
+		// the forward's orientation: type pair (lower, higher) and r = pos[higher] -
+		// pos[lower] (q - p or its exact negation)
+		const bool lower = ct < cj;
+		const uint32_t tj = meta_type(mj);
+		const uint32_t tag = lds_nb.tag[lower ? (tt << 4) | tj : (tj << 4) | tt];
 		if (!(tag & T.enabled & MARSMD_PAIR_TERM_TABULATED))
 			continue;
-		const float3 r = wrap_diff(box, sub3(q, p));
+		const float3 dq = sub3(q, p);
+		const float3 r = wrap_diff(box, lower ? dq : make_float3(-dq.x, -dq.y, -dq.z));
 		const float d = sqrtf(len2(r));
 		if (d < 1e-6f)
 			continue;
 		if (cutoff2 > 0.0f && d * d > cutoff2)
 			continue;
-		const float inv = 1.0f / d;
+		const float inv = lower ? 1.0f / d : -(1.0f / d);
 		const float4 tp = lds_nb.tab[tag >> MARSMD_PAIR_TABLE_SHIFT]; // {start, step_inv, size, offset}
 		const float w = (d - tp.x) * tp.y;
 		int h = int(floorf(w));
@@ -128,11 +143,13 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_records(const float4* __restric
 }
 
 // dL/dknot of every snapshot in one persistent launch (work items as in k_fm_forces),
-// from the records: slot t (canonical x) adds, for each record (y, k, u),
-// g = ((G_y - G_x) . u) step_inv to knot k and -g to knot k + 1, in FP64; records are
-// read four at a time. Per-workgroup FP64 sums in LDS are written to
-// partial[blockIdx.x][knot], or global FP64 atomics go into partial[0] when the tables
-// exceed the LDS budget. Each workgroup visits a fixed sequence of work items, but the
+// from the records: slot t adds, for each record (o, k, v),
+// g = ((G_o - G_t) . v) step_inv to S[k], in FP64; records are read four at a time.
+// Each pair adds g to knot k and -g to knot k + 1, so dL/dU[k] = S[k] - S[k - 1] (a
+// table's last knot never starts a record's bin, so its S is 0 and the difference
+// never reaches across tables); the reduction forms it. Per-workgroup FP64 sums in LDS
+// are written to partial[blockIdx.x][knot], or global FP64 atomics go into partial[0]
+// when the tables exceed the LDS budget. Each workgroup visits a fixed sequence of work items, but the
 // order of the atomic additions within it is not fixed: the sums are deterministic up
 // to FP64 rounding (about 1e-16 relative), not bit for bit.
 template<bool LDS>
@@ -181,9 +198,9 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint_rec(const FmSnapView* _
 				const double dfm = (double(gy[e].x) - double(gx.x)) * double(u.x) +
 								   (double(gy[e].y) - double(gx.y)) * double(u.y) +
 								   (double(gy[e].z) - double(gx.z)) * double(u.z);
-				const double g = dfm * double(u.w);
-				atomicAdd(&acc[kv[e]], g);
-				atomicAdd(&acc[kv[e] + 1], -g);
+				// S[k] += g only: the pair's -g on knot k + 1 is applied once at the end,
+				// as dL/dU[k] = S[k] - S[k - 1] (half the atomic additions)
+				atomicAdd(&acc[kv[e]], dfm * double(u.w));
 			}
 		}
 	}
@@ -211,14 +228,18 @@ __global__ void k_fm_reduce_rows(const double* __restrict__ partial, uint32_t bl
 		s += partial[uint64_t(b) * total + k];
 	seg[uint64_t(y) * total + k] = s;
 }
+// out[k] = S[k] - S[k - 1], S[k] the sum of seg[y][k] over the segments in order.
 __global__ void k_fm_reduce_segs(const double* __restrict__ seg, uint32_t total, double* __restrict__ out) {
 	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= total)
 		return;
-	double s = 0.0;
-	for (uint32_t y = 0; y < kFmSegs; ++y)
+	double s = 0.0, sp = 0.0;
+	for (uint32_t y = 0; y < kFmSegs; ++y) {
 		s += seg[uint64_t(y) * total + k];
-	out[k] = s;
+		if (k > 0)
+			sp += seg[uint64_t(y) * total + k - 1];
+	}
+	out[k] = s - sp;
 }
 
 // The mean squared force error and its gradient in one pass over S x n records:
@@ -247,15 +268,23 @@ __global__ __launch_bounds__(kMseBlock) void k_fm_mse(const float4* __restrict__
 	if (threadIdx.x == 0)
 		block_sums[blockIdx.x] = part[0];
 }
-// loss = (sum of the block sums, in block order) / (3 S n), one thread.
-__global__ void k_fm_mse_total(const double* __restrict__ block_sums, uint32_t blocks, double inv_count,
-							   double* __restrict__ loss) {
-	if (threadIdx.x != 0 || blockIdx.x != 0)
-		return;
-	double s = 0.0;
-	for (uint32_t b = 0; b < blocks; ++b)
-		s += block_sums[b];
-	*loss = s * inv_count;
+// loss = (sum of the block sums) / (3 S n) with one workgroup in a fixed order: thread t
+// sums blocks t, t + 256, ..., then a tree over the threads.
+__global__ __launch_bounds__(kMseBlock) void k_fm_mse_total(const double* __restrict__ block_sums, uint32_t blocks,
+														   double inv_count, double* __restrict__ loss) {
+	__shared__ double part[kMseBlock];
+	double acc = 0.0;
+	for (uint32_t b = threadIdx.x; b < blocks; b += kMseBlock)
+		acc += block_sums[b];
+	part[threadIdx.x] = acc;
+	__syncthreads();
+	for (int o = kMseBlock / 2; o > 0; o >>= 1) {
+		if (int(threadIdx.x) < o)
+			part[threadIdx.x] += part[threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		*loss = part[0] * inv_count;
 }
 
 __global__ void k_fill_f64(double* p, uint64_t n) {
@@ -391,7 +420,7 @@ void FMImpl::mse(const float* force4, const float* target3, float* grad4, double
 	k_fm_mse<<<blocks, kMseBlock, 0, s>>>(reinterpret_cast<const float4*>(force4), target3, count, scale,
 										  reinterpret_cast<float4*>(grad4), mse_sums.data());
 	MMD_LAUNCH_CHECK();
-	k_fm_mse_total<<<1, 64, 0, s>>>(mse_sums.data(), blocks, 1.0 / (3.0 * double(count)), loss);
+	k_fm_mse_total<<<1, kMseBlock, 0, s>>>(mse_sums.data(), blocks, 1.0 / (3.0 * double(count)), loss);
 	MMD_LAUNCH_CHECK();
 }
 

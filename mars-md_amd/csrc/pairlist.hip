@@ -704,12 +704,13 @@ bool PairlistImpl::walk_home(hipStream_t s, bool& outside) {
 	const int m = build_grid.m;
 	if (m < 1)
 		return false;
-	// home level: about 24..190 particles per home cell, below the build grid; the
-	// candidate cells are two levels finer than the home cells
+	// home level: at least home_target particles per home cell on average (by default
+	// about 24..190), below the build grid; the candidate cells are fine_offset levels
+	// finer than the home cells
 	int h = 0;
-	while (h + 1 <= m - 1 && (n >> (3 * (h + 1))) >= 24)
+	while (h + 1 <= m - 1 && (n >> (3 * (h + 1))) >= uint64_t(std::max(1, home_target)))
 		++h;
-	const int mf = std::min(10, h + 2);
+	const int mf = std::min(10, h + std::max(1, fine_offset));
 	const uint64_t tiles = (n + 63) / 64;
 	HomeParams P{};
 	P.spos = spos.data();
@@ -1013,6 +1014,27 @@ float ZSort::compute_max_displacement(const float4* pos, uint64_t nn, const floa
 	float d2;
 	std::memcpy(&d2, &bits, 4);
 	return std::sqrt(d2);
+}
+
+double PairlistImpl::rewalk_ms(hipStream_t s) {
+	if (!built || home_level < 0)
+		return -1.0;
+	hipEvent_t e[2];
+	MMD_HIP(hipEventCreate(&e[0]));
+	MMD_HIP(hipEventCreate(&e[1]));
+	MMD_HIP(hipEventRecord(e[0], s));
+	bool outside = false;
+	const bool ok = walk_home(s, outside);
+	MMD_HIP(hipEventRecord(e[1], s));
+	MMD_HIP(hipEventSynchronize(e[1]));
+	float ms = 0.0f;
+	MMD_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+	(void)hipEventDestroy(e[0]);
+	(void)hipEventDestroy(e[1]);
+	if (!ok || outside)
+		raise(MARSMD_ERR_RUNTIME, "pairlist: the home walk no longer applies to the last build");
+	pairs_valid = false;
+	return double(ms);
 }
 
 const int32_t* PairlistImpl::neighbor_pairs(hipStream_t s) {

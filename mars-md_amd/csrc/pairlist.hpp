@@ -88,6 +88,9 @@ struct PairlistImpl {
 	// home-cell walk: home-level cell ranges and the per-slot capacity of its rows
 	DevArray<uint32_t> home_begin, home_end;
 	uint32_t home_cap = 0;
+	// cell configuration of the home walk: home cells of the finest level holding at
+	// least home_target particles on average, candidate cells fine_offset levels below
+	int home_target = 24, fine_offset = 2;
 	std::vector<int4> stencil_host; // the home walk's fine-cell stencil
 	DevArray<int4> stencil_dev;
 	std::vector<float> stencil_key; // the parameters stencil_dev was made for
@@ -129,6 +132,10 @@ struct PairlistImpl {
 	uint64_t num_pairs() const { return nbr_entries / 2; }
 	NbrList list() const { return NbrList{counts.data(), tile_base.data(), nbr.data()}; }
 	const int32_t* neighbor_pairs(hipStream_t s);
+	// The home walk again over the last build's sorted positions with the current cell
+	// configuration (the same pair set; rows may list it in another order): its time in
+	// ms, or a negative value when the home walk does not apply.
+	double rewalk_ms(hipStream_t s);
 	void reference_cells(hipStream_t s, uint64_t* num_cells);
 	marsmd_pairlist_stats stats() const;
 

@@ -852,7 +852,8 @@ void PatchImpl::k_remap_canon(int32_t* idx, uint64_t count, hipStream_t s) {
 	MMD_LAUNCH_CHECK();
 }
 
-void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t num_steps, hipStream_t s) {
+void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t num_steps, hipStream_t s,
+					bool store_last) {
 	if (P.integrator != MARSMD_INTEGRATOR_BROWNIAN && P.integrator != MARSMD_INTEGRATOR_LANGEVIN)
 		raise(MARSMD_ERR_VALUE, "Unsupported integrator type: " + std::to_string(P.integrator));
 	if (n == 0)
@@ -889,7 +890,7 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 		S.deferred = deferred_kick ? 1 : 0;
 		for (int a = 0; a < 3; ++a)
 			S.ef[a] = P.electric_field[a];
-		const int store = (step == first_step + num_steps) ? 1 : 0;
+		const int store = (store_last && step == first_step + num_steps) ? 1 : 0;
 		float4* pnext = pos[1 - cur].data();
 		const bool bd = P.integrator == MARSMD_INTEGRATOR_BROWNIAN;
 		const bool an = (tables->nb.enabled & kAnalyticTerms) != 0;
@@ -924,6 +925,52 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 		MMD_LAUNCH_CHECK();
 		cur = 1 - cur;
 	}
+}
+
+void PatchImpl::set_cells(int home_target, int fine_offset) {
+	if (home_target < 1 || home_target > 4096)
+		raise(MARSMD_ERR_VALUE, "Patch: home_target must be in [1, 4096]");
+	if (fine_offset < 1 || fine_offset > 4)
+		raise(MARSMD_ERR_VALUE, "Patch: fine_offset must be in [1, 4]");
+	pl->home_target = home_target;
+	pl->fine_offset = fine_offset;
+}
+
+void PatchImpl::autotune_cells(const marsmd_run_params& P, uint64_t first_step, uint64_t num_steps, hipStream_t s,
+							   double* walk_out) {
+	uint64_t step = first_step;
+	if (!pl->built && num_steps > 0) { // the first build
+		run(P, step, 1, s);
+		++step;
+	}
+	double best = -1.0;
+	if (pl->built && pl->home_level >= 0) {
+		const auto measure = [&]() {
+			double b = 1e300;
+			for (int k = 0; k < 2; ++k)
+				b = std::min(b, pl->rewalk_ms(s));
+			return b;
+		};
+		int bt = pl->home_target, bf = pl->fine_offset;
+		best = measure();
+		for (int t : {8, 12, 16, 24, 48})
+			for (int f : {1, 2, 3}) {
+				if (t == bt && f == bf)
+					continue;
+				set_cells(t, f);
+				const double ms = measure();
+				if (ms < 0.99 * best) {
+					best = ms;
+					bt = t;
+					bf = f;
+				}
+			}
+		set_cells(bt, bf);
+		(void)pl->rewalk_ms(s); // the live list as the choice walks it
+	}
+	*walk_out = best > 0.0 ? best : 0.0;
+	if (step < first_step + num_steps)
+		run(P, step, first_step + num_steps - step, s);
 }
 
 void PatchImpl::set_launch(int bpc, bool split) {
@@ -963,25 +1010,34 @@ void PatchImpl::autotune(const marsmd_run_params& P, uint64_t first_step, uint64
 		for (int v : per_cu)
 			cands.push_back(Cand{v, sp});
 	}
-	// the candidates take turns step by step, so each is timed at every point of the
-	// rebuild period (the steps after a rebuild are slower: cold caches); each keeps the
-	// median of its step-kernel times
+	// the candidates take turns step by step, one round of steps (one step per candidate)
+	// enqueued before its events are read, so the stream stays busy and no launch waits
+	// on the host; the order rotates by one every round, so over the rounds each candidate
+	// is timed at every point of the rebuild period (the steps after a rebuild are
+	// slower: cold caches); only the tuned run's real last step stores its forces. Each
+	// candidate keeps the median of its step-kernel times.
 	const bool was_timing = timing;
-	const uint64_t rounds = num_steps / cands.size();
-	std::vector<std::vector<double>> times(cands.size());
+	const uint64_t nc = cands.size();
+	const uint64_t rounds = num_steps / nc;
+	const uint64_t last = first_step + num_steps;
+	std::vector<std::vector<double>> times(nc);
 	uint64_t step = first_step;
-	for (uint64_t r = 0; r < rounds; ++r)
-		for (size_t ci = 0; ci < cands.size(); ++ci) {
-			set_launch(cands[ci].bpc, cands[ci].split);
-			set_kernel_timing(true);
-			run(P, step, 1, s);
+	for (uint64_t r = 0; r < rounds; ++r) {
+		set_kernel_timing(true); // clears the recorded events
+		std::vector<size_t> order(nc);
+		for (uint64_t k = 0; k < nc; ++k) {
+			order[k] = size_t((k + r) % nc);
+			set_launch(cands[order[k]].bpc, cands[order[k]].split);
+			run(P, step, 1, s, step + 1 == last);
 			step += 1;
-			double ms = 0.0;
-			uint64_t k = 0;
-			kernel_timing(&ms, &k);
-			if (k)
-				times[ci].push_back(ms / double(k));
 		}
+		for (uint64_t k = 0; k < nc && 2 * k + 1 < timing_used; ++k) {
+			MMD_HIP(hipEventSynchronize(timing_events[2 * k + 1]));
+			float ms = 0.0f;
+			MMD_HIP(hipEventElapsedTime(&ms, timing_events[2 * k], timing_events[2 * k + 1]));
+			times[order[k]].push_back(double(ms));
+		}
+	}
 	for (size_t ci = 0; ci < cands.size(); ++ci) {
 		std::vector<double>& t = times[ci];
 		if (t.empty())

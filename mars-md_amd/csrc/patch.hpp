@@ -70,7 +70,9 @@ struct PatchImpl {
 	void integrate(float dt, float kT, int integrator, uint64_t step, hipStream_t s);
 	void finish_deferred_kick(float dt, hipStream_t s);
 	void reorder(hipStream_t s);
-	void run(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s);
+	// store_last: the last step keeps its forces in `force` (the caller reads them)
+	void run(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s,
+			 bool store_last = true);
 	void set_kernel_timing(bool enable);
 	void kernel_timing(double* total_ms, uint64_t* launches);
 	void set_launch(int blocks_per_cu, bool split_pmf);
@@ -78,6 +80,12 @@ struct PatchImpl {
 	// the step kernels, and keeps the fastest (its ms per step in *best_ms).
 	void autotune(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s,
 				  double* best_ms);
+	void set_cells(int home_target, int fine_offset);
+	// Steps num_steps real steps after choosing the pairlist walk's cell configuration
+	// (each candidate timed re-walking the first build's positions; the live list is
+	// left as the choice walks it); *walk_ms: the choice's walk time.
+	void autotune_cells(const marsmd_run_params& p, uint64_t first_step, uint64_t num_steps, hipStream_t s,
+						double* walk_ms);
 
   private:
 	void build(float pairlist_cutoff, hipStream_t s);

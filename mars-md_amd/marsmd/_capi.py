@@ -301,6 +301,9 @@ PROTOTYPES = {
     "marsmd_patch_get_kernel_timing": (_I32, [_P, C.POINTER(C.c_double), C.POINTER(_U64)]),
     "marsmd_patch_set_launch": (_I32, [_P, C.POINTER(PatchLaunch)]),
     "marsmd_patch_get_launch": (_I32, [_P, C.POINTER(PatchLaunch)]),
+    "marsmd_patch_set_cells": (_I32, [_P, _I32, _I32]),
+    "marsmd_patch_get_cells": (_I32, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "marsmd_patch_autotune_cells": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, C.POINTER(C.c_double), _P]),
     "marsmd_patch_autotune": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, C.POINTER(PatchLaunch),
                                      C.POINTER(C.c_double), _P]),
     "marsmd_batch_create": (_I32, [C.POINTER(BatchDesc), C.POINTER(_P)]),

@@ -124,6 +124,30 @@ class Patch:
                  _stream(stream))
         return {"blocks_per_cu": lc.blocks_per_cu, "split_pmf": bool(lc.split_pmf), "ms_per_step": ms.value}
 
+    def set_cells(self, home_target: int = 24, fine_offset: int = 2):
+        """Cell configuration of the pairlist walk (home cells holding home_target
+        particles on average, candidate cells fine_offset levels finer); the pair set
+        does not depend on it. Applies from the next build."""
+        api.call("marsmd_patch_set_cells", self._h, int(home_target), int(fine_offset))
+
+    def get_cells(self) -> tuple:
+        a, b = C.c_int32(0), C.c_int32(0)
+        api.call("marsmd_patch_get_cells", self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def autotune_cells(self, num_steps: int, first_step: int = 0, params: Optional[RunParams] = None,
+                       stream=None) -> dict:
+        """Steps first_step+1 .. first_step+num_steps after choosing the walk's cell
+        configuration, timed re-walking the first build's positions."""
+        p = (params or self.spec.params).to_c()
+        ms = C.c_double(0.0)
+        api.call("marsmd_patch_autotune_cells", self._h, C.byref(p), first_step, num_steps, C.byref(ms),
+                 _stream(stream))
+        if num_steps > 0:
+            self._prepared = True
+        t, f = self.get_cells()
+        return {"home_target": t, "fine_offset": f, "walk_ms": ms.value}
+
     def maybe_reorder(self, step: int, params: Optional[RunParams] = None, stream=None) -> bool:
         """ParticleReorderManager::maybe_reorder (ReorderManager.cpp:8-25) with the
         reorderPeriod fallback of SimManager::execute_force_calculation (:343-349)."""

@@ -13,7 +13,8 @@ that replica's patch id.
   same bits as the default eager launch, checked after each of several runs with host
   work and synchronisation between the replays, for BD and Langevin; the autotuner
   steps the same trajectory and its cached choice is reused;
-* particles outside their box are rejected;
+* particles outside their box are rejected; a coordinate that the wrap's rounding
+  leaves an ulp below the box face is kept, with the reference's pair set;
 * the walk's cell configuration per replica size bucket keeps every pair set (forces
   within 1e-5: only their FP32 summation order changes), and its autotuner picks one
   per bucket present, caches it, and a first build with it gives the oracle's pair sets.
@@ -96,6 +97,30 @@ def test_incomplete_reference_grid_after_pair_buffer_growth(cuda):
     ref = orc.build_pairlist(got[0]["pos"], seam.box, rc, max_pairs=grown)
     assert ref["grid"][:2] == (4, (1, 1, 1))
     assert len(ref["pairs"]) + 120 <= len(first["pairs"])
+    assert np.array_equal(sorted_pairs(b.get_neighbor_pairs(0)), ref["pairs"].astype(np.int64))
+
+
+def test_wrap_rounding_at_the_box_face_is_kept(cuda):
+    # L = (1474 / 1e-4)^(1/3) in FP32: PeriodicBox::wrap_scalar sends the float just below
+    # L to a small negative coordinate (floor((x - o) inv_L) rounds to 1). The next build
+    # keeps the particle (its Morton code clamped, as the reference's) and the pair set;
+    # dt = 0 keeps the positions fixed.
+    n = 1474
+    L = float((n / 1e-4) ** (1.0 / 3.0))
+    spec = configs.config0(n=n, L=L, seed=8)
+    spec.positions = np.array(spec.positions, np.float32)
+    spec.positions[0, 0] = np.nextafter(np.float32(L), np.float32(0.0))
+    small = _specs()[0]
+    spec.seed = small.seed
+    rc = 30.0
+    params = configs.RunParams(cutoff=rc, pairlist_cutoff=rc, rebuild_period=5, timestep=0.0, kT=configs.KT)
+    b = Batch([spec, small])
+    b.run(7, params=params)
+    cuda.cuda.synchronize()
+    got = b.copy_particles_to_host()
+    x = got[0]["pos"][:, 0]
+    assert x.min() < 0.0 and x.min() > -1e-3, x.min()
+    ref = orc.build_pairlist(got[0]["pos"], spec.box, rc)
     assert np.array_equal(sorted_pairs(b.get_neighbor_pairs(0)), ref["pairs"].astype(np.int64))
 
 

@@ -68,3 +68,39 @@ def test_patch_autotune_keeps_the_trajectory(cuda):
     assert choice["ms_per_step"] > 0.0
     assert p.get_launch() == {"blocks_per_cu": choice["blocks_per_cu"], "split_pmf": choice["split_pmf"]}
     assert np.array_equal(_state(p), _state(ref))
+
+
+def test_patch_cell_configuration_keeps_the_pair_set(cuda):
+    # the walk's cells change the order of neighbours in a row (so the FP32 summation
+    # order of the forces), not the pair set
+    from oracle import pyoracle as orc
+    from tests.conftest import rel_force_error, sorted_pairs
+
+    spec = _spec(api.INTEGRATOR_LANGEVIN)
+    ref = marsmd.Patch(spec)
+    ref.run(1)
+    p = marsmd.Patch(spec)
+    assert p.get_cells() == (24, 2)
+    p.set_cells(8, 3)
+    assert p.get_cells() == (8, 3)
+    p.run(1)
+    cuda.cuda.synchronize()
+    a = sorted_pairs(p.get_pairlist().get_neighbor_pairs().cpu().numpy())
+    b = sorted_pairs(ref.get_pairlist().get_neighbor_pairs().cpu().numpy())
+    assert np.array_equal(a, b)
+    fa, fb = p.copy_particles_to_host()["force"], ref.copy_particles_to_host()["force"]
+    assert rel_force_error(fa, fb.astype(np.float64)) <= 1e-5
+    with pytest.raises(marsmd.MarsError):
+        p.set_cells(0, 2)
+    # the tuner keeps the pair set of the build it re-walks and steps on
+    q = marsmd.Patch(spec)
+    choice = q.autotune_cells(20)
+    cuda.cuda.synchronize()
+    print(f"case: tuned cells {choice}", flush=True)
+    assert choice["walk_ms"] > 0.0 and q.get_cells() == (choice["home_target"], choice["fine_offset"])
+    q2 = marsmd.Patch(spec)
+    q2.set_cells(choice["home_target"], choice["fine_offset"])
+    q2.run(1)
+    cuda.cuda.synchronize()
+    ref_pairs = orc.build_pairlist(spec.pos4(), spec.box, spec.params.pairlist_cutoff)["pairs"].astype(np.int64)
+    assert np.array_equal(sorted_pairs(q2.get_pairlist().get_neighbor_pairs().cpu().numpy()), ref_pairs)
