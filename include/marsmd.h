@@ -534,6 +534,25 @@ marsmd_status marsmd_patch_get_cells(marsmd_patch* p, int32_t* home_target, int3
 marsmd_status marsmd_patch_autotune_cells(marsmd_patch* p, const marsmd_run_params* params,
                                           uint64_t first_step, uint64_t num_steps, double* walk_ms,
                                           marsmd_stream stream);
+/* Pruned rows of the fused step (marsmd_patch_run), for systems whose particles move
+ * little within a list's life: after each build the step walks a copy of the
+ * neighbour rows, each keeping (in list order) the entries within cutoff + 2 margin,
+ * and falls back to the full rows, on the device, once a particle has moved a margin
+ * from its position at the prune (less the rounding of FP32 distances), so the forces
+ * are the full rows' bits in every mode.
+ * mode 0 (default): off; 1: rows placed within each block of 256 slots by ascending
+ * length (every entry kept); 2: placed so and pruned; 3: pruned at their own slots.
+ * The margin is 1.5x the previous list life's largest displacement (+ skin / 50, at
+ * most skin / 2; a quarter of the skin for the first), or margin when margin >= 0.
+ * Applies from the next prune (the next build or position change). No counterpart in
+ * the reference. */
+marsmd_status marsmd_patch_set_pruning(marsmd_patch* p, int32_t mode, float margin);
+/* The live configuration: mode, the margin of the pruned rows (0: none, rows sorted or
+ * off), whether they keep every entry, and the largest displacement from the positions
+ * at the prune recorded by the fused steps since (0 when none is recorded; a
+ * synchronous read). */
+marsmd_status marsmd_patch_get_pruning(marsmd_patch* p, int32_t* mode, float* margin, int32_t* keep_all,
+                                       float* displacement);
 
 /* ------------------------------------------------------------------ replica batch
  * Many independent replicas of a periodic nonbonded system, each one the reference's

@@ -6,6 +6,7 @@
 #include "pairlist.hpp"
 #include "patch.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -543,6 +544,40 @@ marsmd_status marsmd_patch_get_cells(marsmd_patch* p, int32_t* home_target, int3
 		need(fine_offset, "fine_offset");
 		*home_target = p->impl.pl->home_target;
 		*fine_offset = p->impl.pl->fine_offset;
+	});
+}
+marsmd_status marsmd_patch_set_pruning(marsmd_patch* p, int32_t mode, float margin) {
+	return guard([&] {
+		need(p, "patch");
+		if (mode < 0 || mode > 3)
+			mmd::raise(MARSMD_ERR_VALUE, "Patch: pruning mode must be 0, 1, 2 or 3");
+		if (!(margin < 0.0f || std::isfinite(margin)))
+			mmd::raise(MARSMD_ERR_VALUE, "Patch: pruning margin must be finite");
+		p->impl.prune_mode = mode;
+		p->impl.prune_margin_fixed = margin < 0.0f ? -1.0f : margin;
+		p->impl.prune_valid = false;
+	});
+}
+marsmd_status marsmd_patch_get_pruning(marsmd_patch* p, int32_t* mode, float* margin, int32_t* keep_all,
+									   float* displacement) {
+	return guard([&] {
+		need(p, "patch");
+		need(mode, "mode");
+		need(margin, "margin");
+		need(keep_all, "keep_all");
+		need(displacement, "displacement");
+		mmd::PatchImpl& P = p->impl;
+		*mode = P.prune_mode;
+		*margin = P.prune_valid ? P.prune_margin : 0.0f;
+		*keep_all = P.prune_valid && P.prune_keep_all ? 1 : 0;
+		*displacement = 0.0f;
+		if (P.prune_recording && P.prune_disp.count >= mmd::kPruneDispWords) {
+			uint32_t bits = 0;
+			MMD_HIP(hipMemcpy(&bits, P.prune_disp.data() + 4, 4, hipMemcpyDeviceToHost));
+			float d2;
+			std::memcpy(&d2, &bits, 4);
+			*displacement = std::sqrt(d2);
+		}
 	});
 }
 marsmd_status marsmd_patch_autotune_cells(marsmd_patch* p, const marsmd_run_params* params, uint64_t first_step,

@@ -31,6 +31,26 @@ __device__ __forceinline__ void atomic_add4(float4* p, float x, float y, float z
 	atomic_add_f32(f + 2, z);
 }
 
+// One particle's share of a bonded term: added to fe[atom] with FP32 atomics (the
+// reference's sum, in no fixed order), or, with contrib, stored at its contribution
+// slot for a gather that sums each particle's shares in a fixed order.
+__device__ __forceinline__ void bonded_emit(float4* fe, float4* contrib, uint64_t slot, int atom, float x, float y,
+											float z, bool energy, float e) {
+	if (contrib) {
+		contrib[slot] = make_float4(x, y, z, energy ? e : 0.0f);
+		return;
+	}
+	atomic_add4(&fe[atom], x, y, z);
+	if (energy)
+		atomic_add_f32(reinterpret_cast<float*>(&fe[atom]) + 3, e);
+}
+// The shares of a term that adds nothing (skipped form, zero length, no particle).
+__device__ __forceinline__ void bonded_zero(float4* contrib, uint64_t first, int roles) {
+	if (contrib)
+		for (int r = 0; r < roles; ++r)
+			contrib[first + uint64_t(r)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 // PairNonbondedComputer::operator() (Pairwise.h:214-304) with ANALYTIC / TABULATED
 // selecting its branches; AnalyticalPairKernel::operator() (AnalyticalPairKernels.h:
 // 44-104) is the ANALYTIC-only instance. The pair's own (x, y) orientation is kept.
@@ -84,12 +104,15 @@ __global__ void k_pairwise(const int2* __restrict__ pairs, const float4* __restr
 // TabulatedBondComputer (BondComputer.h:163-191).
 __global__ void k_bonds(const int2* __restrict__ bonds, const float4* __restrict__ pos, float4* fe,
 						const marsmd_table* __restrict__ tables, const int32_t* __restrict__ tidx,
-						const int32_t* __restrict__ forms, DevBox box, int get_energy, uint64_t nb) {
+						const int32_t* __restrict__ forms, DevBox box, int get_energy, uint64_t nb,
+						float4* __restrict__ contrib) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= nb)
 		return;
-	if (forms && forms[i] != MARSMD_FORM_TABULATED)
+	if (forms && forms[i] != MARSMD_FORM_TABULATED) {
+		bonded_zero(contrib, 2 * i, 2);
 		return;
+	}
 	const int2 ij = bonds[i];
 	const float3 r = wrap_diff(box, sub3(pos[ij.y], pos[ij.x]));
 	const float d = sqrtf(len2(r));
@@ -98,18 +121,16 @@ __global__ void k_bonds(const int2* __restrict__ bonds, const float4* __restrict
 		const float inv = 1.0f / d;
 		u = make_float3(inv * r.x, inv * r.y, inv * r.z);
 	}
-	if (d < 1e-6f)
+	if (d < 1e-6f) {
+		bonded_zero(contrib, 2 * i, 2);
 		return;
+	}
 	const marsmd_table t = tables[tidx ? tidx[i] : 0];
 	const FE v = table_compute(d, t.pot, int(t.size), t.start, t.step_inv, t.is_periodic != 0);
 	const float fx = v.f * u.x, fy = v.f * u.y, fz = v.f * u.z;
-	atomic_add4(&fe[ij.x], -fx, -fy, -fz);
-	atomic_add4(&fe[ij.y], fx, fy, fz);
-	if (get_energy) {
-		const float h = v.e * 0.5f;
-		atomic_add_f32(reinterpret_cast<float*>(&fe[ij.x]) + 3, h);
-		atomic_add_f32(reinterpret_cast<float*>(&fe[ij.y]) + 3, h);
-	}
+	const float h = v.e * 0.5f;
+	bonded_emit(fe, contrib, 2 * i, ij.x, -fx, -fy, -fz, get_energy != 0, h);
+	bonded_emit(fe, contrib, 2 * i + 1, ij.y, fx, fy, fz, get_energy != 0, h);
 }
 
 // AngleGeometry + TabulatedAngleComputer (BondGeometry.h:13-45; BondComputer.h:231-268).
@@ -145,24 +166,23 @@ __device__ __forceinline__ void angle_forces(const DevBox& box, const float4& pa
 
 __global__ void k_angles(const int4* __restrict__ angles, const float4* __restrict__ pos, float4* fe,
 						 const marsmd_table* __restrict__ tables, const int32_t* __restrict__ tidx,
-						 const int32_t* __restrict__ forms, DevBox box, int get_energy, uint64_t na) {
+						 const int32_t* __restrict__ forms, DevBox box, int get_energy, uint64_t na,
+						 float4* __restrict__ contrib) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= na)
 		return;
-	if (forms && forms[i] != MARSMD_FORM_TABULATED)
+	if (forms && forms[i] != MARSMD_FORM_TABULATED) {
+		bonded_zero(contrib, 3 * i, 3);
 		return;
+	}
 	const int4 a = angles[i];
 	float3 f1, f3;
 	float third;
 	angle_forces(box, pos[a.x], pos[a.y], pos[a.z], tables[tidx ? tidx[i] : 0], f1, f3, third);
-	atomic_add4(&fe[a.x], f1.x, f1.y, f1.z);
-	atomic_add4(&fe[a.y], -(f1.x + f3.x), -(f1.y + f3.y), -(f1.z + f3.z));
-	atomic_add4(&fe[a.z], f3.x, f3.y, f3.z);
-	if (get_energy) {
-		atomic_add_f32(reinterpret_cast<float*>(&fe[a.x]) + 3, third);
-		atomic_add_f32(reinterpret_cast<float*>(&fe[a.y]) + 3, third);
-		atomic_add_f32(reinterpret_cast<float*>(&fe[a.z]) + 3, third);
-	}
+	const bool e = get_energy != 0;
+	bonded_emit(fe, contrib, 3 * i, a.x, f1.x, f1.y, f1.z, e, third);
+	bonded_emit(fe, contrib, 3 * i + 1, a.y, -(f1.x + f3.x), -(f1.y + f3.y), -(f1.z + f3.z), e, third);
+	bonded_emit(fe, contrib, 3 * i + 2, a.z, f3.x, f3.y, f3.z, e, third);
 }
 
 __device__ __forceinline__ float3 cross3(const float3& v, const float3& w) {
@@ -218,12 +238,15 @@ __device__ __forceinline__ void dihedral_forces(const DevBox& box, const float4&
 
 __global__ void k_dihedrals(const int4* __restrict__ dih, const float4* __restrict__ pos, float4* fe,
 							const marsmd_table* __restrict__ tables, const int32_t* __restrict__ tidx,
-							const int32_t* __restrict__ forms, DevBox box, int get_energy, uint64_t nd) {
+							const int32_t* __restrict__ forms, DevBox box, int get_energy, uint64_t nd,
+							float4* __restrict__ contrib) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= nd)
 		return;
-	if (forms && forms[i] != MARSMD_FORM_TABULATED)
+	if (forms && forms[i] != MARSMD_FORM_TABULATED) {
+		bonded_zero(contrib, 4 * i, 4);
 		return;
+	}
 	const int4 q = dih[i];
 	float3 f[4];
 	float quarter;
@@ -231,12 +254,7 @@ __global__ void k_dihedrals(const int4* __restrict__ dih, const float4* __restri
 	const int idx[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
 	for (int k = 0; k < 4; ++k)
-		atomic_add4(&fe[idx[k]], f[k].x, f[k].y, f[k].z);
-	if (get_energy) {
-#pragma unroll
-		for (int k = 0; k < 4; ++k)
-			atomic_add_f32(reinterpret_cast<float*>(&fe[idx[k]]) + 3, quarter);
-	}
+		bonded_emit(fe, contrib, 4 * i + uint64_t(k), idx[k], f[k].x, f[k].y, f[k].z, get_energy != 0, quarter);
 }
 
 // HarmonicRestraintComputer (BondComputer.h:109-129) with CalcDistance
@@ -244,13 +262,15 @@ __global__ void k_dihedrals(const int4* __restrict__ dih, const float4* __restri
 // (Bonded/Analytical.h:18-30) at r0 = 0: F = unit(wrap_diff(x - x0)) * (-k d), full U.
 __global__ void k_restraints(const int32_t* __restrict__ ids, const float4* __restrict__ pos, float4* fe,
 							 const float4* __restrict__ anchors, const float* __restrict__ kspring, DevBox box,
-							 int get_energy, uint64_t nr) {
+							 int get_energy, uint64_t nr, float4* __restrict__ contrib) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= nr)
 		return;
 	const int id = ids[i];
-	if (id < 0)
+	if (id < 0) {
+		bonded_zero(contrib, i, 1);
 		return;
+	}
 	const float3 r = wrap_diff(box, sub3(pos[id], anchors[i]));
 	const float d = sqrtf(len2(r));
 	float3 u = make_float3(0.0f, 0.0f, 0.0f);
@@ -261,9 +281,7 @@ __global__ void k_restraints(const int32_t* __restrict__ ids, const float4* __re
 	const float k = kspring[i];
 	const float dr = d - 0.0f;
 	const float force = -k * dr;
-	atomic_add4(&fe[id], u.x * force, u.y * force, u.z * force);
-	if (get_energy)
-		atomic_add_f32(reinterpret_cast<float*>(&fe[id]) + 3, 0.5f * k * dr * dr);
+	bonded_emit(fe, contrib, i, id, u.x * force, u.y * force, u.z * force, get_energy != 0, 0.5f * k * dr * dr);
 }
 
 // BDIntegrate::operator() (BD.h:44-92).
@@ -437,47 +455,47 @@ void launch_analytical_pair_nonbonded(const int32_t* pairs, const float* pos, fl
 
 void launch_tabulated_bonds(const int32_t* bonds, const float* pos, float* fe, const marsmd_table* tables,
 							const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
-							uint64_t nb, hipStream_t s) {
+							uint64_t nb, hipStream_t s, float4* contrib) {
 	if (nb == 0)
 		return;
 	k_bonds<<<grid_for(nb, 256), 256, 0, s>>>(reinterpret_cast<const int2*>(bonds),
 											   reinterpret_cast<const float4*>(pos), reinterpret_cast<float4*>(fe),
-											   tables, tidx, forms, make_devbox(box), energy ? 1 : 0, nb);
+											   tables, tidx, forms, make_devbox(box), energy ? 1 : 0, nb, contrib);
 	MMD_LAUNCH_CHECK();
 }
 
 void launch_tabulated_angles(const int32_t* angles, const float* pos, float* fe, const marsmd_table* tables,
 							 const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
-							 uint64_t na, hipStream_t s) {
+							 uint64_t na, hipStream_t s, float4* contrib) {
 	if (na == 0)
 		return;
 	k_angles<<<grid_for(na, 256), 256, 0, s>>>(reinterpret_cast<const int4*>(angles),
 												reinterpret_cast<const float4*>(pos), reinterpret_cast<float4*>(fe),
-												tables, tidx, forms, make_devbox(box), energy ? 1 : 0, na);
+												tables, tidx, forms, make_devbox(box), energy ? 1 : 0, na, contrib);
 	MMD_LAUNCH_CHECK();
 }
 
 void launch_tabulated_dihedrals(const int32_t* dihedrals, const float* pos, float* fe, const marsmd_table* tables,
 								const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
-								uint64_t nd, hipStream_t s) {
+								uint64_t nd, hipStream_t s, float4* contrib) {
 	if (nd == 0)
 		return;
 	k_dihedrals<<<grid_for(nd, 256), 256, 0, s>>>(reinterpret_cast<const int4*>(dihedrals),
 												   reinterpret_cast<const float4*>(pos),
 												   reinterpret_cast<float4*>(fe), tables, tidx, forms,
-												   make_devbox(box), energy ? 1 : 0, nd);
+												   make_devbox(box), energy ? 1 : 0, nd, contrib);
 	MMD_LAUNCH_CHECK();
 }
 
 void launch_harmonic_restraints(const int32_t* ids, const float* pos, float* fe, const float* anchors,
 								const float* kspring, const marsmd_box& box, bool energy, uint64_t nr,
-								hipStream_t s) {
+								hipStream_t s, float4* contrib) {
 	if (nr == 0)
 		return;
 	k_restraints<<<grid_for(nr, 256), 256, 0, s>>>(ids, reinterpret_cast<const float4*>(pos),
 													reinterpret_cast<float4*>(fe),
 													reinterpret_cast<const float4*>(anchors), kspring,
-													make_devbox(box), energy ? 1 : 0, nr);
+													make_devbox(box), energy ? 1 : 0, nr, contrib);
 	MMD_LAUNCH_CHECK();
 }
 

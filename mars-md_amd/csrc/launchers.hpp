@@ -15,18 +15,21 @@ void launch_analytical_pair_nonbonded(const int32_t* pairs, const float* pos, fl
 									  const uint32_t* tags, const marsmd_particle_types& types, const marsmd_box& box,
 									  uint32_t enabled, const marsmd_pair_params* params, bool energy, uint64_t np,
 									  float cutoff2, hipStream_t s);
+// The bonded launchers add each term's forces (and energy shares) to fe with FP32
+// atomics; with contrib they store them instead, share r of term i of a kind with R
+// particles per term at contrib[R i + r] (fe unused), for a fixed-order gather.
 void launch_tabulated_bonds(const int32_t* bonds, const float* pos, float* fe, const marsmd_table* tables,
 							const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
-							uint64_t nb, hipStream_t s);
+							uint64_t nb, hipStream_t s, float4* contrib = nullptr);
 void launch_tabulated_angles(const int32_t* angles, const float* pos, float* fe, const marsmd_table* tables,
 							 const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
-							 uint64_t na, hipStream_t s);
+							 uint64_t na, hipStream_t s, float4* contrib = nullptr);
 void launch_tabulated_dihedrals(const int32_t* dihedrals, const float* pos, float* fe, const marsmd_table* tables,
 								const int32_t* tidx, const int32_t* forms, const marsmd_box& box, bool energy,
-								uint64_t nd, hipStream_t s);
+								uint64_t nd, hipStream_t s, float4* contrib = nullptr);
 void launch_harmonic_restraints(const int32_t* ids, const float* pos, float* fe, const float* anchors,
 								const float* kspring, const marsmd_box& box, bool energy, uint64_t nr,
-								hipStream_t s);
+								hipStream_t s, float4* contrib = nullptr);
 void launch_bd(const marsmd_particle_view& v, const marsmd_particle_types& t, float dt, uint64_t step, float kT,
 			   uint64_t n, const marsmd_box& box, uint64_t seed, uint32_t ctr, const marsmd_grid_view* grids,
 			   const float ef[3], int scheme, hipStream_t s);

@@ -833,6 +833,7 @@ bool PairlistImpl::walk_home(hipStream_t s, bool& outside) {
 void PairlistImpl::build(const float4* pos, const uint32_t* slot_of_canon, bool canon_from_meta,
 						 const int32_t* types, uint64_t nn, float pairlist_cutoff, hipStream_t s) {
 	const auto t0 = std::chrono::high_resolution_clock::now();
+	++list_version;
 	for (int a = 0; a < 3; ++a)
 		if (!(box.size[a] > 0.0f))
 			raise(MARSMD_ERR_VALUE, "ZOrderPairlist has no simulation box. Call set_periodic_box() before building.");
@@ -1034,6 +1035,7 @@ double PairlistImpl::rewalk_ms(hipStream_t s) {
 	if (!ok || outside)
 		raise(MARSMD_ERR_RUNTIME, "pairlist: the home walk no longer applies to the last build");
 	pairs_valid = false;
+	++list_version;
 	return double(ms);
 }
 

@@ -96,6 +96,7 @@ struct PairlistImpl {
 	std::vector<float> stencil_key; // the parameters stencil_dev was made for
 	PinnedWords readback{8};
 	int home_level = -1; // level of the last home-cell walk (-1: count / scan / fill path)
+	uint64_t list_version = 0; // counts the builds and re-walks that rewrote the rows
 	uint64_t nbr_entries = 0; // = 2 * num_pairs
 	uint64_t nbr_storage = 0; // padded entries held by nbr
 	DevArray<uint8_t> scan_scratch;

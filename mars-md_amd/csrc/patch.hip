@@ -141,6 +141,143 @@ __global__ __launch_bounds__(kBlock) void k_integrate_baoab(float4* __restrict__
 	mom[t] = mo;
 }
 
+// ------------------------------------------------------------------ pruned rows
+// The fused step's view of the pruned rows (k_prune). Lane position t of a step tile
+// takes slot lane_slot[t] and that slot's pruned row list.{count, tile_base, nbr}[t].
+// A pruned row keeps, in list order, the entries that lay within cutoff + 2 margin at
+// the prune; an entry it drops is farther than the cutoff at every step at which no
+// particle has moved a margin from its position at the prune, and such an entry adds
+// exactly +-0 to the sums (the step kernels' tabulated path; the analytical path skips
+// it), so a pruned row gives the full row's bits. disp[k & 3] records the largest
+// squared displacement from the prune after fused step k of the prune's life (k = 0:
+// the first step after it), disp[4] the largest over the life; a step whose previous
+// record reaches limit2 = (margin - slack)^2 walks the full rows instead. limit2 = 0:
+// rows sorted only (every entry kept, no record).
+struct PruneView {
+	NbrList list;
+	const uint32_t* lane_slot; // NULL: no pruned rows (lane position = slot)
+	const float4* ref;         // positions at the prune
+	uint32_t* disp;
+	uint32_t since;            // fused steps since the prune before this one
+	float limit2;
+};
+constexpr uint32_t kPruneBins = 64; // rows by group count (0..62), dead lanes last
+
+// Whether entry q of the row of p is kept (squared distance within keep2).
+__device__ __forceinline__ bool prune_keep(const DevBox& box, const float4& p, const float4& q, float keep2) {
+	return len2(wrap_diff_uniform(box, sub3(q, p))) <= keep2;
+}
+
+// Pruned rows of the 256-slot block blockIdx.x (see PruneView): each slot counts the
+// entries of its row to keep (squared distance <= keep2, or all when keep_all), reading
+// the row four entries (one 16-byte load) and four positions at a time and keeping a
+// bit per entry (the first 128; later ones are tested again when written). With sort,
+// the slots take lane positions by ascending group count (ties: lower wavefront, then
+// lower lane first), else their own; each writes its kept entries in row order at its
+// lane position, padding the last group with itself (a zero distance, skipped by the
+// force kernels). The positions become the displacement reference; block 0 clears the
+// record.
+__global__ __launch_bounds__(256) void k_prune(const float4* __restrict__ pos, uint32_t n, NbrList L, DevBox box,
+											   float keep2, int keep_all, int sort, uint32_t cap,
+											   uint32_t* __restrict__ lane_slot, uint32_t* __restrict__ p_count,
+											   uint32_t* __restrict__ p_tile_base, uint32_t* __restrict__ p_nbr,
+											   float4* __restrict__ ref, uint32_t* __restrict__ disp) {
+	__shared__ uint32_t hist[4][kPruneBins];
+	__shared__ uint32_t base[kPruneBins];
+	const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+	const uint32_t s = blockIdx.x * 256u + tid;
+	const bool live = s < n;
+	if (blockIdx.x == 0 && tid < kPruneDispWords)
+		disp[tid] = 0u;
+	if (tid < 4u)
+		p_tile_base[blockIdx.x * 4u + tid] = (blockIdx.x * 4u + tid) * 64u * cap;
+	for (uint32_t i = tid; i < 4u * kPruneBins; i += 256u)
+		(&hist[0][0])[i] = 0u;
+	float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	uint32_t cnt = 0, kept = 0;
+	uint64_t keep_lo = 0, keep_hi = 0; // entries 0..63, 64..127
+	const uint4* __restrict__ row = nullptr;
+	if (live) {
+		p = pos[s];
+		ref[s] = p;
+		cnt = L.count[s];
+		row = reinterpret_cast<const uint4*>(L.nbr + L.tile_base[s >> 6]) + (s & 63u);
+		if (keep_all) {
+			kept = cnt;
+		} else {
+			for (uint32_t g = 0; 4u * g < cnt; ++g) {
+				const uint4 j = row[uint64_t(g) * 64u];
+				const float4 q0 = pos[j.x], q1 = pos[j.y], q2 = pos[j.z], q3 = pos[j.w];
+				// the pads past the count are the slot itself: never kept
+				uint32_t m4 = (prune_keep(box, p, q0, keep2) ? 1u : 0u) | (prune_keep(box, p, q1, keep2) ? 2u : 0u) |
+							  (prune_keep(box, p, q2, keep2) ? 4u : 0u) | (prune_keep(box, p, q3, keep2) ? 8u : 0u);
+				const uint32_t valid = cnt - 4u * g >= 4u ? 15u : (1u << (cnt - 4u * g)) - 1u;
+				m4 &= valid;
+				kept += __popc(m4);
+				if (g < 16u)
+					keep_lo |= uint64_t(m4) << (4u * g);
+				else if (g < 32u)
+					keep_hi |= uint64_t(m4) << (4u * (g - 16u));
+			}
+		}
+	}
+	uint32_t at = tid;
+	if (sort) {
+		const uint32_t bin = live ? min((kept + 3u) >> 2, kPruneBins - 2u) : kPruneBins - 1u;
+		__syncthreads();
+		// rank among the wavefront's lanes of the same bin, lower lanes first
+		uint32_t rank = 0;
+		uint64_t todo = __ballot(1);
+		while (todo) {
+			const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+			const uint32_t b = __shfl(bin, leader);
+			const uint64_t same = __ballot(bin == b);
+			if (bin == b) {
+				rank = __popcll(same & ((1ull << lane) - 1ull));
+				if (int(lane) == leader)
+					hist[wave][b] = __popcll(same);
+			}
+			todo &= ~same;
+		}
+		__syncthreads();
+		if (tid == 0) {
+			uint32_t a = 0;
+			for (uint32_t b = 0; b < kPruneBins; ++b) {
+				base[b] = a;
+				a += hist[0][b] + hist[1][b] + hist[2][b] + hist[3][b];
+			}
+		}
+		__syncthreads();
+		at = base[bin] + rank;
+		for (uint32_t w = 0; w < wave; ++w)
+			at += hist[w][bin];
+	}
+	if (!live)
+		return; // dead lanes sort last: every live slot's lane position is below n
+	const uint32_t t = blockIdx.x * 256u + at;
+	lane_slot[t] = s;
+	p_count[t] = kept;
+	const uint64_t pb = uint64_t(t >> 6) * 64u * cap;
+	uint32_t m = 0;
+	for (uint32_t g = 0; 4u * g < cnt; ++g) {
+		const uint4 j4 = row[uint64_t(g) * 64u];
+		const uint32_t jv[4] = {j4.x, j4.y, j4.z, j4.w};
+#pragma unroll
+		for (int e = 0; e < 4; ++e) {
+			const uint32_t k = 4u * g + uint32_t(e);
+			if (k >= cnt)
+				break;
+			const bool keep = keep_all || (k < 64u	 ? ((keep_lo >> k) & 1u) != 0u
+											: k < 128u ? ((keep_hi >> (k - 64u)) & 1u) != 0u
+													   : prune_keep(box, p, pos[jv[e]], keep2));
+			if (keep)
+				p_nbr[nbr_slot_index(pb, t, m++)] = jv[e];
+		}
+	}
+	for (; (m & 3u) != 0u; ++m)
+		p_nbr[nbr_slot_index(pb, t, m)] = s;
+}
+
 // The fused step: nonbonded + bonded (from `force` when has_bonded) + PMF + integrator,
 // persistent over XCD-local tiles so the table knots are staged once per workgroup.
 // Bounded to 128 VGPRs (4 waves per SIMD; the tricubic sampler alone would take ~140
@@ -151,10 +288,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 												 NbrList L, NbTables T, const float* __restrict__ knots_g,
 												 marsmd_particle_types types_arg,
 												 const marsmd_grid_view* __restrict__ grids, StepParams S,
-												 int has_bonded, int store_force, const float4* __restrict__ pmf_in) {
+												 int has_bonded, int store_force, const float4* __restrict__ pmf_in,
+												 PruneView PR) {
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
 	__shared__ marsmd_particle_types lds_types;
+	__shared__ float lds_red[kBlock / 64];
 	__shared__ TypeStep lds_steps[MARSMD_MAX_TYPES]; // the integrator's per-type constants
 	stage_types(types_arg, lds_types);
 	if (threadIdx.x < uint32_t(types_arg.num_types))
@@ -162,13 +301,23 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 																	 : baoab_type_step(types_arg, int(threadIdx.x), S);
 	stage_tables(T, knots_g, lds_knots, lds_nb);
 	const marsmd_particle_types& types = lds_types;
+	// the pruned rows hold while the previous step's displacement record is below the
+	// limit; else this step walks the full rows (the same bits, more entries)
+	const bool track = PR.lane_slot && PR.limit2 > 0.0f;
+	bool pruned = PR.lane_slot != nullptr;
+	if (track && PR.since > 0)
+		pruned = __uint_as_float(PR.disp[(PR.since - 1u) & 3u]) < PR.limit2;
+	const NbrList LL = pruned ? PR.list : L;
+	float disp2 = 0.0f;
 	const TileRange R = xcd_tiles(uint32_t((n + kBlock - 1) / kBlock));
 	for (uint32_t tile = R.tile; tile < R.end; tile += R.stride) {
-		const uint64_t t = uint64_t(tile) * kBlock + threadIdx.x;
-		if (t >= n)
+		const uint32_t lt = tile * uint32_t(kBlock) + threadIdx.x; // lane position
+		if (lt >= n)
 			continue;
+		const uint32_t t = PR.lane_slot ? PR.lane_slot[lt] : lt; // slot
 		const float4 p = pos[t];
-		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, lds_knots, knots_g, lds_nb, S.box, S.cutoff2, ENERGY);
+		float4 f = nb_force<ANALYTIC>(pruned ? lt : t, p, pos, LL, T, lds_knots, knots_g, lds_nb, S.box, S.cutoff2,
+									  ENERGY);
 		if (has_bonded) {
 			const float4 b = force[t];
 			f.x += b.x;
@@ -199,6 +348,26 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 		pos_next[t] = pn;
 		if (ENERGY || store_force)
 			force[t] = f;
+		if (track)
+			disp2 = fmaxf(disp2, len2(wrap_diff_uniform(S.box, sub3(pn, PR.ref[t]))));
+	}
+	if (track) {
+		// the workgroup's largest squared displacement (non-negative floats order as
+		// their bits) into this step's record and the life's
+		for (int o = 32; o > 0; o >>= 1)
+			disp2 = fmaxf(disp2, __shfl_xor(disp2, o));
+		if ((threadIdx.x & 63u) == 0u)
+			lds_red[threadIdx.x >> 6] = disp2;
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			float m = lds_red[0];
+			for (int w = 1; w < kBlock / 64; ++w)
+				m = fmaxf(m, lds_red[w]);
+			atomicMax(&PR.disp[PR.since & 3u], __float_as_uint(m));
+			atomicMax(&PR.disp[4], __float_as_uint(m));
+			if (blockIdx.x == 0)
+				PR.disp[(PR.since + 1u) & 3u] = 0u; // the next step's record starts empty
+		}
 	}
 }
 
@@ -244,6 +413,67 @@ __global__ void k_gather_state(const uint32_t* __restrict__ sidx, const uint32_t
 }
 
 // canonical -> slot remap of bonded index tuples (stride ints per term, first k used).
+// ------------------------------------------------------------------ bonded gather
+// The bonded terms' shares (the launchers' contribution slots: bonds 2 per term, then
+// angles 3, dihedrals 4, restraints 1) as (slot of the particle, share index) pairs,
+// for a stable sort by slot: each particle's shares then sit in share-index order.
+__global__ void k_share_keys(const int32_t* __restrict__ bonds, uint64_t nb, const int32_t* __restrict__ angles,
+							 uint64_t na, const int32_t* __restrict__ dih, uint64_t nd,
+							 const int32_t* __restrict__ restr, uint64_t nr, uint32_t* __restrict__ keys,
+							 uint32_t* __restrict__ vals) {
+	const uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint64_t a0 = 2 * nb, d0 = a0 + 3 * na, r0 = d0 + 4 * nd, total = r0 + nr;
+	if (q >= total)
+		return;
+	int32_t slot;
+	if (q < a0)
+		slot = bonds[q];
+	else if (q < d0)
+		slot = angles[4 * ((q - a0) / 3) + (q - a0) % 3];
+	else if (q < r0)
+		slot = dih[q - d0];
+	else
+		slot = restr[q - r0];
+	// a restraint without a particle (id < 0) has a zero share: sort it last
+	keys[q] = slot < 0 ? 0xffffffffu : uint32_t(slot);
+	vals[q] = uint32_t(q);
+}
+
+// [begin, end) of each slot's run of shares in the sorted keys (begin = end = 0 when
+// the slot has none: the arrays are cleared first).
+__global__ void k_share_ranges(const uint32_t* __restrict__ keys, uint64_t total, uint64_t n,
+							   uint32_t* __restrict__ begin, uint32_t* __restrict__ end) {
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= total)
+		return;
+	const uint32_t k = keys[i];
+	if (k >= n)
+		return;
+	if (i == 0 || keys[i - 1] != k)
+		begin[k] = uint32_t(i);
+	if (i + 1 == total || keys[i + 1] != k)
+		end[k] = uint32_t(i + 1);
+}
+
+// force[t] = (add ? force[t] : 0) + each of slot t's shares in share-index order, one
+// FP32 addition at a time (the reference adds them with atomics, in no fixed order).
+__global__ void k_bonded_gather(uint64_t n, const uint32_t* __restrict__ begin, const uint32_t* __restrict__ end,
+								const uint32_t* __restrict__ share_of, const float4* __restrict__ shares, int add,
+								float4* __restrict__ force) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	float4 f = add ? force[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	for (uint32_t e = begin[t]; e < end[t]; ++e) {
+		const float4 c = shares[share_of[e]];
+		f.x += c.x;
+		f.y += c.y;
+		f.z += c.z;
+		f.w += c.w;
+	}
+	force[t] = f;
+}
+
 __global__ void k_topo_to_slots(const int32_t* __restrict__ canon, int32_t* __restrict__ slots, uint64_t count,
 								const uint32_t* __restrict__ slot_of_canon, uint64_t n) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -565,6 +795,7 @@ void PatchImpl::rebuild_exclusions(hipStream_t s) {
 }
 
 void PatchImpl::copy_from_host(const float* pos4, const float* mom4, const int32_t* type_ids, uint64_t nn) {
+	prune_valid = false; // positions or rows change
 	if (nn > capacity)
 		raise(MARSMD_ERR_RUNTIME, "Cannot copy " + std::to_string(nn) + " particles to patch with capacity " +
 									  std::to_string(capacity));
@@ -633,6 +864,7 @@ void PatchImpl::copy_to_host(float* pos4, float* mom4, float* force4, int32_t* t
 }
 
 void PatchImpl::build(float pairlist_cutoff, hipStream_t s) {
+	prune_valid = false; // positions or rows change
 	marsmd_exclusion_view ev{};
 	if (num_excl) {
 		ev.offsets = reinterpret_cast<const int32_t*>(excl_offsets.data());
@@ -671,6 +903,7 @@ void PatchImpl::build(float pairlist_cutoff, hipStream_t s) {
 																	   4 * num_angles, slot_of_canon.data(), n);
 		MMD_LAUNCH_CHECK();
 	}
+	build_share_index(s);
 	built = true;
 }
 
@@ -682,10 +915,19 @@ bool PatchImpl::rebuild_policy(float pairlist_cutoff, float cutoff, uint64_t ste
 	force_rebuild = false;
 	rebuild = rebuild || forced;
 	const float skin = pairlist_cutoff - cutoff;
+	// the displacement record of the pruned rows' life, for the next margin (the build
+	// synchronises the stream before prepare_prune reads it)
+	prune_life_tracked = false;
+	if (rebuild && prune_recording) {
+		MMD_HIP(hipMemcpyAsync(prune_host.ptr, prune_disp.data() + 4, 4, hipMemcpyDeviceToHost, s));
+		prune_life_tracked = true;
+	}
 	if (rebuild && !forced && period != 0 && built && skin > 0.0f && n > 0)
 		rebuild = pl->max_displacement(pos[cur].data(), nullptr, n, s) > skin * 0.5f;
 	if (rebuild)
 		build(pairlist_cutoff, s);
+	else
+		prune_life_tracked = false;
 	return rebuild;
 }
 
@@ -735,31 +977,73 @@ void PatchImpl::calculate_nonbonded(float pairlist_cutoff, float cutoff, uint64_
 	MMD_LAUNCH_CHECK();
 }
 
-void PatchImpl::calculate_bonded(bool energy, hipStream_t s) {
+uint64_t PatchImpl::num_shares() const {
+	return 2 * num_bonds + 3 * num_angles + 4 * num_dihedrals + num_restraints;
+}
+
+// Each slot's bonded shares, in share-index order (after the topology's slot arrays
+// change: at every build).
+void PatchImpl::build_share_index(hipStream_t s) {
+	const uint64_t q = num_shares();
+	if (q == 0 || n == 0)
+		return;
+	if (q >= (uint64_t(1) << 32))
+		raise(MARSMD_ERR_CAPACITY, "Patch: more than 2^32 bonded shares");
+	share_keys.resize(2 * q);
+	share_vals.resize(2 * q);
+	share_begin.resize(n);
+	share_end.resize(n);
+	k_share_keys<<<grid_for(q, 256), 256, 0, s>>>(bonds_slot.data(), num_bonds, angles_slot.data(), num_angles,
+												   dihedrals_slot.data(), num_dihedrals, restraint_slot.data(),
+												   num_restraints, share_keys.data(), share_vals.data());
+	MMD_LAUNCH_CHECK();
+	const uint64_t need = radix_sort_scratch_bytes(q);
+	share_scratch.resize(std::max<uint64_t>(need, 1));
+	// every key bit (shares without a particle carry all ones; once per build)
+	radix_sort_pairs(share_keys.data(), share_vals.data(), share_keys.data() + q, share_vals.data() + q, q, 32,
+					 share_scratch.data(), need, s);
+	fill_u32(share_begin.data(), 0u, n, s);
+	fill_u32(share_end.data(), 0u, n, s);
+	k_share_ranges<<<grid_for(q, 256), 256, 0, s>>>(share_keys.data() + q, q, n, share_begin.data(),
+													 share_end.data());
+	MMD_LAUNCH_CHECK();
+}
+
+// The bonded terms' forces: every share stored by the launchers, then summed per slot
+// in a fixed order into `force` (added to it when add, else written), so the result
+// does not depend on the order in which the terms ran.
+void PatchImpl::calculate_bonded(bool energy, hipStream_t s, bool add) {
 	if (n == 0 || num_bonded_terms() == 0)
 		return;
 	if (!built)
 		raise(MARSMD_ERR_RUNTIME, "Patch: calculate_bonded_forces before the first nonbonded pass");
+	const uint64_t q = num_shares();
+	shares.resize(q);
+	float4* c = shares.data();
+	const float* p = reinterpret_cast<const float*>(pos[cur].data());
 	if (num_bonds)
-		launch_tabulated_bonds(bonds_slot.data(), reinterpret_cast<const float*>(pos[cur].data()),
-							   reinterpret_cast<float*>(force.data()), tables->bond_dev.data(), bond_table.data(),
-							   nullptr, box, energy, num_bonds, s);
+		launch_tabulated_bonds(bonds_slot.data(), p, nullptr, tables->bond_dev.data(), bond_table.data(), nullptr, box,
+							   energy, num_bonds, s, c);
+	c += 2 * num_bonds;
 	if (num_angles)
-		launch_tabulated_angles(angles_slot.data(), reinterpret_cast<const float*>(pos[cur].data()),
-								reinterpret_cast<float*>(force.data()), tables->angle_dev.data(), angle_table.data(),
-								nullptr, box, energy, num_angles, s);
+		launch_tabulated_angles(angles_slot.data(), p, nullptr, tables->angle_dev.data(), angle_table.data(), nullptr,
+								box, energy, num_angles, s, c);
+	c += 3 * num_angles;
 	if (num_dihedrals)
-		launch_tabulated_dihedrals(dihedrals_slot.data(), reinterpret_cast<const float*>(pos[cur].data()),
-								   reinterpret_cast<float*>(force.data()), tables->dihedral_dev.data(),
-								   dihedral_table.data(), nullptr, box, energy, num_dihedrals, s);
+		launch_tabulated_dihedrals(dihedrals_slot.data(), p, nullptr, tables->dihedral_dev.data(),
+								   dihedral_table.data(), nullptr, box, energy, num_dihedrals, s, c);
+	c += 4 * num_dihedrals;
 	if (num_restraints)
-		launch_harmonic_restraints(restraint_slot.data(), reinterpret_cast<const float*>(pos[cur].data()),
-								   reinterpret_cast<float*>(force.data()),
+		launch_harmonic_restraints(restraint_slot.data(), p, nullptr,
 								   reinterpret_cast<const float*>(restraint_anchor.data()), restraint_k.data(), box,
-								   energy, num_restraints, s);
+								   energy, num_restraints, s, c);
+	k_bonded_gather<<<grid_for(n, 256), 256, 0, s>>>(n, share_begin.data(), share_end.data(), share_vals.data() + q,
+													  shares.data(), add ? 1 : 0, force.data());
+	MMD_LAUNCH_CHECK();
 }
 
 void PatchImpl::integrate(float dt, float kT, int integrator, uint64_t step, hipStream_t s) {
+	prune_valid = false; // positions or rows change
 	// positions move: a build after this one is no longer presorted by a reorder
 	pl->presorted_next = false;
 	if (n == 0)
@@ -808,6 +1092,7 @@ void PatchImpl::finish_deferred_kick(float dt, hipStream_t s) {
 }
 
 void PatchImpl::reorder(hipStream_t s) {
+	prune_valid = false; // positions or rows change
 	// Patch::reorder_particles (Patch.cpp:456-480): System-mode Morton sort over the
 	// periodic box, relabel canonical indices, remap topology, force a rebuild.
 	if (n == 0)
@@ -852,6 +1137,58 @@ void PatchImpl::k_remap_canon(int32_t* idx, uint64_t count, hipStream_t s) {
 	MMD_LAUNCH_CHECK();
 }
 
+void PatchImpl::prepare_prune(float cutoff, float pairlist_cutoff, hipStream_t s) {
+	if (prune_mode == 0 || n == 0 || !pl->built) {
+		prune_valid = false;
+		prune_recording = false;
+		return;
+	}
+	if (prune_valid && prune_list_version == pl->list_version)
+		return;
+	const float skin = pairlist_cutoff - cutoff;
+	// margin 0: the rows sorted only, every entry kept, no displacement record
+	float margin = 0.0f;
+	if (prune_mode >= 2 && cutoff > 0.0f && skin > 0.0f) {
+		if (prune_margin_fixed >= 0.0f) {
+			margin = prune_margin_fixed;
+		} else if (prune_life_tracked) {
+			// the previous life's largest displacement, with room to spare
+			float d2 = 0.0f;
+			std::memcpy(&d2, prune_host.ptr, 4);
+			margin = 1.5f * std::sqrt(std::max(d2, 0.0f)) + 0.02f * skin;
+		} else {
+			margin = prune_margin > 0.0f ? prune_margin : 0.25f * skin;
+		}
+		margin = std::min(margin, 0.5f * skin); // at skin / 2 the rows keep every entry
+	}
+	prune_life_tracked = false;
+	const uint64_t blocks = (n + 255) / 256;
+	const uint32_t cap = (std::max<uint32_t>(pl->max_neighbors, 1u) + 3u) & ~3u;
+	if (blocks * 256u * uint64_t(cap) >= (uint64_t(1) << 32))
+		raise(MARSMD_ERR_RUNTIME, "Patch: pruned rows beyond 32-bit offsets");
+	prune_lane_slot.resize(n);
+	prune_count.resize(n);
+	prune_tile_base.resize(blocks * 4);
+	prune_nbr.resize(blocks * 256u * cap);
+	prune_ref.resize(n);
+	prune_disp.resize(8);
+	// keep entries within cutoff + 2 margin (squared, rounded up)
+	const double rk = double(cutoff) + 2.0 * double(margin);
+	const float keep2 = float(rk * rk * (1.0 + 1e-6));
+	const int keep_all = margin > 0.0f && margin < 0.5f * skin ? 0 : 1;
+	prune_keep_all = keep_all != 0;
+	k_prune<<<unsigned(blocks), 256, 0, s>>>(pos[cur].data(), uint32_t(n), pl->list(), make_devbox(box), keep2,
+											 keep_all, prune_mode == 3 ? 0 : 1, cap, prune_lane_slot.data(), prune_count.data(),
+											 prune_tile_base.data(), prune_nbr.data(), prune_ref.data(),
+											 prune_disp.data());
+	MMD_LAUNCH_CHECK();
+	prune_margin = margin;
+	prune_recording = margin > 0.0f;
+	prune_valid = true;
+	prune_list_version = pl->list_version;
+	prune_since = 0;
+}
+
 void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t num_steps, hipStream_t s,
 					bool store_last) {
 	if (P.integrator != MARSMD_INTEGRATOR_BROWNIAN && P.integrator != MARSMD_INTEGRATOR_LANGEVIN)
@@ -872,11 +1209,8 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 			efield[a] = P.electric_field[a];
 		interpolation = P.interpolation;
 		rebuild_policy(P.pairlist_cutoff, P.cutoff, step, P.rebuild_period, s);
-		if (has_bonded) {
-			k_zero_force<<<g, kBlock, 0, s>>>(force.data(), n);
-			MMD_LAUNCH_CHECK();
-			calculate_bonded(energy, s);
-		}
+		if (has_bonded)
+			calculate_bonded(energy, s, false); // written, not added: the step adds the rest
 		StepParams S{};
 		S.box = make_devbox(box);
 		S.cutoff2 = cutoff2;
@@ -898,6 +1232,22 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 		const bool pmf_in = split_pmf && num_grids > 0;
 		const StepFn kern = pick_step(bd, energy, an, pmf_in);
 		const unsigned grid = persistent_grid(reinterpret_cast<const void*>(kern), n);
+		prepare_prune(P.cutoff, P.pairlist_cutoff, s);
+		PruneView PR{};
+		if (prune_valid) {
+			PR.list = NbrList{prune_count.data(), prune_tile_base.data(), prune_nbr.data()};
+			PR.lane_slot = prune_lane_slot.data();
+			PR.ref = prune_ref.data();
+			PR.disp = prune_disp.data();
+			PR.since = prune_since++;
+			if (prune_margin > 0.0f) {
+				// the margin less the rounding of FP32 distances and of the periodic wrap
+				// (rows that keep every entry hold at any displacement: recorded only)
+				const float lmax = std::max({box.size[0], box.size[1], box.size[2]});
+				const float lim = prune_margin - (1e-3f + 8.0f * lmax * 0x1p-23f);
+				PR.limit2 = prune_keep_all ? 3.0e38f : lim > 0.0f ? lim * lim : 1e-30f;
+			}
+		}
 		if (pmf_in)
 			pmf_buf.resize(n);
 		if (timing) {
@@ -915,7 +1265,7 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 		}
 		kern<<<grid, kBlock, lds_bytes(), s>>>(pos[cur].data(), pnext, mom.data(), force.data(), n, pl->list(),
 											   tables->nb, tables->nb_knots.data(), types, grids_dev.data(), S,
-											   has_bonded, store, pmf_in ? pmf_buf.data() : nullptr);
+											   has_bonded, store, pmf_in ? pmf_buf.data() : nullptr, PR);
 		if (timing) {
 			MMD_HIP(hipEventRecord(timing_events[timing_used + 1], s));
 			timing_used += 2;

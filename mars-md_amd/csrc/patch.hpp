@@ -10,6 +10,9 @@
 
 namespace mmd {
 
+// words of the pruned rows' displacement record: four steps' maxima (ring), the life's
+constexpr uint64_t kPruneDispWords = 5;
+
 struct PatchImpl {
 	struct Tables;
 	uint64_t capacity = 0, patch_id = 0, base_seed = 0;
@@ -56,6 +59,30 @@ struct PatchImpl {
 	int blocks_per_cu = 0;
 	bool split_pmf = false;
 	DevArray<float4> pmf_buf;
+	// Pruned rows of the fused step (k_prune): the neighbour list's rows holding the
+	// entries within cutoff + 2 margin of the positions at the prune (all entries in
+	// sort-only mode), placed in each 256-slot block by ascending length (modes 1, 2)
+	// or at their own slots (mode 3). mode 0 (default): off, 1: the rows sorted only,
+	// 2: sorted and pruned, 3: pruned; the margin adapts to the displacement of the
+	// previous list's life (fixed when margin_fixed >= 0).
+	int prune_mode = 0;
+	float prune_margin_fixed = -1.0f;
+	float prune_margin = 0.0f;     // margin of the live pruned rows (0: sort only)
+	bool prune_keep_all = true;    // the live pruned rows keep every entry
+	bool prune_valid = false;      // the pruned rows match the list and the positions
+	uint64_t prune_list_version = 0;
+	uint32_t prune_since = 0;      // fused steps since the prune
+	bool prune_recording = false;    // the fused steps since the last prune record displacements
+	bool prune_life_tracked = false; // the previous life's record was copied to prune_host
+	uint64_t prune_fallback_steps = 0, prune_steps = 0;
+	DevArray<uint32_t> prune_lane_slot, prune_count, prune_tile_base, prune_nbr, prune_disp;
+	DevArray<float4> prune_ref;
+	// bonded shares (share r of term i of a kind with R particles at kind base + R i + r)
+	// and, per slot, its shares in share-index order: share_vals[q + begin .. q + end)
+	DevArray<float4> shares;
+	DevArray<uint32_t> share_keys, share_vals, share_begin, share_end;
+	DevArray<uint8_t> share_scratch;
+	PinnedWords prune_host{2};
 
 	explicit PatchImpl(const marsmd_patch_desc& d);
 	~PatchImpl();
@@ -66,7 +93,8 @@ struct PatchImpl {
 	void copy_to_host(float* pos4, float* mom4, float* force4, int32_t* type_ids, int32_t* ids, uint64_t n);
 	void calculate_nonbonded(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period, const float ef[3],
 							 int interp, bool energy, hipStream_t s);
-	void calculate_bonded(bool energy, hipStream_t s);
+	// Bonded forces into `force`: added to it (after calculate_nonbonded) or written.
+	void calculate_bonded(bool energy, hipStream_t s, bool add = true);
 	void integrate(float dt, float kT, int integrator, uint64_t step, hipStream_t s);
 	void finish_deferred_kick(float dt, hipStream_t s);
 	void reorder(hipStream_t s);
@@ -89,10 +117,15 @@ struct PatchImpl {
 
   private:
 	void build(float pairlist_cutoff, hipStream_t s);
+	// (Re)build the pruned rows when the list or the positions changed since the last
+	// prune (after a build: margin from the previous life's displacement).
+	void prepare_prune(float cutoff, float pairlist_cutoff, hipStream_t s);
 	bool rebuild_policy(float pairlist_cutoff, float cutoff, uint64_t step, uint64_t period, hipStream_t s);
 	void rebuild_exclusions(hipStream_t s);
 	void k_remap_canon(int32_t* idx, uint64_t count, hipStream_t s);
 	uint64_t num_bonded_terms() const { return num_bonds + num_angles + num_dihedrals + num_restraints; }
+	uint64_t num_shares() const;
+	void build_share_index(hipStream_t s);
 	unsigned lds_bytes() const;
 	unsigned persistent_grid(const void* kernel, uint64_t n);
 };

@@ -303,6 +303,9 @@ PROTOTYPES = {
     "marsmd_patch_get_launch": (_I32, [_P, C.POINTER(PatchLaunch)]),
     "marsmd_patch_set_cells": (_I32, [_P, _I32, _I32]),
     "marsmd_patch_get_cells": (_I32, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "marsmd_patch_set_pruning": (_I32, [_P, _I32, C.c_float]),
+    "marsmd_patch_get_pruning": (_I32, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_float)]),
     "marsmd_patch_autotune_cells": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, C.POINTER(C.c_double), _P]),
     "marsmd_patch_autotune": (_I32, [_P, C.POINTER(RunParams), _U64, _U64, C.POINTER(PatchLaunch),
                                      C.POINTER(C.c_double), _P]),

@@ -135,6 +135,22 @@ class Patch:
         api.call("marsmd_patch_get_cells", self._h, C.byref(a), C.byref(b))
         return a.value, b.value
 
+    def set_pruning(self, mode: int = 3, margin: float = -1.0):
+        """Pruned rows of the fused step: 0 off (the default), 1 rows placed by length
+        within each 256-slot block, 2 placed so and pruned to cutoff + 2 margin, 3
+        pruned in place (margin < 0: adapted to the displacement of the previous list
+        life). The forces are the full rows' bits in every mode."""
+        api.call("marsmd_patch_set_pruning", self._h, int(mode), float(margin))
+
+    def get_pruning(self) -> dict:
+        """mode, margin of the live pruned rows, whether they keep every entry, and the
+        largest displacement since the prune recorded by the fused steps."""
+        mode, margin, keep_all, disp = C.c_int32(0), C.c_float(0.0), C.c_int32(0), C.c_float(0.0)
+        api.call("marsmd_patch_get_pruning", self._h, C.byref(mode), C.byref(margin), C.byref(keep_all),
+                 C.byref(disp))
+        return {"mode": mode.value, "margin": margin.value, "keep_all": bool(keep_all.value),
+                "displacement": disp.value}
+
     def autotune_cells(self, num_steps: int, first_step: int = 0, params: Optional[RunParams] = None,
                        stream=None) -> dict:
         """Steps first_step+1 .. first_step+num_steps after choosing the walk's cell

@@ -5,7 +5,8 @@ with grids, take the position-dependent force from a kernel of its own; the
 autotuner picks among these on real steps. Each configuration is checked against the
 default launch, bit for bit on positions, momenta, forces and ids, after each of
 several runs (rebuilds, reorders and energy steps inside them), for Langevin with a
-tricubic grid and BD with a linear one; the autotuned run equals the default run.
+tricubic grid, BD with a linear one and the config-1 polymers (bonded forces summed
+per particle in a fixed order); the autotuned run equals the default run.
 """
 import numpy as np
 import pytest
@@ -33,9 +34,17 @@ def _spec(integrator):
     return spec
 
 
-@pytest.mark.parametrize("integrator", [api.INTEGRATOR_LANGEVIN, api.INTEGRATOR_BROWNIAN])
+def _polymers():
+    spec = configs.config1(chains=20, beads=200, L=300.0)
+    spec.params.energy_period = 5
+    return spec
+
+
+@pytest.mark.parametrize("integrator", [api.INTEGRATOR_LANGEVIN, api.INTEGRATOR_BROWNIAN, "polymers"])
 def test_patch_launch_configurations_are_bit_identical(cuda, integrator):
-    spec = _spec(integrator)
+    # the polymers' bonded shares are summed per particle in a fixed order, so their
+    # runs repeat bit for bit too
+    spec = _polymers() if integrator == "polymers" else _spec(integrator)
     chunks = [13, 9, 25]
     ref = marsmd.Patch(spec)
     want, step = [], 0
