@@ -683,6 +683,17 @@ marsmd_status marsmd_fm_adjoint(marsmd_fm* fm, const float* knots, const float* 
  * counterpart in the reference (it has no gradient path). */
 marsmd_status marsmd_fm_mse(marsmd_fm* fm, const float* force_energy4, const float* target3, float* grad_force4,
                             double* loss, marsmd_stream stream);
+/* The same forces and adjoint with each snapshot in its sorted order (the Morton order
+ * of its pairlist build) instead of its particle order, and the permutation of S x n
+ * rows of width 3 or 4 floats from particle order into sorted order (in != out). A
+ * loss evaluated in sorted order (target permuted once) reads and writes every array
+ * in order and lets the adjoint read dL/dF around each particle's slot. No counterpart
+ * in the reference. */
+marsmd_status marsmd_fm_forces_sorted(marsmd_fm* fm, const float* knots, float* force_energy4,
+                                      marsmd_stream stream);
+marsmd_status marsmd_fm_adjoint_sorted(marsmd_fm* fm, const float* knots, const float* grad_force4,
+                                       double* grad_knots, marsmd_stream stream);
+marsmd_status marsmd_fm_to_sorted(marsmd_fm* fm, const float* in, int32_t width, float* out, marsmd_stream stream);
 
 #ifdef __cplusplus
 } /* extern "C" */

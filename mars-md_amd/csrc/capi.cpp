@@ -800,5 +800,34 @@ marsmd_status marsmd_fm_adjoint(marsmd_fm* fm, const float* knots, const float* 
 		fm->impl.adjoint(knots, grad_force4, grad_knots, st(stream));
 	});
 }
+marsmd_status marsmd_fm_forces_sorted(marsmd_fm* fm, const float* knots, float* force_energy4,
+									  marsmd_stream stream) {
+	return guard([&] {
+		need(fm, "fm");
+		need(knots, "knots");
+		need(force_energy4, "force_energy4");
+		fm->impl.forces(knots, force_energy4, st(stream), true);
+	});
+}
+marsmd_status marsmd_fm_adjoint_sorted(marsmd_fm* fm, const float* knots, const float* grad_force4,
+									   double* grad_knots, marsmd_stream stream) {
+	return guard([&] {
+		need(fm, "fm");
+		need(knots, "knots");
+		need(grad_force4, "grad_force4");
+		need(grad_knots, "grad_knots");
+		fm->impl.adjoint(knots, grad_force4, grad_knots, st(stream), true);
+	});
+}
+marsmd_status marsmd_fm_to_sorted(marsmd_fm* fm, const float* in, int32_t width, float* out, marsmd_stream stream) {
+	return guard([&] {
+		need(fm, "fm");
+		need(in, "in");
+		need(out, "out");
+		if (in == out)
+			mmd::raise(MARSMD_ERR_VALUE, "force matching: to_sorted needs distinct in and out");
+		fm->impl.to_sorted(in, width, out, st(stream));
+	});
+}
 
 } // extern "C"

@@ -18,11 +18,13 @@
 // The snapshots are fixed, so everything the adjoint needs of a pair but the gradients
 // is fixed too: at creation each snapshot's pairs are reduced, once, to records of the
 // pairs that carry a gradient (tabulated, 1e-6 <= d <= cutoff, inside the table) at
-// their owning end t, holding the other end's canonical index o, the knot index of the
-// pair's bin and {v, step_inv}, v = u taken from t towards o (-u when t is the higher
-// end: the same r, negated exactly), so that dL/dfm = (G_o - G_t) . v. The adjoint
-// then streams records instead of the full neighbour rows and positions (half the
-// entries, no geometry).
+// their owning end t, holding the other end's slot o, the knot index of the pair's
+// bin and {v, step_inv}, v = u taken from t towards o (-u when t is the higher end:
+// the same r, negated exactly), so that dL/dfm = (G_o - G_t) . v. The adjoint then
+// streams records instead of the full neighbour rows and positions (half the entries,
+// no geometry), with dL/dF first gathered into slot order, so that G_t is read in
+// order and G_o from the slots around t (Morton order) instead of from anywhere in the
+// snapshot.
 #include "fm.hpp"
 
 #include "nbforce.hpp"
@@ -35,6 +37,9 @@ namespace mmd {
 namespace {
 
 constexpr int kFmBlock = 256;
+// The adjoint's workgroups: 512 lanes share one workgroup's FP64 knot sums in LDS, so
+// the LDS budget admits twice the waves of 256-lane workgroups.
+constexpr int kFmAdjBlock = 512;
 constexpr int kFmLdsGrad = 7936; // knots whose gradient a workgroup accumulates in LDS (62 KB)
 
 }
@@ -48,7 +53,7 @@ struct FmSnapView {
 	const uint32_t* nbr;
 	// adjoint records: record k of slot t at nbr_slot_index(tile_base[t / 64], t, k)
 	const uint32_t* rec_count;
-	const uint32_t* rec_y;  // canonical index of the pair's higher end
+	const uint32_t* rec_y;  // slot of the pair's other end
 	const int32_t* rec_k;   // knot index of the pair's bin (table offset + bin)
 	const float4* rec_u;    // {u, step_inv}
 };
@@ -57,11 +62,11 @@ namespace {
 
 // The forces of every snapshot in one persistent launch: work item w is tile w % tiles
 // of snapshot w / tiles (snapshot-major, XCD-local chunks), the tables staged once per
-// workgroup.
+// workgroup; written in each snapshot's particle order, or its sorted order.
 __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const FmSnapView* __restrict__ snaps, uint64_t n,
 														uint32_t tiles, uint32_t items, NbTables T,
 														const float* __restrict__ knots_g, DevBox box, float cutoff2,
-														float4* __restrict__ out) {
+														int sorted, float4* __restrict__ out) {
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
 	stage_tables(T, knots_g, lds_knots, lds_nb);
@@ -74,7 +79,7 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const FmSnapView* __rest
 		const FmSnapView V = snaps[k];
 		const NbrList L{V.counts, V.tile_base, V.nbr};
 		const float4 f = nb_force(uint32_t(t), V.spos[t], V.spos, L, T, lds_knots, knots_g, lds_nb, box, cutoff2, true);
-		out[uint64_t(k) * n + V.sorted_idx[t]] = f;
+		out[uint64_t(k) * n + (sorted ? t : V.sorted_idx[t])] = f;
 	}
 }
 
@@ -134,7 +139,7 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_records(const float4* __restric
 		if (h < 0)
 			h = 0;
 		const uint64_t at = nbr_slot_index(base, uint32_t(t), m);
-		rec_y[at] = cj;
+		rec_y[at] = L.nbr[nbr_slot_index(base, uint32_t(t), k)];
 		rec_k[at] = __float_as_int(tp.w) + h;
 		rec_u[at] = make_float4(inv * r.x, inv * r.y, inv * r.z, tp.y);
 		++m;
@@ -142,8 +147,28 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_records(const float4* __restric
 	rec_count[t] = m;
 }
 
+// Rows of W floats per particle (snapshot k = blockIdx.y) from particle order into
+// the snapshot's sorted (slot) order: out[k n + t] = in[k n + sorted_idx_k[t]].
+template<int W>
+__global__ void k_fm_to_sorted(const FmSnapView* __restrict__ snaps, uint64_t n, const float* __restrict__ in,
+							   float* __restrict__ out) {
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n)
+		return;
+	const uint64_t k = blockIdx.y;
+	const uint64_t src = (k * n + snaps[k].sorted_idx[t]) * W, dst = (k * n + t) * W;
+	if constexpr (W == 4) {
+		*reinterpret_cast<float4*>(out + dst) = *reinterpret_cast<const float4*>(in + src);
+	} else {
+#pragma unroll
+		for (int c = 0; c < W; ++c)
+			out[dst + c] = in[src + c];
+	}
+}
+
 // dL/dknot of every snapshot in one persistent launch (work items as in k_fm_forces),
-// from the records: slot t adds, for each record (o, k, v),
+// from the records and dL/dF in slot order (grad_all): slot t adds, for each record
+// (o, k, v),
 // g = ((G_o - G_t) . v) step_inv to S[k], in FP64; records are read four at a time.
 // Each pair adds g to knot k and -g to knot k + 1, so dL/dU[k] = S[k] - S[k - 1] (a
 // table's last knot never starts a record's bin, so its S is 0 and the difference
@@ -153,10 +178,10 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_records(const float4* __restric
 // order of the atomic additions within it is not fixed: the sums are deterministic up
 // to FP64 rounding (about 1e-16 relative), not bit for bit.
 template<bool LDS>
-__global__ __launch_bounds__(kFmBlock) void k_fm_adjoint_rec(const FmSnapView* __restrict__ snaps, uint64_t n,
-															 uint32_t tiles, uint32_t items, uint32_t total_knots,
-															 const float4* __restrict__ grad_all,
-															 double* __restrict__ partial) {
+__global__ __launch_bounds__(kFmAdjBlock) void k_fm_adjoint_rec(const FmSnapView* __restrict__ snaps, uint64_t n,
+																uint32_t tiles, uint32_t items, uint32_t total_knots,
+																const float4* __restrict__ grad_all,
+																double* __restrict__ partial) {
 	extern __shared__ double lds_grad[];
 	double* acc = LDS ? lds_grad : partial;
 	if (LDS)
@@ -166,30 +191,44 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint_rec(const FmSnapView* _
 	const TileRange R = xcd_tiles(items);
 	for (uint32_t w = R.tile; w < R.end; w += R.stride) {
 		const uint32_t snap = w / tiles;
-		const uint64_t t = uint64_t(w - snap * tiles) * kFmBlock + threadIdx.x;
+		const uint64_t t = uint64_t(w - snap * tiles) * kFmAdjBlock + threadIdx.x;
 		if (t >= n)
 			continue;
 		const FmSnapView V = snaps[snap];
-		const float4* __restrict__ grad = grad_all + uint64_t(snap) * n;
+		const float4* __restrict__ grad = grad_all + uint64_t(snap) * n; // slot order
 		const uint32_t cnt = V.rec_count[t];
 		if (cnt == 0)
 			continue;
-		const uint32_t ct = meta_canon(__float_as_uint(V.spos[t].w));
-		const float4 gx = grad[ct];
+		const float4 gx = grad[t];
 		const uint64_t row = V.tile_base[t >> 6] + uint64_t(t & 63u) * 4u;
-		for (uint32_t g0 = 0; g0 < cnt; g0 += 4) {
-			const uint64_t at = row + uint64_t(g0 >> 2) * 256u;
-			const uint4 ys = *reinterpret_cast<const uint4*>(V.rec_y + at);
-			const int4 ks = *reinterpret_cast<const int4*>(V.rec_k + at);
-			const uint32_t yv[4] = {ys.x, ys.y, ys.z, ys.w};
-			const int kv[4] = {ks.x, ks.y, ks.z, ks.w};
-			float4 uv[4], gy[4];
+		// group g's records (4 at a time) and the gradients at their other ends; the
+		// next group's records are in flight while one group is evaluated
+		uint4 ys = *reinterpret_cast<const uint4*>(V.rec_y + row);
+		int4 ks = *reinterpret_cast<const int4*>(V.rec_k + row);
+		float4 uv[4], gy[4];
 #pragma unroll
-			for (int e = 0; e < 4; ++e) {
-				const bool live = g0 + uint32_t(e) < cnt;
-				uv[e] = live ? V.rec_u[at + uint64_t(e)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-				gy[e] = live ? grad[yv[e]] : gx;
+		for (int e = 0; e < 4; ++e)
+			uv[e] = V.rec_u[row + uint64_t(e)];
+		{
+			const uint32_t yv[4] = {ys.x, ys.y, ys.z, ys.w};
+#pragma unroll
+			for (int e = 0; e < 4; ++e)
+				gy[e] = uint32_t(e) < cnt ? grad[yv[e]] : gx; // past the count: unwritten records
+		}
+		for (uint32_t g0 = 0;; g0 += 4) {
+			const bool more = g0 + 4 < cnt;
+			const uint64_t nat = row + uint64_t((g0 >> 2) + 1) * 256u;
+			uint4 ysn = ys;
+			int4 ksn = ks;
+			float4 un[4] = {uv[0], uv[1], uv[2], uv[3]};
+			if (more) {
+				ysn = *reinterpret_cast<const uint4*>(V.rec_y + nat);
+				ksn = *reinterpret_cast<const int4*>(V.rec_k + nat);
+#pragma unroll
+				for (int e = 0; e < 4; ++e)
+					un[e] = V.rec_u[nat + uint64_t(e)];
 			}
+			const int kv[4] = {ks.x, ks.y, ks.z, ks.w};
 #pragma unroll
 			for (int e = 0; e < 4; ++e) {
 				if (g0 + uint32_t(e) >= cnt)
@@ -201,6 +240,16 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_adjoint_rec(const FmSnapView* _
 				// S[k] += g only: the pair's -g on knot k + 1 is applied once at the end,
 				// as dL/dU[k] = S[k] - S[k - 1] (half the atomic additions)
 				atomicAdd(&acc[kv[e]], dfm * double(u.w));
+			}
+			if (!more)
+				break;
+			ys = ysn;
+			ks = ksn;
+			const uint32_t yv[4] = {ys.x, ys.y, ys.z, ys.w};
+#pragma unroll
+			for (int e = 0; e < 4; ++e) {
+				uv[e] = un[e];
+				gy[e] = g0 + 4u + uint32_t(e) < cnt ? grad[yv[e]] : gx;
 			}
 		}
 	}
@@ -316,6 +365,8 @@ FMImpl::FMImpl(const marsmd_fm_desc& d) : tables(new NbTables{}) {
 	S = d.num_snapshots;
 	if ((n + kFmBlock - 1) / kFmBlock * S >= (uint64_t(1) << 32))
 		raise(MARSMD_ERR_VALUE, "force matching: snapshots x particles / 256 must be below 2^32");
+	if (S > 65535)
+		raise(MARSMD_ERR_VALUE, "force matching: at most 65535 snapshots");
 	box = d.box;
 	cutoff2 = d.cutoff > 0.0f ? d.cutoff * d.cutoff : 0.0f;
 	NbTables& T = *tables;
@@ -391,9 +442,9 @@ FMImpl::~FMImpl() {
 
 // Persistent workgroups for a kernel over the S x tiles work items: as many as are
 // resident (occupancy x CUs), a multiple of 8, at most the items rounded up to 8.
-static unsigned fm_grid(const void* kernel, unsigned lds, uint64_t items) {
+static unsigned fm_grid(const void* kernel, unsigned lds, uint64_t items, int block = kFmBlock) {
 	int occ = 0, dev = 0, cus = 0;
-	MMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kFmBlock, lds));
+	MMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, lds));
 	MMD_HIP(hipGetDevice(&dev));
 	MMD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
 	uint64_t g = uint64_t(std::max(occ, 1)) * uint64_t(std::max(cus, 1));
@@ -401,13 +452,13 @@ static unsigned fm_grid(const void* kernel, unsigned lds, uint64_t items) {
 	return unsigned(std::max<uint64_t>((g + 7) / 8 * 8, 8));
 }
 
-void FMImpl::forces(const float* knots, float* out4, hipStream_t s) {
+void FMImpl::forces(const float* knots, float* out4, hipStream_t s, bool sorted) {
 	const unsigned lds = tables->lds ? unsigned(std::max(tables->total_knots, 2)) * 4u : 0u;
 	const uint32_t tiles = uint32_t((n + kFmBlock - 1) / kFmBlock);
 	const uint64_t items = uint64_t(tiles) * S;
 	const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_forces), lds, items);
 	k_fm_forces<<<grid, kFmBlock, lds, s>>>(snap_views.data(), n, tiles, uint32_t(items), *tables, knots,
-											make_devbox(box), cutoff2, reinterpret_cast<float4*>(out4));
+											make_devbox(box), cutoff2, sorted ? 1 : 0, reinterpret_cast<float4*>(out4));
 	MMD_LAUNCH_CHECK();
 }
 
@@ -424,27 +475,45 @@ void FMImpl::mse(const float* force4, const float* target3, float* grad4, double
 	MMD_LAUNCH_CHECK();
 }
 
-void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStream_t s) {
+void FMImpl::to_sorted(const float* in, int width, float* out, hipStream_t s) {
+	if (width != 3 && width != 4)
+		raise(MARSMD_ERR_VALUE, "force matching: rows of 3 or 4 floats");
+	const dim3 grid(grid_for(n, 256), unsigned(S));
+	if (width == 4)
+		k_fm_to_sorted<4><<<grid, 256, 0, s>>>(snap_views.data(), n, in, out);
+	else
+		k_fm_to_sorted<3><<<grid, 256, 0, s>>>(snap_views.data(), n, in, out);
+	MMD_LAUNCH_CHECK();
+}
+
+void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStream_t s, bool sorted) {
 	(void)knots; // the gradient of a linear table force does not depend on the knots
 	const uint32_t total = uint32_t(tables->total_knots);
 	const bool lds = total <= uint32_t(kFmLdsGrad);
-	const uint32_t tiles = uint32_t((n + kFmBlock - 1) / kFmBlock);
+	const uint32_t tiles = uint32_t((n + kFmAdjBlock - 1) / kFmAdjBlock);
 	const uint64_t items = uint64_t(tiles) * S;
 	const float4* g = reinterpret_cast<const float4*>(grad4);
+	if (!sorted) {
+		grad_slot.resize(S * n);
+		to_sorted(grad4, 4, reinterpret_cast<float*>(grad_slot.data()), s);
+		g = grad_slot.data();
+	}
 	uint32_t rows = 1;
 	if (lds) {
-		const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_adjoint_rec<true>), total * 8u, items);
+		const unsigned grid =
+			fm_grid(reinterpret_cast<const void*>(k_fm_adjoint_rec<true>), total * 8u, items, kFmAdjBlock);
 		rows = grid;
 		partial.resize(uint64_t(rows) * total);
-		k_fm_adjoint_rec<true><<<grid, kFmBlock, total * 8u, s>>>(snap_views.data(), n, tiles, uint32_t(items), total,
-																 g, partial.data());
+		k_fm_adjoint_rec<true><<<grid, kFmAdjBlock, total * 8u, s>>>(snap_views.data(), n, tiles, uint32_t(items),
+																	total, g, partial.data());
 	} else {
-		const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_adjoint_rec<false>), 0u, items);
+		const unsigned grid =
+			fm_grid(reinterpret_cast<const void*>(k_fm_adjoint_rec<false>), 0u, items, kFmAdjBlock);
 		partial.resize(total);
 		k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), total);
 		MMD_LAUNCH_CHECK();
-		k_fm_adjoint_rec<false><<<grid, kFmBlock, 0, s>>>(snap_views.data(), n, tiles, uint32_t(items), total, g,
-														  partial.data());
+		k_fm_adjoint_rec<false><<<grid, kFmAdjBlock, 0, s>>>(snap_views.data(), n, tiles, uint32_t(items), total,
+															 g, partial.data());
 	}
 	MMD_LAUNCH_CHECK();
 	seg_sums.resize(uint64_t(kFmSegs) * total);

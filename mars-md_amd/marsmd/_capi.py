@@ -332,6 +332,9 @@ PROTOTYPES = {
     "marsmd_fm_forces": (_I32, [_P, _P, _P, _P]),
     "marsmd_fm_mse": (_I32, [_P, _P, _P, _P, _P, _P]),
     "marsmd_fm_adjoint": (_I32, [_P, _P, _P, _P, _P]),
+    "marsmd_fm_forces_sorted": (_I32, [_P, _P, _P, _P]),
+    "marsmd_fm_adjoint_sorted": (_I32, [_P, _P, _P, _P, _P]),
+    "marsmd_fm_to_sorted": (_I32, [_P, _P, _I32, _P, _P]),
 }
 
 LIB_PATH = Path(__file__).resolve().parent / "libmarsmd_hip.so"

@@ -72,15 +72,43 @@ class ForceMatching:
         api.call("marsmd_fm_num_pairs", self._h, C.byref(v))
         return v.value
 
-    def forces(self, knots, stream=None):
-        """(S, n, 4) float32: force and energy of every snapshot for these knots."""
+    def _knots(self, knots):
         torch = _torch()
         k = knots.detach().to(device="cuda", dtype=torch.float32).contiguous()
         if k.numel() != self.total_knots:
             raise ValueError(f"expected {self.total_knots} knots, got {k.numel()}")
+        return k
+
+    def forces(self, knots, stream=None, sorted_order: bool = False):
+        """(S, n, 4) float32: force and energy of every snapshot for these knots, in
+        each snapshot's particle order (or its sorted order: see to_sorted)."""
+        torch = _torch()
+        k = self._knots(knots)
         out = torch.empty((self.S, self.n, 4), dtype=torch.float32, device="cuda")
-        api.call("marsmd_fm_forces", self._h, k.data_ptr(), out.data_ptr(), _stream(stream))
+        api.call("marsmd_fm_forces_sorted" if sorted_order else "marsmd_fm_forces", self._h, k.data_ptr(),
+                 out.data_ptr(), _stream(stream))
         return out
+
+    def to_sorted(self, rows, stream=None):
+        """(S, n, 3 or 4) float32 rows from each snapshot's particle order into its sorted
+        order (the Morton order of its pairlist build)."""
+        torch = _torch()
+        r = rows.detach().to(device="cuda", dtype=torch.float32).contiguous()
+        if r.shape[:2] != (self.S, self.n) or r.dim() != 3 or r.shape[2] not in (3, 4):
+            raise ValueError(f"rows must be ({self.S}, {self.n}, 3 or 4)")
+        out = torch.empty_like(r)
+        api.call("marsmd_fm_to_sorted", self._h, r.data_ptr(), r.shape[2], out.data_ptr(), _stream(stream))
+        return out
+
+    def _sorted_target(self, target):
+        """to_sorted(target), kept while the same target tensor is passed unchanged."""
+        key = (target.data_ptr(), target._version, tuple(target.shape), target.dtype)
+        cached = getattr(self, "_target_cache", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        t = self.to_sorted(target)
+        self._target_cache = (key, t)
+        return t
 
     def mse(self, forces, target, stream=None):
         """(loss, dL/dF) of the mean squared force error in one pass: loss a device FP64
@@ -101,16 +129,18 @@ class ForceMatching:
                  _stream(stream))
         return loss, grad
 
-    def adjoint(self, knots, grad_forces, stream=None):
-        """dL/dknot (float64, total knots) for dL/dF = grad_forces (S, n, 3 or 4)."""
+    def adjoint(self, knots, grad_forces, stream=None, sorted_order: bool = False):
+        """dL/dknot (float64, total knots) for dL/dF = grad_forces (S, n, 3 or 4), in
+        particle order (or sorted order)."""
         torch = _torch()
-        k = knots.detach().to(device="cuda", dtype=torch.float32).contiguous()
+        k = self._knots(knots)
         g = grad_forces.detach().to(device="cuda", dtype=torch.float32)
         if g.shape[-1] == 3:
             g = torch.nn.functional.pad(g, (0, 1))
         g = g.contiguous()
         out = torch.empty(self.total_knots, dtype=torch.float64, device="cuda")
-        api.call("marsmd_fm_adjoint", self._h, k.data_ptr(), g.data_ptr(), out.data_ptr(), _stream(stream))
+        api.call("marsmd_fm_adjoint_sorted" if sorted_order else "marsmd_fm_adjoint", self._h, k.data_ptr(),
+                 g.data_ptr(), out.data_ptr(), _stream(stream))
         return out
 
 
@@ -178,12 +208,15 @@ def _loss_function():
 
     class ForceMatchingLoss(torch.autograd.Function):
         """mean((F(knots) - target)^2) as two library calls (forces, then the loss and
-        its dL/dF in one pass); backward is the FP64 adjoint of that dL/dF."""
+        its dL/dF in one pass); backward is the FP64 adjoint of that dL/dF. Everything
+        runs in each snapshot's sorted order (the target permuted once and kept while
+        the same target tensor is passed): the loss is a sum over all rows, so its
+        value does not depend on their order."""
 
         @staticmethod
         def forward(ctx, knots, fm: ForceMatching, target):
-            f = fm.forces(knots)
-            loss, grad_f = fm.mse(f, target)
+            f = fm.forces(knots, sorted_order=True)
+            loss, grad_f = fm.mse(f, fm._sorted_target(target))
             ctx.fm = fm
             ctx.save_for_backward(knots, grad_f)
             return loss
@@ -191,7 +224,7 @@ def _loss_function():
         @staticmethod
         def backward(ctx, g):
             knots, grad_f = ctx.saved_tensors
-            gk = ctx.fm.adjoint(knots, grad_f)
+            gk = ctx.fm.adjoint(knots, grad_f, sorted_order=True)
             return (gk * g.to(gk.dtype)).to(knots.dtype), None, None
 
     return ForceMatchingLoss

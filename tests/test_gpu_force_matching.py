@@ -98,3 +98,28 @@ def test_fused_loss_matches_the_elementwise_loss(cuda):
     f2 = tabulated_pair_forces(k2, fm)
     ((f2 - target) ** 2).sum().div(f2.numel()).backward()
     assert torch.allclose(k1.grad.double(), k2.grad.double(), rtol=1e-5, atol=1e-12 * k1.grad.abs().max().item())
+
+
+def test_sorted_order_entry_points(cuda):
+    # forces and the adjoint in each snapshot's sorted order are the particle-order ones
+    # permuted by to_sorted; the loss keeps its sorted target while the target tensor
+    # is unchanged and permutes it again after an in-place change
+    torch = cuda
+    w, fm = _setup(n=2500, L=270.0, S=3, knots=256, seed=11)
+    knots = fm.knots()
+    f = fm.forces(knots)
+    fs = fm.forces(knots, sorted_order=True)
+    assert torch.equal(fm.to_sorted(f), fs)
+    assert torch.equal(fm.to_sorted(f[..., :3].contiguous()), fs[..., :3])
+    G = torch.randn((fm.S, fm.n, 4), device="cuda", dtype=torch.float32)
+    g = fm.adjoint(knots, G)
+    gs = fm.adjoint(knots, fm.to_sorted(G), sorted_order=True)
+    assert torch.allclose(g, gs, rtol=1e-12, atol=1e-12 * g.abs().max().item())
+    target = fm.forces(fm.knots(w.target_tables))[..., :3].clone()
+    l1 = force_matching_loss(knots, fm, target).item()
+    ref = ((f[..., :3].double() - target.double()) ** 2).sum().item() / target.numel()
+    assert abs(l1 - ref) <= 1e-6 * ref
+    target[0, :10] += 1.0
+    l2 = force_matching_loss(knots, fm, target).item()
+    ref2 = ((f[..., :3].double() - target.double()) ** 2).sum().item() / target.numel()
+    assert abs(l2 - ref2) <= 1e-6 * ref2 and l2 != l1
