@@ -337,3 +337,35 @@ def test_fused_run_matches_unfused_and_oracle(cuda, which):
     # FP32 noise grows slowly over a short run; the oracle's trajectory stays close
     close = _min_image(a["pos"][:, :3] - o["pos"][:, :3], L).max(axis=1) < 1e-2
     assert close.mean() > 0.999
+
+
+@pytest.mark.parametrize("moves", ["below_half_skin", "beyond_half_skin"])
+def test_period_step_rebuilds_only_beyond_half_the_skin(cuda, moves):
+    # Patch::calculate_nonbonded_forces (Patch.cpp:85-121): a rebuild-period step
+    # rebuilds only when a particle has moved more than skin / 2 since the last build;
+    # reordering is off, so only that check decides (BD time steps of 1e-6 and 5e-4 ns)
+    spec = configs.config0(n=4000, L=170.0, seed=3)
+    dt = 1e-6 if moves == "below_half_skin" else 5e-4
+    P = configs.RunParams(cutoff=30.0, pairlist_cutoff=35.0, rebuild_period=5, reorder_period=10**6, timestep=dt,
+                          kT=configs.KT, integrator=api.INTEGRATOR_BROWNIAN)
+    skin = P.pairlist_cutoff - P.cutoff
+    L = spec.box.size[0]
+    patch = marsmd.Patch(spec)
+    patch.run(5, params=P)  # step 1 builds
+    cuda.cuda.synchronize()
+    g = patch.copy_particles_to_host()  # the positions step 6 starts from
+    assert np.array_equal(g["id"], np.arange(spec.n))
+    moved = np.sqrt((_min_image(g["pos"][:, :3] - spec.pos4()[:, :3], L) ** 2).sum(axis=1)).max()
+    patch.run(1, first_step=5, params=P)  # step 6: a rebuild-period step
+    cuda.cuda.synchronize()
+    gp = sorted_pairs(patch.get_pairlist().get_neighbor_pairs().cpu().numpy())
+    first = orc.build_pairlist(spec.pos4(), spec.box, P.pairlist_cutoff)["pairs"].astype(np.int64)
+    fresh = orc.build_pairlist(g["pos"], spec.box, P.pairlist_cutoff)["pairs"].astype(np.int64)
+    print(f"case: {moves}, largest displacement {moved:.3f} A", flush=True)
+    if moves == "below_half_skin":
+        assert moved < 0.5 * skin
+        assert not np.array_equal(first, fresh)  # the positions did change the set
+        assert np.array_equal(gp, first)         # but the list is step 1's
+    else:
+        assert moved > 0.5 * skin
+        assert np.array_equal(gp, fresh)

@@ -187,6 +187,40 @@ def test_config0_pairs_and_reference_cells(cuda):
     assert np.array_equal(nb.cpu().numpy().view(np.uint32), onb)
 
 
+@pytest.mark.parametrize("n,L,rc,periodic,buffer", [
+    (20000, 300.0, 12.0, (True, True, True), 100000),    # 16 cells per axis
+    (50000, 500.0, 15.0, (True, False, True), 100000),   # 16, one open axis
+    (30000, 300.0, 9.0, (False, False, False), 100000),  # 16, every axis open
+    (60000, 1000.0, 30.0, (True, True, True), 10**7),    # 32 cells per axis
+    (300000, 2000.0, 30.0, (True, True, True), 10**7),   # 64 cells per axis
+])
+def test_reference_cell_tables_on_finer_grids(cuda, n, L, rc, periodic, buffer):
+    # select_cell_grid's finer grids (the pair buffer of the first build given as the
+    # pairlist's particle capacity, as Pairlist sizes it): the cell ranges and the
+    # BuildCellNeighborsKernel table (ZOrderNeighbor.h:55-139) equal the oracle's
+    rs = np.random.default_rng(n)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rs.uniform(0, L, size=(n, 3))
+    box = PeriodicBox((L, L, L), periodic=periodic)
+    print(f"case: n={n} L={L} rc={rc} periodic={periodic} buffer={buffer}", flush=True)
+    pl = marsmd.ZOrderPairlist(buffer, 1 << 62)
+    pl.set_periodic_box(box)
+    pl.build_pairlist(_dev(cuda, pos), n, rc)
+    cuda.cuda.synchronize()
+    gp = sorted_pairs(pl.get_neighbor_pairs().cpu().numpy())
+    ref = orc.build_pairlist(pos, box, rc, max_pairs=buffer)
+    m, r, slots = ref["grid"]
+    assert m >= 4
+    assert np.array_equal(gp, ref["pairs"].astype(np.int64))
+    st = pl.get_statistics()
+    assert (st.cell_grid_bits, tuple(st.cell_radii), st.neighbors_per_cell) == (m, r, slots)
+    b, e, nb = pl.get_reference_cells()
+    ob, oe, onb = orc.cell_tables(ref["codes"], m, r, slots, [int(x) for x in periodic])
+    assert np.array_equal(b.cpu().numpy().view(np.uint32), ob)
+    assert np.array_equal(e.cpu().numpy().view(np.uint32), oe)
+    assert np.array_equal(nb.cpu().numpy().view(np.uint32).ravel(), onb.ravel())
+
+
 def test_config1_pairs_with_exclusions(cuda):
     spec = configs.config1(chains=40, beads=250, L=400.0)
     pos = spec.pos4()
