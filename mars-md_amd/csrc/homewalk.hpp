@@ -230,10 +230,21 @@ __device__ __forceinline__ void wave_lds_sync() {
 // The hits of the wave's nh home particles (visited one by one from LDS, the next one
 // read ahead) among one block of up to 64 candidates (lane: slot j, position q) by the
 // reference's predicate: the mask over the block's lanes of home particle i, in lane i.
+// Lane i (< 64, wave-uniform) of the 64-bit per-lane mask {lo, hi} takes the wave-uniform
+// mask m: one v_writelane per half, the lane in M0 (a select over lanes would take a
+// compare, two moves of the mask into vector registers and two selects).
+__device__ __forceinline__ void set_lane_mask(int& lo, int& hi, uint64_t m, uint32_t i) {
+	const uint32_t mlo = uint32_t(m), mhi = uint32_t(m >> 32);
+	asm("v_writelane_b32 %0, %1, %2" : "+v"(lo) : "s"(mlo), "{m0}"(i));
+	asm("v_writelane_b32 %0, %1, %2" : "+v"(hi) : "s"(mhi), "{m0}"(i));
+}
+__device__ __forceinline__ uint64_t lane_mask(int lo, int hi) {
+	return uint64_t(uint32_t(lo)) | (uint64_t(uint32_t(hi)) << 32);
+}
+
 __device__ __forceinline__ uint64_t exact_hit_masks(const HomeParams& P, const float4* home_pos, uint32_t nh,
 													 uint32_t c0s, bool live, uint32_t j, const float4& q) {
-	const uint32_t lane = threadIdx.x & 63u;
-	uint64_t mine = 0;
+	int mine_lo = 0, mine_hi = 0;
 	float4 pn = home_pos[0];
 	for (uint32_t i = 0; i < nh; ++i) {
 		const float4 pi = pn;
@@ -242,10 +253,9 @@ __device__ __forceinline__ uint64_t exact_hit_masks(const HomeParams& P, const f
 		bool in = live && j != ti && pair_predicate_home(P.box, P.rc2, P.ex, pi, ti, q, j);
 		if (in && P.ref_filter)
 			in = in_reference_stencil(P, pi, q); // symmetric in its two ends
-		const uint64_t hits = __ballot(in);
-		mine = lane == i ? hits : mine;
+		set_lane_mask(mine_lo, mine_hi, __ballot(in), i);
 	}
-	return mine;
+	return lane_mask(mine_lo, mine_hi);
 }
 
 // The same masks from the home-frame prefilter (the candidate moved into the home frame
@@ -256,8 +266,8 @@ __device__ __forceinline__ uint64_t exact_hit_masks(const HomeParams& P, const f
 __device__ __forceinline__ uint64_t home_frame_hit_masks(const HomeParams& P, const float4* home_pos, uint32_t nh,
 														 uint32_t c0s, bool live, uint32_t j, const float4& q,
 														 float qx, float qy, float qz) {
-	const uint32_t lane = threadIdx.x & 63u;
-	uint64_t mine = 0, band = 0;
+	int mine_lo = 0, mine_hi = 0;
+	uint64_t band = 0;
 	float4 pn = home_pos[0];
 	for (uint32_t i = 0; i < nh; ++i) {
 		const float4 pi = pn;
@@ -266,8 +276,9 @@ __device__ __forceinline__ uint64_t home_frame_hit_masks(const HomeParams& P, co
 		const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
 		const uint64_t hits = __ballot(d2 <= P.pre_lo2);
 		band |= __ballot(d2 <= P.pre_hi2) & ~hits;
-		mine = lane == i ? hits : mine;
+		set_lane_mask(mine_lo, mine_hi, hits, i);
 	}
+	uint64_t mine = lane_mask(mine_lo, mine_hi);
 	if (band)
 		mine = exact_hit_masks(P, home_pos, nh, c0s, live, j, q);
 	return mine;
