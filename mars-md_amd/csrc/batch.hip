@@ -160,8 +160,9 @@ __global__ void k_b_cell_ranges(const uint32_t* __restrict__ codes, uint64_t tot
 }
 
 // The home-cell walk over the work items of all replicas (item_off[r] = first item of
-// replica r, one item per home cell).
-__global__ __launch_bounds__(256) void k_b_walk(const RepGeom* __restrict__ geom, const uint32_t* __restrict__ item_off,
+// replica r, one item per home cell), at 6 waves per SIMD (80 VGPRs) as the Patch's walk:
+// more waves hide the per-home-particle LDS reads.
+__global__ __launch_bounds__(256, 6) void k_b_walk(const RepGeom* __restrict__ geom, const uint32_t* __restrict__ item_off,
 												uint32_t R, uint32_t num_items, const float4* __restrict__ spos,
 												const uint32_t* __restrict__ home_begin,
 												const uint32_t* __restrict__ home_end,

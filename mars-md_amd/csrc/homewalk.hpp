@@ -492,8 +492,11 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 	}
 }
 
+// Bounded to 80 VGPRs (6 waves per SIMD, a few dwords of spill): the home-particle loop
+// waits on an LDS read per home particle, which more waves hide (re-walk at 1M beads
+// 0.48 -> 0.46 ms against 90 VGPRs and 5 waves).
 template<bool FILL>
-__global__ __launch_bounds__(256) void k_walk_home(HomeParams P, uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(256, 6) void k_walk_home(HomeParams P, uint32_t* __restrict__ counts,
 												   uint32_t* __restrict__ nbr) {
 	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
 	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
