@@ -173,6 +173,7 @@ __global__ __launch_bounds__(256, 6) void k_b_walk(const RepGeom* __restrict__ g
 	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
 	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
 	__shared__ float4 home_pos[kHomeWaves][kHomeLds];
+	__shared__ uint32_t keep[kHomeWaves][kHomeKeep];
 	const int wv = threadIdx.x >> 6;
 	const TileRange TR = xcd_tiles((num_items + kHomeWaves - 1) / kHomeWaves);
 	for (uint32_t gidx = TR.tile; gidx < TR.end; gidx += TR.stride) {
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(256, 6) void k_b_walk(const RepGeom* __restrict__ g
 		P.pre_hi2 = G.pre_hi2;
 		P.stencil = G.stencil_n ? stencils + G.stencil_off : nullptr;
 		P.stencil_n = G.stencil_n;
-		walk_home_cell<true>(P, item - item_off[r], seg_b[wv], seg_pre[wv], home_pos[wv], counts, nbr);
+		walk_home_cell<true>(P, item - item_off[r], seg_b[wv], seg_pre[wv], home_pos[wv], keep[wv], counts, nbr);
 	}
 }
 

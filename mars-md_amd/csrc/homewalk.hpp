@@ -67,6 +67,8 @@ constexpr int kHomeWaves = 4;
 constexpr int kSegMax = 448;
 // LDS per wave for the home particles: 64 positions, then 64 candidate slots
 constexpr int kHomeLds = 64 + 16;
+// LDS per wave for the queue of candidates kept by the bounding-box test
+constexpr int kHomeKeep = 128;
 
 struct HomeParams {
 	const float4* spos;
@@ -316,6 +318,57 @@ __device__ __forceinline__ void test_candidate_block(const HomeParams& P, const 
 	wave_lds_sync();
 }
 
+// The home-frame coordinates of a candidate at q (see candidate_at).
+__device__ __forceinline__ void home_frame(const HomeParams& P, bool live, bool frame, const float4& p0,
+										   const float4& q, float& qx, float& qy, float& qz) {
+	if (!frame) {
+		qx = q.x;
+		qy = q.y;
+		qz = q.z;
+		return;
+	}
+	qx = qy = qz = INFINITY;
+	if (live) {
+		qx = P.box.periodic[0] ? q.x - P.box.L[0] * rintf((q.x - p0.x) * P.box.inv_L[0]) : q.x;
+		qy = P.box.periodic[1] ? q.y - P.box.L[1] * rintf((q.y - p0.y) * P.box.inv_L[1]) : q.y;
+		qz = P.box.periodic[2] ? q.z - P.box.L[2] * rintf((q.z - p0.z) * P.box.inv_L[2]) : q.z;
+	}
+}
+
+// Candidate qi of segments [0, nseg) (ntot particles): its slot j and position q, and q
+// in the home frame (moved by whole box lengths next to the anchor p0, so the image is
+// exact; +inf on lanes without a candidate) when `frame`.
+__device__ __forceinline__ void candidate_at(const HomeParams& P, const uint32_t* seg_b, const uint32_t* seg_pre,
+											 uint32_t nseg, uint32_t qi, bool live, bool frame, const float4& p0,
+											 uint32_t& j, float4& q, float& qx, float& qy, float& qz) {
+	j = 0xFFFFFFFFu;
+	q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	if (live) {
+		uint32_t lo = 0, hi = nseg; // seg_pre[lo] <= qi < seg_pre[hi]
+		while (hi - lo > 1) {
+			const uint32_t mid = (lo + hi) >> 1;
+			if (seg_pre[mid] <= qi)
+				lo = mid;
+			else
+				hi = mid;
+		}
+		j = seg_b[lo] + (qi - seg_pre[lo]);
+		q = P.spos[j];
+	}
+	home_frame(P, live, frame, p0, q, qx, qy, qz);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+	for (int o = 32; o > 0; o >>= 1)
+		v = fminf(v, __shfl_xor(v, o));
+	return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+	for (int o = 32; o > 0; o >>= 1)
+		v = fmaxf(v, __shfl_xor(v, o));
+	return v;
+}
+
 // Tests the candidates of segments [0, nseg) (ntot particles) against the wave's home
 // particles (lane l holds home particle t = c0 + l when valid, with its count) and
 // records the hits. The candidates are spread over the lanes, 64 at a time, and the
@@ -323,11 +376,18 @@ __device__ __forceinline__ void test_candidate_block(const HomeParams& P, const 
 // is evaluated by exactly one lane, so a wave whose home cell holds fewer than 64
 // particles does not idle lanes on every test. Every row lists its neighbours in
 // candidate order.
+// With the home-frame prefilter, a candidate whose squared distance to the bounding box
+// of the home particles exceeds pre_hi2 is dropped first (the fine cells of the stencil
+// reach past that box: at 1M beads most candidates are dropped): that distance is at
+// most d2' to every home particle, computed with the same monotone operations, so each
+// dropped candidate is one the prefilter would decide "out" for every home particle.
+// The kept ones (in candidate order, through a 128-entry LDS queue `keep`) are tested
+// 64 at a time, so the pair set and each row's order are unchanged.
 template<bool FILL>
 __device__ __forceinline__ void home_test_segments(const HomeParams& P, const uint32_t* seg_b, uint32_t* seg_pre,
 												   uint32_t nseg, uint32_t ntot, bool valid, uint32_t t,
 												   const float4& p, uint32_t& cnt, uint32_t* __restrict__ nbr,
-												   float4* home_pos) {
+												   float4* home_pos, uint32_t* keep) {
 	const uint32_t lane = threadIdx.x & 63u;
 	if (lane == 0)
 		seg_pre[nseg] = ntot;
@@ -337,43 +397,71 @@ __device__ __forceinline__ void home_test_segments(const HomeParams& P, const ui
 	const uint32_t c0 = t - lane;                       // first home slot of this chunk
 	const uint32_t nh = uint32_t(__popcll(__ballot(valid))); // valid lanes are a prefix
 	const float4 p0 = home_pos[0];                       // the home frame's anchor
+	const bool fast = P.pre && !P.ref_filter && !P.ex.rigid_body_id && !P.ex.offsets;
+	if (!fast) {
+		for (uint32_t q0 = 0; q0 < ntot; q0 += 64) {
+			const bool live = q0 + lane < ntot;
+			uint32_t j;
+			float4 q;
+			float qx, qy, qz;
+			candidate_at(P, seg_b, seg_pre, nseg, q0 + lane, live, false, p0, j, q, qx, qy, qz);
+			test_candidate_block<false, FILL>(P, home_pos, cand, nh, c0, live, j, q, qx, qy, qz, cnt, nbr);
+		}
+		wave_lds_sync();
+		return;
+	}
+	const float lox = wave_min(valid ? p.x : INFINITY), hix = wave_max(valid ? p.x : -INFINITY);
+	const float loy = wave_min(valid ? p.y : INFINITY), hiy = wave_max(valid ? p.y : -INFINITY);
+	const float loz = wave_min(valid ? p.z : INFINITY), hiz = wave_max(valid ? p.z : -INFINITY);
+	const uint64_t lt = (uint64_t(1) << lane) - 1;
+	uint32_t nk = 0; // queued candidates (wave-uniform), fewer than 64 between blocks
 	for (uint32_t q0 = 0; q0 < ntot; q0 += 64) {
-		const uint32_t qi = q0 + lane;
-		const bool live = qi < ntot;
-		uint32_t j = 0xFFFFFFFFu;
-		float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		if (live) {
-			uint32_t lo = 0, hi = nseg; // seg_pre[lo] <= qi < seg_pre[hi]
-			while (hi - lo > 1) {
-				const uint32_t mid = (lo + hi) >> 1;
-				if (seg_pre[mid] <= qi)
-					lo = mid;
-				else
-					hi = mid;
+		const bool live = q0 + lane < ntot;
+		uint32_t j;
+		float4 q;
+		float qx, qy, qz;
+		candidate_at(P, seg_b, seg_pre, nseg, q0 + lane, live, true, p0, j, q, qx, qy, qz);
+		const float ax = fmaxf(fmaxf(lox - qx, qx - hix), 0.0f);
+		const float ay = fmaxf(fmaxf(loy - qy, qy - hiy), 0.0f);
+		const float az = fmaxf(fmaxf(loz - qz, qz - hiz), 0.0f);
+		const bool kept = live && !(fmaf(ax, ax, fmaf(ay, ay, az * az)) > P.pre_hi2);
+		const uint64_t km = __ballot(kept);
+		if (kept)
+			keep[nk + uint32_t(__popcll(km & lt))] = j;
+		nk += uint32_t(__popcll(km));
+		if (nk >= 64) {
+			wave_lds_sync();
+			{
+				const uint32_t jb = keep[lane];
+				const float4 qb = P.spos[jb];
+				float bx, by, bz;
+				home_frame(P, true, true, p0, qb, bx, by, bz);
+				test_candidate_block<true, FILL>(P, home_pos, cand, nh, c0, true, jb, qb, bx, by, bz, cnt, nbr);
 			}
-			j = seg_b[lo] + (qi - seg_pre[lo]);
-			q = P.spos[j];
+			const uint32_t rest = nk - 64;
+			const uint32_t v = lane < rest ? keep[64 + lane] : 0u;
+			wave_lds_sync();
+			if (lane < rest)
+				keep[lane] = v;
+			nk = rest;
+			wave_lds_sync();
 		}
-		if (P.pre && !P.ref_filter && !P.ex.rigid_body_id && !P.ex.offsets) {
-			// the candidate in the home frame (whole box lengths, so the image is exact);
-			// lanes without a candidate are infinitely far
-			float qx = INFINITY, qy = INFINITY, qz = INFINITY;
-			if (live) {
-				qx = P.box.periodic[0] ? q.x - P.box.L[0] * rintf((q.x - p0.x) * P.box.inv_L[0]) : q.x;
-				qy = P.box.periodic[1] ? q.y - P.box.L[1] * rintf((q.y - p0.y) * P.box.inv_L[1]) : q.y;
-				qz = P.box.periodic[2] ? q.z - P.box.L[2] * rintf((q.z - p0.z) * P.box.inv_L[2]) : q.z;
-			}
-			test_candidate_block<true, FILL>(P, home_pos, cand, nh, c0, live, j, q, qx, qy, qz, cnt, nbr);
-		} else {
-			test_candidate_block<false, FILL>(P, home_pos, cand, nh, c0, live, j, q, q.x, q.y, q.z, cnt, nbr);
-		}
+	}
+	if (nk > 0) {
+		wave_lds_sync();
+		const bool live = lane < nk;
+		const uint32_t jb = live ? keep[lane] : 0xFFFFFFFFu;
+		const float4 qb = live ? P.spos[jb] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		float bx, by, bz;
+		home_frame(P, live, true, p0, qb, bx, by, bz);
+		test_candidate_block<true, FILL>(P, home_pos, cand, nh, c0, live, jb, qb, bx, by, bz, cnt, nbr);
 	}
 	wave_lds_sync();
 }
 
 template<bool FILL>
 __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg_b, uint32_t* seg_pre, float4* home_pos,
-							   uint32_t* __restrict__ counts, uint32_t* __restrict__ nbr) {
+							   uint32_t* keep, uint32_t* __restrict__ counts, uint32_t* __restrict__ nbr) {
 	const int lane = threadIdx.x & 63;
 	const uint32_t hb = P.home_begin[home], he = P.home_end[home];
 	if (hb >= he)
@@ -464,7 +552,7 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 			const uint64_t mask = __ballot(c > 0);
 			const uint32_t nnew = uint32_t(__popcll(mask));
 			if (nseg + nnew > uint32_t(kSegMax)) {
-				home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr, home_pos);
+				home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr, home_pos, keep);
 				nseg = 0;
 				ntot = 0;
 				flushed = true;
@@ -479,7 +567,7 @@ __device__ void walk_home_cell(const HomeParams& P, uint32_t home, uint32_t* seg
 			nseg += nnew;
 			ntot += sum;
 		}
-		home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr, home_pos);
+		home_test_segments<FILL>(P, seg_b, seg_pre, nseg, ntot, valid, t, p, cnt, nbr, home_pos, keep);
 		cached = !flushed;
 		if (valid) {
 			counts[t] = cnt;
@@ -501,12 +589,13 @@ __global__ __launch_bounds__(256, 6) void k_walk_home(HomeParams P, uint32_t* __
 	__shared__ uint32_t seg_b[kHomeWaves][kSegMax];
 	__shared__ uint32_t seg_pre[kHomeWaves][kSegMax + 1];
 	__shared__ float4 home_pos[kHomeWaves][kHomeLds];
+	__shared__ uint32_t keep[kHomeWaves][kHomeKeep];
 	const int wv = threadIdx.x >> 6;
 	const TileRange R = xcd_tiles((P.num_home + kHomeWaves - 1) / kHomeWaves);
 	for (uint32_t g = R.tile; g < R.end; g += R.stride) {
 		const uint32_t home = g * kHomeWaves + uint32_t(wv);
 		if (home < P.num_home)
-			walk_home_cell<FILL>(P, home, seg_b[wv], seg_pre[wv], home_pos[wv], counts, nbr);
+			walk_home_cell<FILL>(P, home, seg_b[wv], seg_pre[wv], home_pos[wv], keep[wv], counts, nbr);
 	}
 }
 
