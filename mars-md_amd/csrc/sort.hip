@@ -6,8 +6,12 @@
 // Unit_test/System/zorder_tests.cpp:139-151). One pass per digit of 8 bits, or of 10
 // bits when that takes fewer passes (3 instead of 4 for 30-bit Morton keys, 1 instead
 // of 2 for keys below 2^10); the last digit is masked to key_bits:
-//   1. k_rs_hist     per-tile digit histogram (wave-aggregated LDS counts),
-//   2. exclusive scan over [digit][tile] (digit-major; reduce / spine / apply),
+//   1. k_rs_hist     per-tile digit histogram (wave-aggregated LDS counts), and each
+//                    digit's total over the tiles (one device-scope addition per tile
+//                    and digit present),
+//   2. k_rs_offsets  one workgroup per digit: the totals of the lower digits, then the
+//                    exclusive scan of its row of tile counts (the digit-major scan
+//                    over [digit][tile] in one launch),
 //   3. k_rs_scatter  per-tile stable ranks from wave64 ballot peer masks, staged in
 //                    LDS in digit order, then written to offset[digit][tile] + rank.
 // A tile is 4 waves x 16 rows x 64 lanes = 4096 keys; wave w owns the contiguous
@@ -51,7 +55,7 @@ template<int BITS>
 __global__ __launch_bounds__(kRsBlock) void k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n,
 													  int shift, uint32_t mask,
 													  uint32_t* __restrict__ tile_hist,
-													  uint32_t num_tiles) {
+													  uint32_t num_tiles, uint32_t* __restrict__ totals) {
 	constexpr uint32_t B = 1u << BITS;
 	__shared__ uint32_t h[B];
 	for (uint32_t i = threadIdx.x; i < B; i += kRsBlock)
@@ -75,8 +79,12 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_hist(const uint32_t* __restrict
 			atomicAdd(&h[d], uint32_t(__popcll(peers)));
 	}
 	__syncthreads();
-	for (uint32_t i = threadIdx.x; i < B; i += kRsBlock)
-		tile_hist[uint64_t(i) * num_tiles + blockIdx.x] = h[i];
+	for (uint32_t i = threadIdx.x; i < B; i += kRsBlock) {
+		const uint32_t c = h[i];
+		tile_hist[uint64_t(i) * num_tiles + blockIdx.x] = c;
+		if (c)
+			atomicAdd(&totals[i], c);
+	}
 }
 
 // Block-wide exclusive scan for blockDim.x = nthreads (multiple of 64, <= 1024).
@@ -109,6 +117,33 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum, uint32_t* t
 		*total = wsum[nw];
 	__syncthreads();
 	return out;
+}
+
+// Digit d = blockIdx.x: row d of tile counts becomes its offsets, (sum of the totals of
+// the digits below d) + (exclusive scan of the row), in place; the next pass's total of
+// digit d is cleared (that buffer is not read here).
+__global__ __launch_bounds__(kRsBlock) void k_rs_offsets(uint32_t* __restrict__ tile_hist, uint32_t num_tiles,
+														 const uint32_t* __restrict__ totals,
+														 uint32_t* __restrict__ totals_next) {
+	__shared__ uint32_t wsum[kRsBlock / 64 + 1];
+	const uint32_t d = blockIdx.x;
+	uint32_t below = 0;
+	for (uint32_t i = threadIdx.x; i < d; i += kRsBlock)
+		below += totals[i];
+	uint32_t base;
+	(void)block_exclusive_scan(below, wsum, &base);
+	if (threadIdx.x == 0)
+		totals_next[d] = 0u;
+	uint32_t* __restrict__ row = tile_hist + uint64_t(d) * num_tiles;
+	for (uint32_t c0 = 0; c0 < num_tiles; c0 += kRsBlock) {
+		const uint32_t i = c0 + threadIdx.x;
+		const uint32_t v = i < num_tiles ? row[i] : 0u;
+		uint32_t chunk;
+		const uint32_t ex = block_exclusive_scan(v, wsum, &chunk);
+		if (i < num_tiles)
+			row[i] = base + ex;
+		base += chunk;
+	}
 }
 
 // In-place exclusive scan of `count` values with one block of 1024 threads.
@@ -284,8 +319,7 @@ inline uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
 
 uint64_t radix_sort_scratch_bytes(uint64_t n) {
 	const uint64_t tiles = (n + kRsTile - 1) / kRsTile;
-	return align256(n * 4) * 2 + align256(tiles * kRsMaxBuckets * 4 + 4) +
-		   scan_scratch_bytes(tiles * kRsMaxBuckets);
+	return align256(n * 4) * 2 + align256(tiles * kRsMaxBuckets * 4 + 4) + align256(2 * kRsMaxBuckets * 4);
 }
 
 void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
@@ -309,13 +343,18 @@ void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
 	uint32_t* tmp_k = reinterpret_cast<uint32_t*>(sp);
 	uint32_t* tmp_v = reinterpret_cast<uint32_t*>(sp + align256(n * 4));
 	uint32_t* hist = reinterpret_cast<uint32_t*>(sp + 2 * align256(n * 4));
-	void* scan_tmp = sp + 2 * align256(n * 4) + align256(uint64_t(tiles) * kRsMaxBuckets * 4 + 4);
+	// digit totals of this pass and the next (k_rs_offsets clears the next pass's)
+	uint32_t* totals = reinterpret_cast<uint32_t*>(sp + 2 * align256(n * 4) +
+												   align256(uint64_t(tiles) * kRsMaxBuckets * 4 + 4));
 	const int bits = (key_bits + 9) / 10 < (key_bits + 7) / 8 ? 10 : 8;
 	const int passes = (key_bits + bits - 1) / bits;
 	const uint64_t buckets = uint64_t(1) << bits;
 	const uint32_t* src_k = keys_in;
 	const uint32_t* src_v = vals_in;
+	fill_u32(totals, 0u, buckets, s);
 	for (int p = 0; p < passes; ++p) {
+		uint32_t* tot = totals + (p & 1) * kRsMaxBuckets;
+		uint32_t* tot_next = totals + ((p + 1) & 1) * kRsMaxBuckets;
 		const bool to_out = ((passes - 1 - p) % 2) == 0;
 		uint32_t* dk = to_out ? keys_out : tmp_k;
 		uint32_t* dv = to_out ? vals_out : tmp_v;
@@ -323,11 +362,12 @@ void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
 		const int width = key_bits - shift < bits ? key_bits - shift : bits;
 		const uint32_t mask = (1u << width) - 1u;
 		if (bits == 10)
-			k_rs_hist<10><<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles);
+			k_rs_hist<10><<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles, tot);
 		else
-			k_rs_hist<8><<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles);
+			k_rs_hist<8><<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles, tot);
 		MMD_LAUNCH_CHECK();
-		exclusive_scan_u32(hist, hist, uint64_t(tiles) * buckets, scan_tmp, s);
+		k_rs_offsets<<<unsigned(buckets), kRsBlock, 0, s>>>(hist, tiles, tot, tot_next);
+		MMD_LAUNCH_CHECK();
 		if (bits == 10)
 			k_rs_scatter<10><<<tiles, kRsBlock, 0, s>>>(src_k, src_v, dk, dv, n, shift, mask, hist, tiles);
 		else
@@ -346,15 +386,15 @@ uint64_t scan_scratch_bytes(uint64_t n) {
 // ------------------------------------------------- sort without allocation
 // A workspace for radix_sort_pairs_static, fixed at build time: the same layout as
 // radix_sort_scratch_bytes(kStaticSortMaxKeys) (two key/value staging arrays, the
-// [digit][tile] histogram with its total, the scan's block sums). Every GPU that loads
-// the library holds one copy until the process ends. Nothing in it has to start at
-// zero: each pass writes every histogram entry and block sum before reading it.
+// [digit][tile] histogram, two rows of digit totals). Every GPU that loads the library
+// holds one copy until the process ends. Nothing in it has to start at zero: each pass
+// writes every histogram entry before reading it, and the digit totals are cleared by a
+// kernel at the start of each sort (and by each pass for the next one).
 namespace {
 constexpr uint64_t cx_align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
 constexpr uint64_t cx_sort_scratch_bytes(uint64_t n) {
-	return cx_align256(n * 4) * 2 +
-		   cx_align256((n + kRsTile - 1) / kRsTile * kRsMaxBuckets * 4 + 4) +
-		   cx_align256(((n + kRsTile - 1) / kRsTile * kRsMaxBuckets + kScanTile - 1) / kScanTile * 4 + 4);
+	return cx_align256(n * 4) * 2 + cx_align256((n + kRsTile - 1) / kRsTile * kRsMaxBuckets * 4 + 4) +
+		   cx_align256(2 * kRsMaxBuckets * 4);
 }
 constexpr uint64_t kStaticSortScratchBytes = cx_sort_scratch_bytes(kStaticSortMaxKeys);
 
