@@ -337,7 +337,7 @@ def main():
     # warmup: the autotuners' real steps (the walk's cell configuration timed on
     # re-walks of the first build's positions, then launch candidates timed on the step
     # kernels), then the configuration they keep
-    warmup = max(args.warmup, 60)
+    warmup = max(args.warmup, 120)
     cell_steps = 10
     cells = patch.autotune_cells(cell_steps)
     choice = patch.autotune(warmup - cell_steps, first_step=cell_steps)
