@@ -101,13 +101,14 @@ class ForceMatching:
         return out
 
     def _sorted_target(self, target):
-        """to_sorted(target), kept while the same target tensor is passed unchanged."""
-        key = (target.data_ptr(), target._version, tuple(target.shape), target.dtype)
+        """to_sorted(target), kept while the same tensor object is passed again without an
+        in-place change since (its version counter): a new tensor, even at the address of
+        an old one, is permuted again."""
         cached = getattr(self, "_target_cache", None)
-        if cached is not None and cached[0] == key:
-            return cached[1]
+        if cached is not None and cached[0] is target and cached[1] == target._version:
+            return cached[2]
         t = self.to_sorted(target)
-        self._target_cache = (key, t)
+        self._target_cache = (target, target._version, t)
         return t
 
     def mse(self, forces, target, stream=None):

@@ -123,3 +123,9 @@ def test_sorted_order_entry_points(cuda):
     l2 = force_matching_loss(knots, fm, target).item()
     ref2 = ((f[..., :3].double() - target.double()) ** 2).sum().item() / target.numel()
     assert abs(l2 - ref2) <= 1e-6 * ref2 and l2 != l1
+    # a new tensor object with other values is permuted again
+    other = target.clone()
+    other[1, :10] -= 2.0
+    l3 = force_matching_loss(knots, fm, other).item()
+    ref3 = ((f[..., :3].double() - other.double()) ** 2).sum().item() / other.numel()
+    assert abs(l3 - ref3) <= 1e-6 * ref3
