@@ -347,8 +347,10 @@ struct __attribute__((aligned(4))) GridRun4 {
 // (The host pass of the compile only parses these device functions.)
 #if defined(__HIP_DEVICE_COMPILE__)
 #define MMD_GLOBAL_AS __attribute__((address_space(1)))
+#define MMD_CONST_AS __attribute__((address_space(4)))
 #else
 #define MMD_GLOBAL_AS
+#define MMD_CONST_AS
 #endif
 typedef MMD_GLOBAL_AS const float gfloat;
 __device__ __forceinline__ gfloat* grid_data(const marsmd_grid_view& g) { return (gfloat*)g.data; }
@@ -537,7 +539,13 @@ __device__ __forceinline__ void for_each_grid(int gid, const marsmd_grid_view* _
 		const int lane = __ffsll((unsigned long long)pending) - 1;
 		const int ug = __builtin_amdgcn_readlane(gid, lane);
 		if (gid == ug) {
-			const marsmd_grid_view g = grids[ug]; // uniform copy: scalar registers
+			// the view through a wave-uniform pointer in the constant address space, so it
+			// is read with scalar loads into scalar registers (a vector load of the same
+			// record per lane puts a memory round trip and 32 VGPRs before the sampler)
+			const uint64_t a = reinterpret_cast<uint64_t>(grids + ug);
+			const uint64_t ua = uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(a))))) |
+								(uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(a >> 32))))) << 32);
+			const marsmd_grid_view g = *reinterpret_cast<MMD_CONST_AS const marsmd_grid_view*>(ua);
 			body(g);
 		}
 		pending &= ~__ballot(gid == ug);
