@@ -454,57 +454,90 @@ __device__ __forceinline__ float dot4(const float w[4], float v0, float v1, floa
 	return fmaf(w[3], v3, fmaf(w[2], v2, fmaf(w[1], v1, w[0] * v0)));
 }
 
-// sample_grid_cubic (BaseGridDevice.h:405-475): the x pass runs over the 4 x 4 (y, z)
-// tap lines, then y, then z, as the reference does. A stencil inside the grid reads its
-// 16 runs of 4 z taps at fixed offsets from its first tap; otherwise each tap axis is
-// mapped once (boundary condition) and the four z taps of a (x, y) line are one 16-byte
-// load when they do not cross a grid face.
-// INTERIOR: read a stencil that lies inside the grid at fixed offsets (fewer address
-// operations, more registers; for kernels that have registers to spare).
-template<bool INTERIOR = false>
-__device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, float py, float pz, int bc) {
+// The stencil of sample_grid_cubic (BaseGridDevice.h:405-475): home index h =
+// floor(grid coordinate) per axis, the fractions w, and the taps h - 1 .. h + 2.
+struct CubicStencil {
+	int hx, hy, hz;
+	float wx, wy, wz;
+};
+__device__ __forceinline__ CubicStencil cubic_stencil(const marsmd_grid_view& g, float px, float py, float pz) {
 	const float v[3] = {px - g.origin[0], py - g.origin[1], pz - g.origin[2]};
 	float gp[3];
 	mat_transform(g.basis_inv, v, gp);
-	const int hx = int(floorf(gp[0])), hy = int(floorf(gp[1])), hz = int(floorf(gp[2]));
-	const float wx = gp[0] - float(hx), wy = gp[1] - float(hy), wz = gp[2] - float(hz);
-	const CrWeights Wx = cr_weights(wx), Wy = cr_weights(wy), Wz = cr_weights(wz);
+	CubicStencil c;
+	c.hx = int(floorf(gp[0]));
+	c.hy = int(floorf(gp[1]));
+	c.hz = int(floorf(gp[2]));
+	c.wx = gp[0] - float(c.hx);
+	c.wy = gp[1] - float(c.hy);
+	c.wz = gp[2] - float(c.hz);
+	return c;
+}
+
+// The 64 taps rr[y][x][z] of a stencil inside a grid of fewer than 2^30 values: 16 runs
+// of 4 z taps at 32-bit byte offsets from the grid's (wave-uniform) base, the first run's
+// offset per lane and the others a uniform stride from it, so each load is a scalar base
+// plus a vector offset and the addresses take a handful of 32-bit adds.
+__device__ __forceinline__ void cubic_taps_interior(const marsmd_grid_view& g, const CubicStencil& c,
+													float rr[4][4][4]) {
+	const uint32_t ny = uint32_t(g.dims[1]), nz = uint32_t(g.dims[2]);
+	const uint32_t sy = 4u * nz, sx = sy * ny; // bytes per y row, per x plane
+	const uint32_t first = uint32_t(c.hx - 1) * sx + uint32_t(c.hy - 1) * sy + 4u * uint32_t(c.hz - 1);
+	MMD_GLOBAL_AS const char* __restrict__ base = (MMD_GLOBAL_AS const char*)g.data;
+#pragma unroll
+	for (int ix = 0; ix < 4; ++ix) {
+		const uint32_t ox = first + uint32_t(ix) * sx;
+#pragma unroll
+		for (int iy = 0; iy < 4; ++iy) {
+			const GridRun4 r = *(MMD_GLOBAL_AS const GridRun4*)(base + (ox + uint32_t(iy) * sy));
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				rr[iy][ix][q] = r.v[q];
+		}
+	}
+}
+
+// The same taps of any stencil: each tap axis is mapped once (boundary condition) and
+// the four z taps of an (x, y) line are one 16-byte load when they do not cross a grid
+// face.
+__device__ __forceinline__ void cubic_taps_mapped(const marsmd_grid_view& g, const CubicStencil& c, int bc,
+												  float rr[4][4][4]) {
 	const int nx = int(g.dims[0]), ny = int(g.dims[1]), nz = int(g.dims[2]);
 	const int64_t sx = int64_t(ny) * int64_t(nz);
-	float rr[4][4][4]; // [y][x][z] taps
-	if (INTERIOR && hx >= 1 && hx + 2 < nx && hy >= 1 && hy + 2 < ny && hz >= 1 && hz + 2 < nz) {
-		// every tap inside the grid: the 16 runs at offsets (x sx + y nz) from the first
-		gfloat* __restrict__ base = grid_data(g) + ((int64_t(hx - 1) * ny + (hy - 1)) * nz + (hz - 1));
+	int64_t ox[4];
 #pragma unroll
-		for (int iy = 0; iy < 4; ++iy) {
+	for (int i = 0; i < 4; ++i)
+		ox[i] = tap_offset(c.hx - 1 + i, nx, bc, sx);
 #pragma unroll
-			for (int ix = 0; ix < 4; ++ix) {
-				const GridRun4 r = *(MMD_GLOBAL_AS const GridRun4*)(base + (ix * sx + iy * nz));
+	for (int iy = 0; iy < 4; ++iy) {
+		const int64_t oy = tap_offset(c.hy - 1 + iy, ny, bc, int64_t(nz));
+#pragma unroll
+		for (int ix = 0; ix < 4; ++ix) {
+			if (ox[ix] < 0 || oy < 0) {
 #pragma unroll
 				for (int q = 0; q < 4; ++q)
-					rr[iy][ix][q] = r.v[q];
-			}
-		}
-	} else {
-		int64_t ox[4];
-#pragma unroll
-		for (int i = 0; i < 4; ++i)
-			ox[i] = tap_offset(hx - 1 + i, nx, bc, sx);
-#pragma unroll
-		for (int iy = 0; iy < 4; ++iy) {
-			const int64_t oy = tap_offset(hy - 1 + iy, ny, bc, int64_t(nz));
-#pragma unroll
-			for (int ix = 0; ix < 4; ++ix) {
-				if (ox[ix] < 0 || oy < 0) {
-#pragma unroll
-					for (int q = 0; q < 4; ++q)
-						rr[iy][ix][q] = 0.0f;
-				} else {
-					load_run<4>(grid_data(g) + ox[ix] + oy, hz - 1, nz, bc, rr[iy][ix]);
-				}
+					rr[iy][ix][q] = 0.0f;
+			} else {
+				load_run<4>(grid_data(g) + ox[ix] + oy, c.hz - 1, nz, bc, rr[iy][ix]);
 			}
 		}
 	}
+}
+
+// sample_grid_cubic (BaseGridDevice.h:405-475): the x pass runs over the 4 x 4 (y, z)
+// tap lines, then y, then z, as the reference does. Stencils inside a grid of fewer than
+// 2^30 values read their taps at 32-bit offsets, the others through the boundary
+// mapping: the same values, so the same bits either way.
+__device__ inline GridSample sample_cubic(const marsmd_grid_view& g, float px, float py, float pz, int bc) {
+	const CubicStencil c = cubic_stencil(g, px, py, pz);
+	const uint64_t nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+	float rr[4][4][4]; // [y][x][z] taps
+	if (nx * ny * nz < (1ull << 30) && c.hx >= 1 && int64_t(c.hx) + 2 < int64_t(nx) && c.hy >= 1 &&
+		int64_t(c.hy) + 2 < int64_t(ny) && c.hz >= 1 && int64_t(c.hz) + 2 < int64_t(nz))
+		cubic_taps_interior(g, c, rr);
+	else
+		cubic_taps_mapped(g, c, bc, rr);
+	const CrWeights Wx = cr_weights(c.wx), Wy = cr_weights(c.wy), Wz = cr_weights(c.wz);
 	float d1[4][4], v1[4][4];
 #pragma unroll
 	for (int iy = 0; iy < 4; ++iy) {
@@ -555,7 +588,7 @@ __device__ __forceinline__ void for_each_grid(int gid, const marsmd_grid_view* _
 // compute_position_dependent_force (src/Interactions/Nonbonded/Pmf.h:68-124) for one
 // interpolation scheme. The returned .w is the PMF energy (the reference's .t). Each
 // lane adds its type's terms in term order, as the reference does.
-template<bool CUBIC, bool INTERIOR = false>
+template<bool CUBIC>
 __device__ inline float4 position_dependent_force_t(float px, float py, float pz, int type,
 													const marsmd_particle_types& types,
 													const marsmd_grid_view* __restrict__ grids, const float ef[3]) {
@@ -577,7 +610,7 @@ __device__ inline float4 position_dependent_force_t(float px, float py, float pz
 			if (g.data == nullptr)
 				return;
 			const int bc = term.boundary_condition >= 0 ? term.boundary_condition : g.boundary_condition;
-			const GridSample s = CUBIC ? sample_cubic<INTERIOR>(g, px, py, pz, bc) : sample_linear(g, px, py, pz, bc);
+			const GridSample s = CUBIC ? sample_cubic(g, px, py, pz, bc) : sample_linear(g, px, py, pz, bc);
 			const float ns = -term.scale;
 			f.x += s.grad[0] * ns;
 			f.y += s.grad[1] * ns;
@@ -590,7 +623,7 @@ __device__ inline float4 position_dependent_force_t(float px, float py, float pz
 		for_each_grid(types.force_grid_id[type][a], grids, [&](const marsmd_grid_view& g) {
 			if (g.data == nullptr)
 				return;
-			const float v = CUBIC ? sample_cubic<INTERIOR>(g, px, py, pz, g.boundary_condition).value
+			const float v = CUBIC ? sample_cubic(g, px, py, pz, g.boundary_condition).value
 								  : sample_linear(g, px, py, pz, g.boundary_condition).value;
 			const float add = types.force_grid_scale[type][a] * v;
 			if (a == 0)
@@ -604,14 +637,13 @@ __device__ inline float4 position_dependent_force_t(float px, float py, float pz
 	return f;
 }
 
-template<bool INTERIOR = false>
 __device__ inline float4 position_dependent_force(float px, float py, float pz, int type,
 												  const marsmd_particle_types& types,
 												  const marsmd_grid_view* __restrict__ grids, const float ef[3],
 												  int scheme) {
 	return scheme == MARSMD_INTERP_LINEAR
-			   ? position_dependent_force_t<false, INTERIOR>(px, py, pz, type, types, grids, ef)
-			   : position_dependent_force_t<true, INTERIOR>(px, py, pz, type, types, grids, ef);
+			   ? position_dependent_force_t<false>(px, py, pz, type, types, grids, ef)
+			   : position_dependent_force_t<true>(px, py, pz, type, types, grids, ef);
 }
 
 __device__ __forceinline__ float atomic_add_f32(float* p, float v) {

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_pmf_force(const float4* __restrict__
 		if (t >= n)
 			continue;
 		const float4 p = pos[t];
-		out[t] = position_dependent_force<true>(p.x, p.y, p.z, int(meta_type(fb(p.w))), types,
+		out[t] = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types,
 										  S.has_grids ? grids : nullptr, S.ef, S.scheme);
 	}
 }
