@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restr
 		P.step = step;
 		P.seed = mix_seed(base_seed, G.pid, uint64_t(step));
 		const float4 p = pos[t];
-		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, lds_knots, knots_g, lds_nb, P.box, P.cutoff2, ENERGY);
+		float4 f = nb_force<ANALYTIC>(uint32_t(t), p, pos, L, T, lds_knots, knots_g, lds_nb, P.box, P.cut_d, ENERGY);
 		float4 pn;
 		if (INTEG == MARSMD_INTEGRATOR_BROWNIAN) {
 			const float4 pmf = position_dependent_force(p.x, p.y, p.z, int(meta_type(fb(p.w))), types, nullptr, P.ef,
@@ -981,7 +981,7 @@ void BatchImpl::launch_steps(const marsmd_run_params& P, uint64_t first, uint64_
 	const unsigned grid = step_grid(reinterpret_cast<const void*>(kern));
 	const unsigned lds = tables->lds ? unsigned(std::max(tables->total_knots, 2)) * 4u : 0u;
 	StepParams S{};
-	S.cutoff2 = P.cutoff > 0.0f ? P.cutoff * P.cutoff : 0.0f;
+	S.cut_d = cutoff_threshold(P.cutoff > 0.0f ? P.cutoff * P.cutoff : 0.0f);
 	S.dt = P.timestep;
 	S.kT = P.kT;
 	S.ctr = 0;

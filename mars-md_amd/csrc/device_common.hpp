@@ -16,15 +16,43 @@ namespace mmd {
 
 constexpr int kWave = 64;
 constexpr uint32_t kInvalidCell = 0xFFFFFFFFu;
-constexpr uint32_t kCanonBits = 24; // meta word of a position record: type << 24 | canon
+// Meta word of a position record: canon << 8 | type. The canonical index sits in the
+// high bits, so two particles' meta words order as their canonical indices (one
+// compare decides a pair's orientation), and the type is one AND away. The word is
+// only ever moved bitwise (as a float it may be any pattern, NaN included).
+constexpr uint32_t kCanonBits = 24;
+constexpr uint32_t kTypeBits = 8;
 constexpr uint32_t kCanonMask = (1u << kCanonBits) - 1u;
 constexpr uint64_t kMaxParticles = (1ull << kCanonBits);
 
 __host__ __device__ inline uint32_t make_meta(uint32_t type, uint32_t canon) {
-	return (type << kCanonBits) | (canon & kCanonMask);
+	return ((canon & kCanonMask) << kTypeBits) | (type & ((1u << kTypeBits) - 1u));
 }
-__host__ __device__ inline uint32_t meta_canon(uint32_t m) { return m & kCanonMask; }
-__host__ __device__ inline uint32_t meta_type(uint32_t m) { return m >> kCanonBits; }
+__host__ __device__ inline uint32_t meta_canon(uint32_t m) { return m >> kTypeBits; }
+__host__ __device__ inline uint32_t meta_type(uint32_t m) { return m & ((1u << kTypeBits) - 1u); }
+
+// The pair predicate d * d > cutoff2 (an FP32 product, d >= 0) as d > t: fl(d * d) never
+// decreases as d grows, so {d : fl(d * d) > cutoff2} = {d > t} with t the largest float
+// whose square rounds to at most cutoff2 (+inf when cutoff2 <= 0: no cutoff). Host code.
+inline float cutoff_threshold(float cutoff2) {
+	if (!(cutoff2 > 0.0f) || !(cutoff2 < 3.0e38f))
+		return __builtin_inff();
+	volatile float t = __builtin_sqrtf(cutoff2);
+	for (;;) {
+		const volatile float sq = t * t;
+		if (!(sq > cutoff2))
+			break;
+		t = __builtin_nextafterf(t, 0.0f);
+	}
+	for (;;) {
+		const volatile float up = __builtin_nextafterf(t, __builtin_inff());
+		const volatile float sq = up * up;
+		if (!(sq <= cutoff2))
+			break;
+		t = up;
+	}
+	return t;
+}
 
 // Orthorhombic box as kernel argument; inv_size is 1/L computed once on the host
 // (the same correctly rounded value PeriodicBox::wrap_diff_scalar forms each call).

@@ -65,7 +65,7 @@ namespace {
 // workgroup; written in each snapshot's particle order, or its sorted order.
 __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const FmSnapView* __restrict__ snaps, uint64_t n,
 														uint32_t tiles, uint32_t items, NbTables T,
-														const float* __restrict__ knots_g, DevBox box, float cutoff2,
+														const float* __restrict__ knots_g, DevBox box, float cut_d,
 														int sorted, float4* __restrict__ out) {
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_forces(const FmSnapView* __rest
 			continue;
 		const FmSnapView V = snaps[k];
 		const NbrList L{V.counts, V.tile_base, V.nbr};
-		const float4 f = nb_force(uint32_t(t), V.spos[t], V.spos, L, T, lds_knots, knots_g, lds_nb, box, cutoff2, true);
+		const float4 f = nb_force(uint32_t(t), V.spos[t], V.spos, L, T, lds_knots, knots_g, lds_nb, box, cut_d, true);
 		out[uint64_t(k) * n + (sorted ? t : V.sorted_idx[t])] = f;
 	}
 }
@@ -458,7 +458,8 @@ void FMImpl::forces(const float* knots, float* out4, hipStream_t s, bool sorted)
 	const uint64_t items = uint64_t(tiles) * S;
 	const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_forces), lds, items);
 	k_fm_forces<<<grid, kFmBlock, lds, s>>>(snap_views.data(), n, tiles, uint32_t(items), *tables, knots,
-											make_devbox(box), cutoff2, sorted ? 1 : 0, reinterpret_cast<float4*>(out4));
+											make_devbox(box), cutoff_threshold(cutoff2), sorted ? 1 : 0,
+											reinterpret_cast<float4*>(out4));
 	MMD_LAUNCH_CHECK();
 }
 

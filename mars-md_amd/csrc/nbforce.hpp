@@ -34,8 +34,10 @@ struct NbLds {
 	float4 coef[MARSMD_MAX_TYPES]; // {charge, softcore radius, softcore eps, 0}
 	// per type pair: {start, step_inv, last, offset} of its enabled table, last = size - 2
 	// (int bits; -2 when the pair has no enabled tabulated term): one read per pair on
-	// the tabulated-only path
-	float4 pairtab[MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
+	// the tabulated-only path. Indexed from one end of the pair, both orientations:
+	// [o << 8 | own type << 4 | other type], o = 1 when the own end is the pair's x
+	// (lower canonical index), so no type selects are needed per pair
+	float4 pairtab[2 * MARSMD_MAX_TYPES * MARSMD_MAX_TYPES];
 };
 
 __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
@@ -50,12 +52,13 @@ __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
 template<bool ANALYTIC>
 __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t tt, const float4& q,
 										const NbTables& T, const float* __restrict__ knots, const NbLds& S,
-										const DevBox& box, float cutoff2, bool energy, float& fx, float& fy,
+										const DevBox& box, float cut_d, bool energy, float& fx, float& fy,
 										float& fz, float& e) {
 	const uint32_t mj = fb(q.w);
-	const uint32_t cj = meta_canon(mj);
 	const uint32_t tj = meta_type(mj);
-	const bool t_is_x = ct < cj;
+	// ct < cj, from the meta words (canonical index in the high bits; a pad entry is the
+	// slot itself: equal words, not lower)
+	const bool t_is_x = fb(p.w) < mj;
 	const float4 px = t_is_x ? p : q;
 	const float4 py = t_is_x ? q : p;
 	const uint32_t tx = t_is_x ? tt : tj;
@@ -66,7 +69,7 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		// skip adds +-0, which leaves the sums' bits unchanged (they start at +0 and can
 		// never become -0). Straight-line code lets the four pairs of a group overlap
 		// their LDS reads.
-		const float4 tp = S.pairtab[(tx << 4) | ty];
+		const float4 tp = S.pairtab[(t_is_x ? 256u : 0u) + ((tt << 4) | tj)];
 		const int last = __float_as_int(tp.z); // table size - 2
 		const bool tab = last >= 0;
 		// py - px is q - p or its negation, exactly (rounding is sign-symmetric)
@@ -80,7 +83,7 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 		const float inv = __builtin_amdgcn_rcpf(fmaxf(d, 1e-6f));
 		// the pairs the reference evaluates (d >= 1e-6, within the cutoff); at d == 1e-6
 		// exactly its unit vector is 0, so the force needs d > 1e-6
-		const bool in_cut = tab && !(cutoff2 > 0.0f && d * d > cutoff2);
+		const bool in_cut = tab && !(d > cut_d);
 		// TabulatedPotential::compute (TabulatedPotential.h:19-52), non-periodic; a pair
 		// without the term reads knots 0 and 1 (last -2 clamps the bin to 0) and adds 0
 		const float* pot = knots + __float_as_int(tp.w);
@@ -123,7 +126,7 @@ __device__ __forceinline__ void nb_pair(const float4& p, uint32_t ct, uint32_t t
 	}
 	if (d < 1e-6f)
 		return;
-	if (cutoff2 > 0.0f && d * d > cutoff2)
+	if (d > cut_d)
 		return;
 	float fm = 0.0f, en = 0.0f;
 	if (ANALYTIC && (terms & kAnalyticTerms)) {
@@ -163,7 +166,7 @@ __device__ __forceinline__ float4 pos_at(const float4* __restrict__ pos, uint32_
 template<bool ANALYTIC = false>
 __device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, const float4* __restrict__ pos,
 											   const NbrList& L, const NbTables& T, const float* __restrict__ knots,
-											   const NbLds& S, const DevBox& box, float cutoff2, bool energy) {
+											   const NbLds& S, const DevBox& box, float cut_d, bool energy) {
 	const uint32_t mt = fb(p.w);
 	const uint32_t ct = meta_canon(mt);
 	const uint32_t tt = meta_type(mt);
@@ -188,10 +191,10 @@ __device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, con
 			if (g + 2 < ng)
 				ja = row[uint64_t(g + 2) * 64u];
 		}
-		nb_pair<ANALYTIC>(p, ct, tt, a0, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, a1, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, a2, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, a3, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, a0, T, knots, S, box, cut_d, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, a1, T, knots, S, box, cut_d, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, a2, T, knots, S, box, cut_d, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, a3, T, knots, S, box, cut_d, energy, fx, fy, fz, e);
 		if (g + 1 >= ng)
 			break;
 		if (g + 2 < ng) {
@@ -202,10 +205,10 @@ __device__ __forceinline__ float4 nb_force_list(uint32_t t, const float4& p, con
 			if (g + 3 < ng)
 				jb = row[uint64_t(g + 3) * 64u];
 		}
-		nb_pair<ANALYTIC>(p, ct, tt, b0, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, b1, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, b2, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
-		nb_pair<ANALYTIC>(p, ct, tt, b3, T, knots, S, box, cutoff2, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, b0, T, knots, S, box, cut_d, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, b1, T, knots, S, box, cut_d, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, b2, T, knots, S, box, cut_d, energy, fx, fy, fz, e);
+		nb_pair<ANALYTIC>(p, ct, tt, b3, T, knots, S, box, cut_d, energy, fx, fy, fz, e);
 		if (g + 2 >= ng)
 			break;
 	}
@@ -218,13 +221,15 @@ __device__ __forceinline__ void stage_params(const NbTables& T, NbLds& S) {
 	const int nt2 = T.num_types * T.num_types;
 	for (int i = threadIdx.x; i < nt2; i += blockDim.x) {
 		const uint32_t tag = T.tag[i];
-		const int k = ((i / T.num_types) << 4) | (i % T.num_types);
-		S.tag[k] = tag;
+		const int x = i / T.num_types, y = i % T.num_types;
+		S.tag[(x << 4) | y] = tag;
 		const int ti = int(tag >> MARSMD_PAIR_TABLE_SHIFT);
-		S.pairtab[k] = (tag & T.enabled & MARSMD_PAIR_TERM_TABULATED) && ti < T.num_tables
-						   ? make_float4(T.start[ti], T.step_inv[ti], __int_as_float(T.size[ti] - 2),
-										 __int_as_float(T.offset[ti]))
-						   : make_float4(0.0f, 0.0f, __int_as_float(-2), __int_as_float(0));
+		const float4 v = (tag & T.enabled & MARSMD_PAIR_TERM_TABULATED) && ti < T.num_tables
+							 ? make_float4(T.start[ti], T.step_inv[ti], __int_as_float(T.size[ti] - 2),
+										   __int_as_float(T.offset[ti]))
+							 : make_float4(0.0f, 0.0f, __int_as_float(-2), __int_as_float(0));
+		S.pairtab[256 + ((x << 4) | y)] = v; // seen from x
+		S.pairtab[(y << 4) | x] = v;         // seen from y
 	}
 	for (int i = threadIdx.x; i < T.num_tables; i += blockDim.x)
 		S.tab[i] = make_float4(T.start[i], T.step_inv[i], __int_as_float(T.size[i]), __int_as_float(T.offset[i]));
@@ -262,10 +267,10 @@ template<bool ANALYTIC = false>
 __device__ __forceinline__ float4 nb_force(uint32_t t, const float4& p, const float4* __restrict__ pos,
 										   const NbrList& L, const NbTables& T, const float* lds_knots,
 										   const float* __restrict__ knots_g, const NbLds& S, const DevBox& box,
-										   float cutoff2, bool energy) {
+										   float cut_d, bool energy) {
 	if (T.lds)
-		return nb_force_list<ANALYTIC>(t, p, pos, L, T, lds_knots, S, box, cutoff2, energy);
-	return nb_force_list<ANALYTIC>(t, p, pos, L, T, knots_g, S, box, cutoff2, energy);
+		return nb_force_list<ANALYTIC>(t, p, pos, L, T, lds_knots, S, box, cut_d, energy);
+	return nb_force_list<ANALYTIC>(t, p, pos, L, T, knots_g, S, box, cut_d, energy);
 }
 
 } // namespace mmd

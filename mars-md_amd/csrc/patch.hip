@@ -3,7 +3,7 @@
 // rebuild / reorder policy on the host, and per-particle kernels.
 //
 // Layout in HBM (slot order = sorted order of the last pairlist build):
-//   pos[2]  float4 {x, y, z, meta}; meta = type << 24 | canonical index. Two
+//   pos[2]  float4 {x, y, z, meta}; meta = canonical index << 8 | type. Two
 //           buffers so the fused step reads step-n positions while writing n+1.
 //   mom     float4 {px, py, pz, 0}            (Langevin)
 //   force   float4 {fx, fy, fz, energy}       (ForceEnergy; bonded scatter target)
@@ -42,14 +42,14 @@ constexpr int kBlock = 256;
 template<bool ANALYTIC>
 __global__ __launch_bounds__(kBlock) void k_nb_forces(const float4* __restrict__ pos, uint64_t n, NbrList L,
 													  NbTables T, const float* __restrict__ knots_g, DevBox box,
-													  float cutoff2, int energy, float4* __restrict__ force) {
+													  float cut_d, int energy, float4* __restrict__ force) {
 	extern __shared__ float lds_knots[];
 	__shared__ NbLds lds_nb;
 	stage_tables(T, knots_g, lds_knots, lds_nb);
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (t >= n)
 		return;
-	force[t] = nb_force<ANALYTIC>(uint32_t(t), pos[t], pos, L, T, lds_knots, knots_g, lds_nb, box, cutoff2, energy != 0);
+	force[t] = nb_force<ANALYTIC>(uint32_t(t), pos[t], pos, L, T, lds_knots, knots_g, lds_nb, box, cut_d, energy != 0);
 }
 
 __global__ void k_zero_force(float4* __restrict__ f, uint64_t n) {
@@ -309,15 +309,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 		pruned = __uint_as_float(PR.disp[(PR.since - 1u) & 3u]) < PR.limit2;
 	const NbrList LL = pruned ? PR.list : L;
 	float disp2 = 0.0f;
-	const TileRange R = xcd_tiles(uint32_t((n + kBlock - 1) / kBlock));
-	for (uint32_t tile = R.tile; tile < R.end; tile += R.stride) {
-		const uint32_t lt = tile * uint32_t(kBlock) + threadIdx.x; // lane position
-		if (lt >= n)
-			continue;
-		const uint32_t t = PR.lane_slot ? PR.lane_slot[lt] : lt; // slot
+	// the rest of slot t's step once its nonbonded force f is summed (kernel arguments
+	// captured by value: by reference the register allocation around the grid sampler
+	// spills)
+	const auto finish = [&, pos, pos_next, mom, force, grids, S, has_bonded, store_force, pmf_in, PR](uint32_t t,
+																										 float4 f) {
 		const float4 p = pos[t];
-		float4 f = nb_force<ANALYTIC>(pruned ? lt : t, p, pos, LL, T, lds_knots, knots_g, lds_nb, S.box, S.cutoff2,
-									  ENERGY);
 		if (has_bonded) {
 			const float4 b = force[t];
 			f.x += b.x;
@@ -350,6 +347,15 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(const float4* __restrict__ p
 			force[t] = f;
 		if (track)
 			disp2 = fmaxf(disp2, len2(wrap_diff_uniform(S.box, sub3(pn, PR.ref[t]))));
+	};
+	const TileRange R = xcd_tiles(uint32_t((n + kBlock - 1) / kBlock));
+	for (uint32_t tile = R.tile; tile < R.end; tile += R.stride) {
+		const uint32_t lt = tile * uint32_t(kBlock) + threadIdx.x; // lane position
+		if (lt >= n)
+			continue;
+		const uint32_t t = PR.lane_slot ? PR.lane_slot[lt] : lt; // slot
+		finish(t, nb_force<ANALYTIC>(pruned ? lt : t, pos[t], pos, LL, T, lds_knots, knots_g, lds_nb, S.box, S.cut_d,
+									 ENERGY));
 	}
 	if (track) {
 		// the workgroup's largest squared displacement (non-negative floats order as
@@ -969,10 +975,10 @@ void PatchImpl::calculate_nonbonded(float pairlist_cutoff, float cutoff, uint64_
 		efield[a] = ef[a];
 	interpolation = interp;
 	rebuild_policy(pairlist_cutoff, cutoff, step, period, s);
-	const float cutoff2 = cutoff > 0.0f ? cutoff * cutoff : 0.0f;
+	const float cut_d = cutoff_threshold(cutoff > 0.0f ? cutoff * cutoff : 0.0f);
 	auto kern = (tables->nb.enabled & kAnalyticTerms) ? k_nb_forces<true> : k_nb_forces<false>;
 	kern<<<grid_for(n, kBlock), kBlock, lds_bytes(), s>>>(pos[cur].data(), n, pl->list(), tables->nb,
-														  tables->nb_knots.data(), make_devbox(box), cutoff2,
+														  tables->nb_knots.data(), make_devbox(box), cut_d,
 														  energy ? 1 : 0, force.data());
 	MMD_LAUNCH_CHECK();
 }
@@ -1196,7 +1202,7 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 	if (n == 0)
 		return;
 	const int64_t rp = P.reorder_period > 0 ? P.reorder_period : int64_t(P.rebuild_period);
-	const float cutoff2 = P.cutoff > 0.0f ? P.cutoff * P.cutoff : 0.0f;
+	const float cut_d = cutoff_threshold(P.cutoff > 0.0f ? P.cutoff * P.cutoff : 0.0f);
 	const bool has_bonded = num_bonded_terms() > 0;
 	const unsigned g = grid_for(n, kBlock);
 	for (uint64_t step = first_step + 1; step <= first_step + num_steps; ++step) {
@@ -1213,7 +1219,7 @@ void PatchImpl::run(const marsmd_run_params& P, uint64_t first_step, uint64_t nu
 			calculate_bonded(energy, s, false); // written, not added: the step adds the rest
 		StepParams S{};
 		S.box = make_devbox(box);
-		S.cutoff2 = cutoff2;
+		S.cut_d = cut_d;
 		S.dt = P.timestep;
 		S.kT = P.kT;
 		S.seed = mix_seed(base_seed, patch_id, step);

@@ -8,7 +8,7 @@ namespace mmd {
 
 struct StepParams {
 	DevBox box;
-	float cutoff2;
+	float cut_d; // pairs farther than this are beyond the cutoff (cutoff_threshold)
 	float dt;
 	float kT;
 	uint64_t seed;
