@@ -258,9 +258,11 @@ __global__ __launch_bounds__(256) void k_b_max_disp(const float4* __restrict__ c
 }
 
 // The fused step over every replica's particles, persistent over XCD-local tiles: the
-// replica of a slot gives its box and its seed mix.
+// replica of a slot gives its box and its seed mix. Bounded to 96 VGPRs (5 waves per
+// SIMD; no spills in the BD case, 2 dwords in the others): 160 -> 156.5 us at 512
+// replicas against 4 waves with 123 VGPRs.
 template<int INTEG, bool ENERGY, bool ANALYTIC>
-__global__ __launch_bounds__(kBatchBlock, 4) void k_b_step(const float4* __restrict__ pos, float4* __restrict__ pos_next,
+__global__ __launch_bounds__(kBatchBlock, 5) void k_b_step(const float4* __restrict__ pos, float4* __restrict__ pos_next,
 														  float4* __restrict__ mom, float4* __restrict__ force,
 														  uint64_t total, const uint32_t* __restrict__ rep_of_slot,
 														  const RepGeom* __restrict__ geom, NbrList L, NbTables T,
