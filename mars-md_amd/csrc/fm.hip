@@ -53,9 +53,15 @@ struct FmSnapView {
 	const uint32_t* nbr;
 	// adjoint records: record k of slot t at nbr_slot_index(tile_base[t / 64], t, k)
 	const uint32_t* rec_count;
-	const uint32_t* rec_y;  // slot of the pair's other end
-	const int32_t* rec_k;   // knot index of the pair's bin (table offset + bin)
-	const float4* rec_u;    // {u, step_inv}
+	const uint32_t* rec_y; // slot of the pair's other end
+	// {u, code}: code (int bits) = table << kFmKnotBits | knot index of the pair's bin
+	// (table offset + bin); the adjoint reads the table's step_inv from LDS
+	const float4* rec_u;
+};
+constexpr int kFmKnotBits = 26;
+// step_inv of every table, for the adjoint (a kernel argument staged in LDS)
+struct FmSteps {
+	float step_inv[kMaxTables];
 };
 
 namespace {
@@ -96,7 +102,7 @@ __device__ __forceinline__ bool fm_owns(uint32_t a, uint32_t b) { return ((a ^ b
 __global__ __launch_bounds__(kFmBlock) void k_fm_records(const float4* __restrict__ spos, uint64_t n,
 														 NbrList L, NbTables T, DevBox box, float cutoff2,
 														 uint32_t* __restrict__ rec_count, uint32_t* __restrict__ rec_y,
-														 int32_t* __restrict__ rec_k, float4* __restrict__ rec_u) {
+														 float4* __restrict__ rec_u) {
 	__shared__ NbLds lds_nb;
 	stage_params(T, lds_nb);
 	__syncthreads();
@@ -140,8 +146,8 @@ __global__ __launch_bounds__(kFmBlock) void k_fm_records(const float4* __restric
 			h = 0;
 		const uint64_t at = nbr_slot_index(base, uint32_t(t), m);
 		rec_y[at] = L.nbr[nbr_slot_index(base, uint32_t(t), k)];
-		rec_k[at] = __float_as_int(tp.w) + h;
-		rec_u[at] = make_float4(inv * r.x, inv * r.y, inv * r.z, tp.y);
+		const int code = int((tag >> MARSMD_PAIR_TABLE_SHIFT) << kFmKnotBits) | (__float_as_int(tp.w) + h);
+		rec_u[at] = make_float4(inv * r.x, inv * r.y, inv * r.z, __int_as_float(code));
 		++m;
 	}
 	rec_count[t] = m;
@@ -168,8 +174,8 @@ __global__ void k_fm_to_sorted(const FmSnapView* __restrict__ snaps, uint64_t n,
 
 // dL/dknot of every snapshot in one persistent launch (work items as in k_fm_forces),
 // from the records and dL/dF in slot order (grad_all): slot t adds, for each record
-// (o, k, v),
-// g = ((G_o - G_t) . v) step_inv to S[k], in FP64; records are read four at a time.
+// (o, k, v) of table j,
+// g = ((G_o - G_t) . v) step_inv_j to S[k], in FP64; records are read four at a time.
 // Each pair adds g to knot k and -g to knot k + 1, so dL/dU[k] = S[k] - S[k - 1] (a
 // table's last knot never starts a record's bin, so its S is 0 and the difference
 // never reaches across tables); the reduction forms it. Per-workgroup FP64 sums in LDS
@@ -180,13 +186,16 @@ __global__ void k_fm_to_sorted(const FmSnapView* __restrict__ snaps, uint64_t n,
 template<bool LDS>
 __global__ __launch_bounds__(kFmAdjBlock) void k_fm_adjoint_rec(const FmSnapView* __restrict__ snaps, uint64_t n,
 																uint32_t tiles, uint32_t items, uint32_t total_knots,
-																const float4* __restrict__ grad_all,
+																const float4* __restrict__ grad_all, FmSteps steps,
 																double* __restrict__ partial) {
 	extern __shared__ double lds_grad[];
+	__shared__ float lds_step[kMaxTables];
 	double* acc = LDS ? lds_grad : partial;
 	if (LDS)
 		for (uint32_t i = threadIdx.x; i < total_knots; i += blockDim.x)
 			lds_grad[i] = 0.0;
+	if (threadIdx.x < uint32_t(kMaxTables))
+		lds_step[threadIdx.x] = steps.step_inv[threadIdx.x];
 	__syncthreads();
 	const TileRange R = xcd_tiles(items);
 	for (uint32_t w = R.tile; w < R.end; w += R.stride) {
@@ -204,7 +213,6 @@ __global__ __launch_bounds__(kFmAdjBlock) void k_fm_adjoint_rec(const FmSnapView
 		// group g's records (4 at a time) and the gradients at their other ends; the
 		// next group's records are in flight while one group is evaluated
 		uint4 ys = *reinterpret_cast<const uint4*>(V.rec_y + row);
-		int4 ks = *reinterpret_cast<const int4*>(V.rec_k + row);
 		float4 uv[4], gy[4];
 #pragma unroll
 		for (int e = 0; e < 4; ++e)
@@ -219,32 +227,29 @@ __global__ __launch_bounds__(kFmAdjBlock) void k_fm_adjoint_rec(const FmSnapView
 			const bool more = g0 + 4 < cnt;
 			const uint64_t nat = row + uint64_t((g0 >> 2) + 1) * 256u;
 			uint4 ysn = ys;
-			int4 ksn = ks;
 			float4 un[4] = {uv[0], uv[1], uv[2], uv[3]};
 			if (more) {
 				ysn = *reinterpret_cast<const uint4*>(V.rec_y + nat);
-				ksn = *reinterpret_cast<const int4*>(V.rec_k + nat);
 #pragma unroll
 				for (int e = 0; e < 4; ++e)
 					un[e] = V.rec_u[nat + uint64_t(e)];
 			}
-			const int kv[4] = {ks.x, ks.y, ks.z, ks.w};
 #pragma unroll
 			for (int e = 0; e < 4; ++e) {
 				if (g0 + uint32_t(e) >= cnt)
 					break;
 				const float4 u = uv[e];
+				const int code = __float_as_int(u.w);
 				const double dfm = (double(gy[e].x) - double(gx.x)) * double(u.x) +
 								   (double(gy[e].y) - double(gx.y)) * double(u.y) +
 								   (double(gy[e].z) - double(gx.z)) * double(u.z);
 				// S[k] += g only: the pair's -g on knot k + 1 is applied once at the end,
 				// as dL/dU[k] = S[k] - S[k - 1] (half the atomic additions)
-				atomicAdd(&acc[kv[e]], dfm * double(u.w));
+				atomicAdd(&acc[code & ((1 << kFmKnotBits) - 1)], dfm * double(lds_step[code >> kFmKnotBits]));
 			}
 			if (!more)
 				break;
 			ys = ysn;
-			ks = ksn;
 			const uint32_t yv[4] = {ys.x, ys.y, ys.z, ys.w};
 #pragma unroll
 			for (int e = 0; e < 4; ++e) {
@@ -348,7 +353,6 @@ struct FMImpl::Snap {
 	DevArray<float4> spos;
 	DevArray<uint32_t> sorted_idx, counts, tile_base, nbr;
 	DevArray<uint32_t> rec_count, rec_y;
-	DevArray<int32_t> rec_k;
 	DevArray<float4> rec_u;
 };
 
@@ -391,6 +395,8 @@ FMImpl::FMImpl(const marsmd_fm_desc& d) : tables(new NbTables{}) {
 			raise(MARSMD_ERR_VALUE, "force matching: a pair tag names a missing table");
 	T.total_knots = total;
 	T.lds = total <= kLdsKnots ? 1 : 0;
+	if (total >= (1 << kFmKnotBits))
+		raise(MARSMD_ERR_VALUE, "force matching: at most 2^26 - 1 knots in all");
 	PairlistImpl pl(n, ~uint64_t(0));
 	pl.set_box(box);
 	const float plc = d.pairlist_cutoff > 0.0f ? d.pairlist_cutoff : d.cutoff;
@@ -416,19 +422,18 @@ FMImpl::FMImpl(const marsmd_fm_desc& d) : tables(new NbTables{}) {
 		// the adjoint's records, laid out like the rows (at most as many as row entries)
 		sn->rec_count.resize(n);
 		sn->rec_y.resize(std::max<uint64_t>(pl.nbr_storage, 4));
-		sn->rec_k.resize(std::max<uint64_t>(pl.nbr_storage, 4));
 		sn->rec_u.resize(std::max<uint64_t>(pl.nbr_storage, 4));
 		const NbrList L{sn->counts.data(), sn->tile_base.data(), sn->nbr.data()};
 		k_fm_records<<<grid_for(n, kFmBlock), kFmBlock, 0, nullptr>>>(sn->spos.data(), n, L, T, make_devbox(box),
 																	  cutoff2, sn->rec_count.data(), sn->rec_y.data(),
-																	  sn->rec_k.data(), sn->rec_u.data());
+																	  sn->rec_u.data());
 		MMD_LAUNCH_CHECK();
 	}
 	std::vector<FmSnapView> views(S);
 	for (uint64_t k = 0; k < S; ++k)
 		views[k] = FmSnapView{snaps[k]->spos.data(),   snaps[k]->sorted_idx.data(), snaps[k]->counts.data(),
 							  snaps[k]->tile_base.data(), snaps[k]->nbr.data(),  snaps[k]->rec_count.data(),
-							  snaps[k]->rec_y.data(),  snaps[k]->rec_k.data(),  snaps[k]->rec_u.data()};
+							  snaps[k]->rec_y.data(),  snaps[k]->rec_u.data()};
 	snap_views.resize(S);
 	MMD_HIP(hipMemcpy(snap_views.data(), views.data(), sizeof(FmSnapView) * S, hipMemcpyHostToDevice));
 	MMD_HIP(hipDeviceSynchronize());
@@ -499,6 +504,8 @@ void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStr
 		to_sorted(grad4, 4, reinterpret_cast<float*>(grad_slot.data()), s);
 		g = grad_slot.data();
 	}
+	FmSteps steps{};
+	std::memcpy(steps.step_inv, tables->step_inv, sizeof steps.step_inv);
 	uint32_t rows = 1;
 	if (lds) {
 		const unsigned grid =
@@ -506,7 +513,7 @@ void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStr
 		rows = grid;
 		partial.resize(uint64_t(rows) * total);
 		k_fm_adjoint_rec<true><<<grid, kFmAdjBlock, total * 8u, s>>>(snap_views.data(), n, tiles, uint32_t(items),
-																	total, g, partial.data());
+																	total, g, steps, partial.data());
 	} else {
 		const unsigned grid =
 			fm_grid(reinterpret_cast<const void*>(k_fm_adjoint_rec<false>), 0u, items, kFmAdjBlock);
@@ -514,7 +521,7 @@ void FMImpl::adjoint(const float* knots, const float* grad4, double* out, hipStr
 		k_fill_f64<<<grid_for(total, 256), 256, 0, s>>>(partial.data(), total);
 		MMD_LAUNCH_CHECK();
 		k_fm_adjoint_rec<false><<<grid, kFmAdjBlock, 0, s>>>(snap_views.data(), n, tiles, uint32_t(items), total,
-															 g, partial.data());
+															 g, steps, partial.data());
 	}
 	MMD_LAUNCH_CHECK();
 	seg_sums.resize(uint64_t(kFmSegs) * total);
