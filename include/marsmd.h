@@ -694,6 +694,12 @@ marsmd_status marsmd_fm_forces_sorted(marsmd_fm* fm, const float* knots, float* 
 marsmd_status marsmd_fm_adjoint_sorted(marsmd_fm* fm, const float* knots, const float* grad_force4,
                                        double* grad_knots, marsmd_stream stream);
 marsmd_status marsmd_fm_to_sorted(marsmd_fm* fm, const float* in, int32_t width, float* out, marsmd_stream stream);
+/* marsmd_fm_forces_sorted followed by marsmd_fm_mse in one pass, the forces never
+ * stored: target3 and grad_force4 in each snapshot's sorted order; grad_force4 has
+ * marsmd_fm_mse's bits, the loss the same FP64 terms summed in another fixed grouping.
+ * No counterpart in the reference (it has no gradient path). */
+marsmd_status marsmd_fm_loss_sorted(marsmd_fm* fm, const float* knots, const float* target3, float* grad_force4,
+                                    double* loss, marsmd_stream stream);
 
 #ifdef __cplusplus
 } /* extern "C" */

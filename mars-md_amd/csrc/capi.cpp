@@ -800,6 +800,17 @@ marsmd_status marsmd_fm_adjoint(marsmd_fm* fm, const float* knots, const float* 
 		fm->impl.adjoint(knots, grad_force4, grad_knots, st(stream));
 	});
 }
+marsmd_status marsmd_fm_loss_sorted(marsmd_fm* fm, const float* knots, const float* target3, float* grad_force4,
+									double* loss, marsmd_stream stream) {
+	return guard([&] {
+		need(fm, "force matching");
+		need(knots, "knots");
+		need(target3, "target");
+		need(grad_force4, "grad_force4");
+		need(loss, "loss");
+		fm->impl.loss_sorted(knots, target3, grad_force4, loss, st(stream));
+	});
+}
 marsmd_status marsmd_fm_forces_sorted(marsmd_fm* fm, const float* knots, float* force_energy4,
 									  marsmd_stream stream) {
 	return guard([&] {

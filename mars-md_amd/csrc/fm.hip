@@ -341,6 +341,47 @@ __global__ __launch_bounds__(kMseBlock) void k_fm_mse_total(const double* __rest
 		*loss = part[0] * inv_count;
 }
 
+// The loss of every snapshot in sorted order in one persistent launch (work items as
+// in k_fm_forces): slot t's force F goes straight into k_fm_mse's terms, d = F - target
+// (FP32), grad = d * scale and the FP64 d . d summed per workgroup in a fixed tree, so
+// grad has k_fm_mse's bits and F itself is never stored; k_fm_mse_total then forms
+// the loss from the block sums (in another grouping than the two-pass path's).
+__global__ __launch_bounds__(kFmBlock) void k_fm_loss(const FmSnapView* __restrict__ snaps, uint64_t n,
+													  uint32_t tiles, uint32_t items, NbTables T,
+													  const float* __restrict__ knots_g, DevBox box, float cut_d,
+													  const float* __restrict__ target3, float scale,
+													  float4* __restrict__ grad, double* __restrict__ block_sums) {
+	static_assert(kFmBlock == kMseBlock, "one tree over the workgroup");
+	extern __shared__ float lds_knots[];
+	__shared__ NbLds lds_nb;
+	__shared__ double part[kFmBlock];
+	stage_tables(T, knots_g, lds_knots, lds_nb);
+	double acc = 0.0;
+	const TileRange R = xcd_tiles(items);
+	for (uint32_t w = R.tile; w < R.end; w += R.stride) {
+		const uint32_t k = w / tiles;
+		const uint64_t t = uint64_t(w - k * tiles) * kFmBlock + threadIdx.x;
+		if (t >= n)
+			continue;
+		const FmSnapView V = snaps[k];
+		const NbrList L{V.counts, V.tile_base, V.nbr};
+		const float4 f = nb_force(uint32_t(t), V.spos[t], V.spos, L, T, lds_knots, knots_g, lds_nb, box, cut_d, false);
+		const uint64_t i = uint64_t(k) * n + t;
+		const float dx = f.x - target3[3 * i], dy = f.y - target3[3 * i + 1], dz = f.z - target3[3 * i + 2];
+		grad[i] = make_float4(dx * scale, dy * scale, dz * scale, 0.0f);
+		acc += double(dx) * double(dx) + double(dy) * double(dy) + double(dz) * double(dz);
+	}
+	part[threadIdx.x] = acc;
+	__syncthreads();
+	for (int o = kFmBlock / 2; o > 0; o >>= 1) {
+		if (int(threadIdx.x) < o)
+			part[threadIdx.x] += part[threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		block_sums[blockIdx.x] = part[0];
+}
+
 __global__ void k_fill_f64(double* p, uint64_t n) {
 	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i < n)
@@ -478,6 +519,22 @@ void FMImpl::mse(const float* force4, const float* target3, float* grad4, double
 										  reinterpret_cast<float4*>(grad4), mse_sums.data());
 	MMD_LAUNCH_CHECK();
 	k_fm_mse_total<<<1, kMseBlock, 0, s>>>(mse_sums.data(), blocks, 1.0 / (3.0 * double(count)), loss);
+	MMD_LAUNCH_CHECK();
+}
+
+void FMImpl::loss_sorted(const float* knots, const float* target3, float* grad4, double* loss, hipStream_t s) {
+	const unsigned lds = tables->lds ? unsigned(std::max(tables->total_knots, 2)) * 4u : 0u;
+	const uint32_t tiles = uint32_t((n + kFmBlock - 1) / kFmBlock);
+	const uint64_t items = uint64_t(tiles) * S;
+	const uint64_t count = S * n;
+	const unsigned grid = fm_grid(reinterpret_cast<const void*>(k_fm_loss), lds, items);
+	mse_sums.resize(grid);
+	const float scale = float(2.0 / (3.0 * double(count))); // as FMImpl::mse
+	k_fm_loss<<<grid, kFmBlock, lds, s>>>(snap_views.data(), n, tiles, uint32_t(items), *tables, knots,
+										   make_devbox(box), cutoff_threshold(cutoff2), target3, scale,
+										   reinterpret_cast<float4*>(grad4), mse_sums.data());
+	MMD_LAUNCH_CHECK();
+	k_fm_mse_total<<<1, kMseBlock, 0, s>>>(mse_sums.data(), grid, 1.0 / (3.0 * double(count)), loss);
 	MMD_LAUNCH_CHECK();
 }
 

@@ -36,6 +36,10 @@ struct FMImpl {
 	// against target3 (S x n x 3 FP32): *loss (device FP64) = sum |F - target|^2 / (3 S n),
 	// and grad4 (S x n float4) = (F - target) 2 / (3 S n), the loss's dL/dF.
 	void mse(const float* force4, const float* target3, float* grad4, double* loss, hipStream_t s);
+	// The same loss and dL/dF of the forces for the given knots, each snapshot in its
+	// sorted order (target3 too) and without storing the forces: grad4 has mse()'s bits;
+	// the loss sums the same FP64 terms in another fixed grouping.
+	void loss_sorted(const float* knots, const float* target3, float* grad4, double* loss, hipStream_t s);
 	// dL/dknot (device FP64) for dL/dF = grad4 (S x n float4, .w ignored; in sorted
 	// order when sorted, else particle order).
 	void adjoint(const float* knots, const float* grad4, double* out, hipStream_t s, bool sorted = false);

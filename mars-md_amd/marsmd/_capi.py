@@ -333,6 +333,7 @@ PROTOTYPES = {
     "marsmd_fm_mse": (_I32, [_P, _P, _P, _P, _P, _P]),
     "marsmd_fm_adjoint": (_I32, [_P, _P, _P, _P, _P]),
     "marsmd_fm_forces_sorted": (_I32, [_P, _P, _P, _P]),
+    "marsmd_fm_loss_sorted": (_I32, [_P, _P, _P, _P, _P, _P]),
     "marsmd_fm_adjoint_sorted": (_I32, [_P, _P, _P, _P, _P]),
     "marsmd_fm_to_sorted": (_I32, [_P, _P, _I32, _P, _P]),
 }

@@ -130,6 +130,22 @@ class ForceMatching:
                  _stream(stream))
         return loss, grad
 
+    def loss_sorted(self, knots, target_sorted, stream=None):
+        """(loss, dL/dF) of forces(knots, sorted_order=True) against target_sorted
+        (S, n, 3), each snapshot in its sorted order, in one pass that never stores the
+        forces: dL/dF has mse()'s bits; the loss sums the same FP64 terms in another
+        fixed grouping."""
+        torch = _torch()
+        k = self._knots(knots)
+        t = target_sorted.detach().to(device="cuda", dtype=torch.float32).contiguous()
+        if t.shape != (self.S, self.n, 3):
+            raise ValueError(f"target must be ({self.S}, {self.n}, 3)")
+        grad = torch.empty((self.S, self.n, 4), dtype=torch.float32, device="cuda")
+        loss = torch.empty((), dtype=torch.float64, device="cuda")
+        api.call("marsmd_fm_loss_sorted", self._h, k.data_ptr(), t.data_ptr(), grad.data_ptr(), loss.data_ptr(),
+                 _stream(stream))
+        return loss, grad
+
     def adjoint(self, knots, grad_forces, stream=None, sorted_order: bool = False):
         """dL/dknot (float64, total knots) for dL/dF = grad_forces (S, n, 3 or 4), in
         particle order (or sorted order)."""
@@ -208,16 +224,15 @@ def _loss_function():
     torch = _torch()
 
     class ForceMatchingLoss(torch.autograd.Function):
-        """mean((F(knots) - target)^2) as two library calls (forces, then the loss and
-        its dL/dF in one pass); backward is the FP64 adjoint of that dL/dF. Everything
+        """mean((F(knots) - target)^2) as one library call (the forces go straight into
+        the loss and its dL/dF); backward is the FP64 adjoint of that dL/dF. Everything
         runs in each snapshot's sorted order (the target permuted once and kept while
         the same target tensor is passed): the loss is a sum over all rows, so its
         value does not depend on their order."""
 
         @staticmethod
         def forward(ctx, knots, fm: ForceMatching, target):
-            f = fm.forces(knots, sorted_order=True)
-            loss, grad_f = fm.mse(f, fm._sorted_target(target))
+            loss, grad_f = fm.loss_sorted(knots, fm._sorted_target(target))
             ctx.fm = fm
             ctx.save_for_backward(knots, grad_f)
             return loss

@@ -129,3 +129,20 @@ def test_sorted_order_entry_points(cuda):
     l3 = force_matching_loss(knots, fm, other).item()
     ref3 = ((f[..., :3].double() - other.double()) ** 2).sum().item() / other.numel()
     assert abs(l3 - ref3) <= 1e-6 * ref3
+
+
+def test_one_pass_loss_is_forces_then_mse(cuda):
+    # marsmd_fm_loss_sorted never stores the forces: its dL/dF must be the two-pass
+    # path's (forces in sorted order, then the loss kernel) bit for bit, and its loss
+    # the same FP64 terms summed in another fixed grouping
+    torch = cuda
+    w, fm = _setup(n=2200, L=265.0, S=3, knots=256, seed=13)
+    knots = fm.knots()
+    target = fm.to_sorted(fm.forces(fm.knots(w.target_tables))[..., :3].contiguous())
+    loss2, grad2 = fm.mse(fm.forces(knots, sorted_order=True), target)
+    loss1, grad1 = fm.loss_sorted(knots, target)
+    assert torch.equal(grad1, grad2)
+    assert abs(loss1.item() - loss2.item()) <= 1e-13 * loss2.item()
+    # repeated calls give the same bits
+    loss3, grad3 = fm.loss_sorted(knots, target)
+    assert torch.equal(grad3, grad1) and loss3.item() == loss1.item()
