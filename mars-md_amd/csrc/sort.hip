@@ -50,28 +50,35 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
 	return valid ? peers : 0;
 }
 
-// Digit d of a key is (key >> shift) & mask, mask < 2^BITS.
+// Digit d of a key is (key >> shift) & mask, mask < 2^BITS. A tile's counts do not
+// depend on which wave reads which key, so the histogram spreads a tile over 16 waves
+// (4 rows of keys each): one workgroup per tile leaves the GPU mostly idle, and more
+// waves per tile hide the loads' latency.
+constexpr int kRsHistBlock = 1024;
 template<int BITS>
-__global__ __launch_bounds__(kRsBlock) void k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n,
-													  int shift, uint32_t mask,
-													  uint32_t* __restrict__ tile_hist,
-													  uint32_t num_tiles, uint32_t* __restrict__ totals) {
+__global__ __launch_bounds__(kRsHistBlock) void k_rs_hist(const uint32_t* __restrict__ keys, uint64_t n,
+														  int shift, uint32_t mask,
+														  uint32_t* __restrict__ tile_hist,
+														  uint32_t num_tiles, uint32_t* __restrict__ totals) {
 	constexpr uint32_t B = 1u << BITS;
+	constexpr int kWaves = kRsHistBlock / 64;
+	constexpr int kRows = kRsTile / kRsHistBlock;
+	static_assert(kRows * kRsHistBlock == kRsTile, "whole rows of keys per wave");
 	__shared__ uint32_t h[B];
-	for (uint32_t i = threadIdx.x; i < B; i += kRsBlock)
+	for (uint32_t i = threadIdx.x; i < B; i += kRsHistBlock)
 		h[i] = 0;
 	const int lane = threadIdx.x & 63;
 	const int w = threadIdx.x >> 6;
-	const uint64_t wbase = uint64_t(blockIdx.x) * kRsTile + uint64_t(w) * (kRsTile / 4);
-	uint32_t mk[kRsRows];
+	const uint64_t wbase = uint64_t(blockIdx.x) * kRsTile + uint64_t(w) * (kRsTile / kWaves);
+	uint32_t mk[kRows];
 #pragma unroll
-	for (int r = 0; r < kRsRows; ++r) {
+	for (int r = 0; r < kRows; ++r) {
 		const uint64_t i = wbase + uint64_t(r) * 64 + lane;
 		mk[r] = i < n ? keys[i] : 0u;
 	}
 	__syncthreads();
 #pragma unroll
-	for (int r = 0; r < kRsRows; ++r) {
+	for (int r = 0; r < kRows; ++r) {
 		const bool valid = wbase + uint64_t(r) * 64 + lane < n;
 		const uint32_t d = (mk[r] >> shift) & mask;
 		const uint64_t peers = digit_peers<BITS>(d, valid);
@@ -79,7 +86,7 @@ __global__ __launch_bounds__(kRsBlock) void k_rs_hist(const uint32_t* __restrict
 			atomicAdd(&h[d], uint32_t(__popcll(peers)));
 	}
 	__syncthreads();
-	for (uint32_t i = threadIdx.x; i < B; i += kRsBlock) {
+	for (uint32_t i = threadIdx.x; i < B; i += kRsHistBlock) {
 		const uint32_t c = h[i];
 		tile_hist[uint64_t(i) * num_tiles + blockIdx.x] = c;
 		if (c)
@@ -362,9 +369,9 @@ void radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
 		const int width = key_bits - shift < bits ? key_bits - shift : bits;
 		const uint32_t mask = (1u << width) - 1u;
 		if (bits == 10)
-			k_rs_hist<10><<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles, tot);
+			k_rs_hist<10><<<tiles, kRsHistBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles, tot);
 		else
-			k_rs_hist<8><<<tiles, kRsBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles, tot);
+			k_rs_hist<8><<<tiles, kRsHistBlock, 0, s>>>(src_k, n, shift, mask, hist, tiles, tot);
 		MMD_LAUNCH_CHECK();
 		k_rs_offsets<<<unsigned(buckets), kRsBlock, 0, s>>>(hist, tiles, tot, tot_next);
 		MMD_LAUNCH_CHECK();
