@@ -100,6 +100,119 @@ __global__ void k_b_rep_keys(const uint32_t* __restrict__ vals_s, uint64_t total
 	rvals[i] = old;
 }
 
+// The same order in one launch when every replica has at most kSegSortMax particles:
+// a stable LSD radix sort of each replica's Morton keys (three 10-bit digits) by one
+// workgroup of kSegWaves waves. The encode leaves replica r's entries at [slot0, slot0 + n)
+// in canonical order, so sorting each such range stably by key gives the order of the
+// global stable sort by key followed by the stable sort by replica. Wave w takes the run
+// [w C, w C + C) of the range (C = ceil(n / kSegWaves)) in batches of 64 held in
+// registers, counts its digits into hist[digit][w], and after the block's exclusive scan
+// (digit-major, wave-minor) scatters each batch in order: the entries of one batch with
+// one digit rank by lane (their peer mask from 10 ballots) and the lowest of them moves
+// the wave's offset for that digit on. Equal digits therefore keep wave, batch and lane
+// order. Passes ping-pong through the replica's range of two global buffer pairs (L2
+// resident; the barrier orders one pass's stores before the next pass's loads); the last
+// pass writes only the old slots, the order the gather reads.
+constexpr int kSegWaves = 16;
+constexpr int kSegThreads = kSegWaves * 64;
+constexpr int kSegPerLane = 32;
+constexpr int kSegStride = kSegWaves + 1; // hist row per digit, padded against bank conflicts
+constexpr uint32_t kSegSpare = 1024u * kSegStride; // the spare bin
+constexpr uint32_t kSegSortMax = uint32_t(kSegThreads) * kSegPerLane;
+
+__global__ __launch_bounds__(kSegThreads) void k_b_seg_sort(const RepGeom* __restrict__ geom, uint32_t* keys_a,
+															uint32_t* vals_a, uint32_t* keys_b, uint32_t* vals_b,
+															uint32_t* __restrict__ out_vals) {
+	__shared__ uint32_t hist[1024 * kSegStride + 1];
+	__shared__ uint32_t wsum[kSegWaves];
+	const uint32_t base = geom[blockIdx.x].slot0, n = geom[blockIdx.x].n;
+	const int wv = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
+	const uint32_t C = (n + kSegWaves - 1) / kSegWaves; // <= 64 * kSegPerLane
+	const uint32_t w0_ = min(n, uint32_t(wv) * C), w1_ = min(n, w0_ + C);
+	const uint64_t lt = (uint64_t(1) << lane) - 1;
+	uint32_t key[kSegPerLane], val[kSegPerLane];
+#pragma unroll 1
+	for (int pass = 0; pass < 3; ++pass) {
+		const uint32_t* kin = pass == 1 ? keys_b : keys_a;
+		const uint32_t* vin = pass == 1 ? vals_b : vals_a;
+		uint32_t* kout = pass == 0 ? keys_b : keys_a;
+		uint32_t* vout = pass == 0 ? vals_b : (pass == 1 ? vals_a : out_vals);
+		const int shift = 10 * pass;
+		// the run's bounds, opaque to the compiler each pass: otherwise it hoists every
+		// batch's address and mask out of the pass loop (64 SGPRs and 32 VGPRs)
+		uint32_t w0 = w0_, w1 = w1_;
+		asm volatile("" : "+v"(w0), "+v"(w1));
+		const uint32_t nb = (w1 - w0 + 63u) / 64u; // batches of this wave
+		for (uint32_t i = threadIdx.x; i <= kSegSpare; i += kSegThreads)
+			hist[i] = 0u;
+		__syncthreads();
+		if (w1 > w0) { // loads clamped into the run, a lane past its end counted in a spare bin
+#pragma unroll
+			for (int j = 0; j < kSegPerLane; ++j) {
+				if (uint32_t(j) < nb) { // wave-uniform
+					const uint32_t e = min(w0 + uint32_t(j) * 64u + uint32_t(lane), w1 - 1u);
+					key[j] = kin[base + e];
+					val[j] = vin[base + e];
+				}
+			}
+#pragma unroll
+			for (int j = 0; j < kSegPerLane; ++j) {
+				if (uint32_t(j) < nb) {
+					const bool live = w0 + uint32_t(j) * 64u + uint32_t(lane) < w1;
+					atomicAdd(&hist[live ? ((key[j] >> shift) & 1023u) * kSegStride + uint32_t(wv) : kSegSpare], 1u);
+				}
+			}
+		}
+		__syncthreads();
+		{ // thread t owns digit t: its waves' counts become their first destinations
+			const uint32_t row = threadIdx.x * kSegStride;
+			uint32_t s = 0;
+#pragma unroll
+			for (int w = 0; w < kSegWaves; ++w)
+				s += hist[row + uint32_t(w)];
+			const uint32_t incl = wave_inclusive_scan(s);
+			if (lane == 63)
+				wsum[wv] = incl;
+			__syncthreads();
+			uint32_t run = incl - s;
+			for (int w = 0; w < wv; ++w)
+				run += wsum[w];
+#pragma unroll
+			for (int w = 0; w < kSegWaves; ++w) {
+				const uint32_t c = hist[row + uint32_t(w)];
+				hist[row + uint32_t(w)] = run;
+				run += c;
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < kSegPerLane; ++j) {
+			if (uint32_t(j) >= nb) // wave-uniform
+				break;
+			const bool live = w0 + uint32_t(j) * 64u + uint32_t(lane) < w1;
+			const uint32_t d = (key[j] >> shift) & 1023u;
+			uint64_t peers = __ballot(live);
+#pragma unroll
+			for (int b = 0; b < 10; ++b) {
+				const uint64_t ones = __ballot(live && ((d >> b) & 1u));
+				peers &= ((d >> b) & 1u) ? ones : ~ones;
+			}
+			if (live) {
+				uint32_t* h = &hist[d * kSegStride + uint32_t(wv)];
+				const uint32_t first = *h;
+				const uint32_t rank = uint32_t(__popcll(peers & lt));
+				if (rank == 0u)
+					*h = first + uint32_t(__popcll(peers));
+				const uint32_t dst = base + first + rank;
+				if (pass < 2)
+					kout[dst] = key[j];
+				vout[dst] = val[j];
+			}
+		}
+		__syncthreads();
+	}
+}
+
 // New slot i holds the particle of current slot old_slot[i], its Morton code computed
 // again from the same position with the same operations as k_b_encode.
 __global__ void k_b_gather(const uint32_t* __restrict__ old_slot, uint64_t total,
@@ -779,18 +892,24 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 	rkeys_s.resize(total);
 	rvals_s.resize(total);
 	codes.resize(total);
-	sort_scratch.resize(radix_sort_scratch_bytes(total));
 	fill_u32(scalars.data(), 0u, 8, s);
 	k_b_encode<<<grid_for(total, 256), 256, 0, s>>>(pos[cur].data(), total, rep_of_slot.data(), gd, keys.data(),
 													vals.data(), scalars.data());
 	MMD_LAUNCH_CHECK();
-	radix_sort_pairs(keys.data(), vals.data(), keys_s.data(), vals_s.data(), total, 30, sort_scratch.data(),
-					 sort_scratch.count, s);
-	k_b_rep_keys<<<grid_for(total, 256), 256, 0, s>>>(vals_s.data(), total, rep_of_slot.data(), rkeys.data(),
-													  rvals.data());
-	MMD_LAUNCH_CHECK();
-	radix_sort_pairs(rkeys.data(), rvals.data(), rkeys_s.data(), rvals_s.data(), total, rep_bits, sort_scratch.data(),
-					 sort_scratch.count, s);
+	if (*std::max_element(count.begin(), count.end()) <= kSegSortMax) {
+		k_b_seg_sort<<<unsigned(R), kSegThreads, 0, s>>>(gd, keys.data(), vals.data(), keys_s.data(), vals_s.data(),
+														 rvals_s.data());
+		MMD_LAUNCH_CHECK();
+	} else {
+		sort_scratch.resize(radix_sort_scratch_bytes(total));
+		radix_sort_pairs(keys.data(), vals.data(), keys_s.data(), vals_s.data(), total, 30, sort_scratch.data(),
+						 sort_scratch.count, s);
+		k_b_rep_keys<<<grid_for(total, 256), 256, 0, s>>>(vals_s.data(), total, rep_of_slot.data(), rkeys.data(),
+														  rvals.data());
+		MMD_LAUNCH_CHECK();
+		radix_sort_pairs(rkeys.data(), rvals.data(), rkeys_s.data(), rvals_s.data(), total, rep_bits,
+						 sort_scratch.data(), sort_scratch.count, s);
+	}
 	const int nxt = 1 - cur;
 	k_b_gather<<<grid_for(total, 256), 256, 0, s>>>(rvals_s.data(), total, rep_of_slot.data(), gd, pos[cur].data(),
 													mom.data(), ids.data(), relabel, pos[nxt].data(), snap.data(),

@@ -6,7 +6,9 @@ that replica's patch id.
 * 25 steps (step 1 build, reorders and forced rebuilds at 21): identical canonical
   relabelling (exported ids) and positions within the whole-run trajectory tolerance
   of the single-patch test;
-* the batch equals the same replicas stepped one by one as GPU Patches;
+* the batch equals the same replicas stepped one by one as GPU Patches, also when one
+  replica is above the one-workgroup sort's 32768 particles (the rebuild then takes the
+  global two-key radix sort);
 * a replica whose reference cell grid misses pairs across the periodic seam (after the
   pair-buffer growth of its first build) keeps the reference's pair set;
 * every launch configuration (workgroups per CU, hipGraph segment replay) gives the
@@ -72,6 +74,27 @@ def test_steps_match_oracle_and_single_patches(cuda):
         g = p.copy_particles_to_host()
         assert np.array_equal(got[r]["id"], g["id"])
         assert _min_image(got[r]["pos"][:, :3] - g["pos"][:, :3], L).max() < 1e-3, f"replica {r}"
+
+
+def test_replica_above_the_workgroup_sort_matches_single_patches(cuda):
+    specs = configs.config3(replicas=3, n_min=300, n_max=3000, density=4e-4, seed=17)
+    big = configs.config3(replicas=1, n_min=34_000, n_max=36_000, density=4e-4, seed=17)[0]
+    big.patch_id = len(specs)
+    specs.append(big)
+    assert big.n > 32768
+    b = Batch(specs)
+    steps = 22  # the reorder at step 21 sorts again
+    b.run(steps)
+    cuda.cuda.synchronize()
+    got = b.copy_particles_to_host()
+    for r, s in enumerate(specs):
+        print(f"case: replica {r} n={s.n}", flush=True)
+        p = marsmd.Patch(s)
+        p.run(steps)
+        cuda.cuda.synchronize()
+        g = p.copy_particles_to_host()
+        assert np.array_equal(got[r]["id"], g["id"]), f"replica {r}: canonical order differs"
+        assert _min_image(got[r]["pos"][:, :3] - g["pos"][:, :3], s.box.size[0]).max() < 1e-3, f"replica {r}"
 
 
 def test_incomplete_reference_grid_after_pair_buffer_growth(cuda):
