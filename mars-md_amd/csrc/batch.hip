@@ -102,41 +102,56 @@ __global__ void k_b_rep_keys(const uint32_t* __restrict__ vals_s, uint64_t total
 
 // The same order in one launch when every replica has at most kSegSortMax particles:
 // a stable LSD radix sort of each replica's Morton keys (three 10-bit digits) by one
-// workgroup of kSegWaves waves. The encode leaves replica r's entries at [slot0, slot0 + n)
-// in canonical order, so sorting each such range stably by key gives the order of the
-// global stable sort by key followed by the stable sort by replica. Wave w takes the run
-// [w C, w C + C) of the range (C = ceil(n / kSegWaves)) in batches of 64 held in
-// registers, counts its digits into hist[digit][w], and after the block's exclusive scan
-// (digit-major, wave-minor) scatters each batch in order: the entries of one batch with
-// one digit rank by lane (their peer mask from 10 ballots) and the lowest of them moves
-// the wave's offset for that digit on. Equal digits therefore keep wave, batch and lane
-// order. Passes ping-pong through the replica's range of two global buffer pairs (L2
-// resident; the barrier orders one pass's stores before the next pass's loads); the last
-// pass writes only the old slots, the order the gather reads.
+// workgroup of kSegWaves waves, the replica resident in LDS. The encode leaves replica
+// r's entries at [slot0, slot0 + n) in canonical order, so sorting each such range
+// stably by key gives the order of the global stable sort by key followed by the stable
+// sort by replica. Wave w takes the run [w C, w C + C) of the range (C = ceil(n /
+// kSegWaves)) in batches of 64 held in registers, counts its digits, and after the
+// block's exclusive scan (digit-major, wave-minor) scatters each batch in order: the
+// entries of one batch with one digit rank by lane (their peer mask from 10 ballots) and
+// the lowest of them moves the wave's offset for that digit on. Equal digits therefore
+// keep wave, batch and lane order. A pass reads the whole range into registers before
+// its first barrier and scatters it back after the scan, so one LDS copy suffices; the
+// entries carry their canonical position (16 bits), and the old slots are read once,
+// in order, at the end. Counts and offsets stay below 2^16, so two waves share a
+// histogram word (wave 2k in the low half, 2k + 1 in the high half).
 constexpr int kSegWaves = 16;
 constexpr int kSegThreads = kSegWaves * 64;
-constexpr int kSegPerLane = 32;
-constexpr int kSegStride = kSegWaves + 1; // hist row per digit, padded against bank conflicts
-constexpr uint32_t kSegSpare = 1024u * kSegStride; // the spare bin
-constexpr uint32_t kSegSortMax = uint32_t(kSegThreads) * kSegPerLane;
+constexpr int kSegPerLane = 20;
+constexpr uint32_t kSegSortMax = uint32_t(kSegThreads) * kSegPerLane; // 20480: 159.8 KB of LDS
+constexpr int kSegStride = kSegWaves / 2 + 1;                          // words per digit, padded
+constexpr uint32_t kSegSpare = 1024u * kSegStride;                      // the bin of idle lanes
 
-__global__ __launch_bounds__(kSegThreads) void k_b_seg_sort(const RepGeom* __restrict__ geom, uint32_t* keys_a,
-															uint32_t* vals_a, uint32_t* keys_b, uint32_t* vals_b,
+__global__ __launch_bounds__(kSegThreads) void k_b_seg_sort(const RepGeom* __restrict__ geom,
+															const uint32_t* __restrict__ order,
+															const uint32_t* __restrict__ keys,
+															const uint32_t* __restrict__ vals,
 															uint32_t* __restrict__ out_vals) {
+	__shared__ uint32_t s_key[kSegSortMax];
+	__shared__ uint16_t s_pos[kSegSortMax];
 	__shared__ uint32_t hist[1024 * kSegStride + 1];
 	__shared__ uint32_t wsum[kSegWaves];
-	const uint32_t base = geom[blockIdx.x].slot0, n = geom[blockIdx.x].n;
+	const uint32_t r = order[blockIdx.x];
+	const uint32_t base = geom[r].slot0, n = geom[r].n;
 	const int wv = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
+	const uint32_t half = 16u * uint32_t(wv & 1), col = uint32_t(wv >> 1);
 	const uint32_t C = (n + kSegWaves - 1) / kSegWaves; // <= 64 * kSegPerLane
 	const uint32_t w0_ = min(n, uint32_t(wv) * C), w1_ = min(n, w0_ + C);
 	const uint64_t lt = (uint64_t(1) << lane) - 1;
-	uint32_t key[kSegPerLane], val[kSegPerLane];
+	uint32_t key[kSegPerLane], at[kSegPerLane];
+	if (w1_ > w0_) { // the first pass's entries, from global memory (reads clamped into the run)
+		const uint32_t nb = (w1_ - w0_ + 63u) / 64u;
+#pragma unroll
+		for (int j = 0; j < kSegPerLane; ++j) {
+			if (uint32_t(j) < nb) { // wave-uniform
+				const uint32_t e = min(w0_ + uint32_t(j) * 64u + uint32_t(lane), w1_ - 1u);
+				key[j] = keys[base + e];
+				at[j] = e;
+			}
+		}
+	}
 #pragma unroll 1
 	for (int pass = 0; pass < 3; ++pass) {
-		const uint32_t* kin = pass == 1 ? keys_b : keys_a;
-		const uint32_t* vin = pass == 1 ? vals_b : vals_a;
-		uint32_t* kout = pass == 0 ? keys_b : keys_a;
-		uint32_t* vout = pass == 0 ? vals_b : (pass == 1 ? vals_a : out_vals);
 		const int shift = 10 * pass;
 		// the run's bounds, opaque to the compiler each pass: otherwise it hoists every
 		// batch's address and mask out of the pass loop (64 SGPRs and 32 VGPRs)
@@ -146,30 +161,35 @@ __global__ __launch_bounds__(kSegThreads) void k_b_seg_sort(const RepGeom* __res
 		for (uint32_t i = threadIdx.x; i <= kSegSpare; i += kSegThreads)
 			hist[i] = 0u;
 		__syncthreads();
-		if (w1 > w0) { // loads clamped into the run, a lane past its end counted in a spare bin
+		if (w1 > w0) { // reads clamped into the run, a lane past its end counted in the spare bin
+			if (pass > 0) {
 #pragma unroll
-			for (int j = 0; j < kSegPerLane; ++j) {
-				if (uint32_t(j) < nb) { // wave-uniform
-					const uint32_t e = min(w0 + uint32_t(j) * 64u + uint32_t(lane), w1 - 1u);
-					key[j] = kin[base + e];
-					val[j] = vin[base + e];
+				for (int j = 0; j < kSegPerLane; ++j) {
+					if (uint32_t(j) < nb) { // wave-uniform
+						const uint32_t e = min(w0 + uint32_t(j) * 64u + uint32_t(lane), w1 - 1u);
+						key[j] = s_key[e];
+						at[j] = s_pos[e];
+					}
 				}
 			}
 #pragma unroll
 			for (int j = 0; j < kSegPerLane; ++j) {
 				if (uint32_t(j) < nb) {
 					const bool live = w0 + uint32_t(j) * 64u + uint32_t(lane) < w1;
-					atomicAdd(&hist[live ? ((key[j] >> shift) & 1023u) * kSegStride + uint32_t(wv) : kSegSpare], 1u);
+					atomicAdd(&hist[live ? ((key[j] >> shift) & 1023u) * kSegStride + col : kSegSpare],
+							  live ? 1u << half : 1u);
 				}
 			}
 		}
 		__syncthreads();
 		{ // thread t owns digit t: its waves' counts become their first destinations
 			const uint32_t row = threadIdx.x * kSegStride;
-			uint32_t s = 0;
+			uint32_t c[kSegWaves / 2], s = 0;
 #pragma unroll
-			for (int w = 0; w < kSegWaves; ++w)
-				s += hist[row + uint32_t(w)];
+			for (int k = 0; k < kSegWaves / 2; ++k) {
+				c[k] = hist[row + uint32_t(k)];
+				s += (c[k] & 0xFFFFu) + (c[k] >> 16);
+			}
 			const uint32_t incl = wave_inclusive_scan(s);
 			if (lane == 63)
 				wsum[wv] = incl;
@@ -178,17 +198,18 @@ __global__ __launch_bounds__(kSegThreads) void k_b_seg_sort(const RepGeom* __res
 			for (int w = 0; w < wv; ++w)
 				run += wsum[w];
 #pragma unroll
-			for (int w = 0; w < kSegWaves; ++w) {
-				const uint32_t c = hist[row + uint32_t(w)];
-				hist[row + uint32_t(w)] = run;
-				run += c;
+			for (int k = 0; k < kSegWaves / 2; ++k) {
+				const uint32_t lo = run;
+				run += c[k] & 0xFFFFu;
+				hist[row + uint32_t(k)] = lo | (run << 16);
+				run += c[k] >> 16;
 			}
 		}
 		__syncthreads();
 #pragma unroll
 		for (int j = 0; j < kSegPerLane; ++j) {
 			if (uint32_t(j) >= nb) // wave-uniform
-				break;
+				continue;
 			const bool live = w0 + uint32_t(j) * 64u + uint32_t(lane) < w1;
 			const uint32_t d = (key[j] >> shift) & 1023u;
 			uint64_t peers = __ballot(live);
@@ -198,18 +219,25 @@ __global__ __launch_bounds__(kSegThreads) void k_b_seg_sort(const RepGeom* __res
 				peers &= ((d >> b) & 1u) ? ones : ~ones;
 			}
 			if (live) {
-				uint32_t* h = &hist[d * kSegStride + uint32_t(wv)];
-				const uint32_t first = *h;
+				// the other half of the word is the sibling wave's: it moves by atomic adds
+				uint32_t* h = &hist[d * kSegStride + col];
+				const uint32_t first = (*h >> half) & 0xFFFFu;
 				const uint32_t rank = uint32_t(__popcll(peers & lt));
 				if (rank == 0u)
-					*h = first + uint32_t(__popcll(peers));
-				const uint32_t dst = base + first + rank;
+					atomicAdd(h, uint32_t(__popcll(peers)) << half);
+				const uint32_t dst = first + rank;
 				if (pass < 2)
-					kout[dst] = key[j];
-				vout[dst] = val[j];
+					s_key[dst] = key[j];
+				s_pos[dst] = uint16_t(at[j]);
 			}
 		}
 		__syncthreads();
+	}
+#pragma unroll
+	for (int k = 0; k < kSegPerLane; ++k) { // every load in flight at once
+		const uint32_t i = threadIdx.x + uint32_t(k) * kSegThreads;
+		if (i < n)
+			out_vals[base + i] = vals[base + s_pos[i]];
 	}
 }
 
@@ -662,6 +690,14 @@ BatchImpl::BatchImpl(const marsmd_batch_desc& d) : tables(new NbTables{}) {
 		std::fill(rep.begin() + long(off[r]), rep.begin() + long(off[r + 1]), uint32_t(r));
 	rep_of_slot.resize(total);
 	MMD_HIP(hipMemcpy(rep_of_slot.data(), rep.data(), 4 * total, hipMemcpyHostToDevice));
+	// the per-replica sort's workgroups, largest replica first (the dispatcher hands the
+	// later, smaller ones to the CUs that free up first)
+	std::vector<uint32_t> order(R);
+	for (uint64_t r = 0; r < R; ++r)
+		order[r] = uint32_t(r);
+	std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return count[a] > count[b]; });
+	sort_order.resize(R);
+	MMD_HIP(hipMemcpy(sort_order.data(), order.data(), 4 * R, hipMemcpyHostToDevice));
 	for (int b = 0; b < 2; ++b)
 		pos[b].resize(total);
 	mom.resize(total);
@@ -897,7 +933,7 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 													vals.data(), scalars.data());
 	MMD_LAUNCH_CHECK();
 	if (*std::max_element(count.begin(), count.end()) <= kSegSortMax) {
-		k_b_seg_sort<<<unsigned(R), kSegThreads, 0, s>>>(gd, keys.data(), vals.data(), keys_s.data(), vals_s.data(),
+		k_b_seg_sort<<<unsigned(R), kSegThreads, 0, s>>>(gd, sort_order.data(), keys.data(), vals.data(),
 														 rvals_s.data());
 		MMD_LAUNCH_CHECK();
 	} else {

@@ -29,6 +29,7 @@ struct BatchImpl {
 	DevArray<int32_t> ids, ids_tmp;
 	DevArray<float4> mom_tmp;
 	DevArray<uint32_t> rep_of_slot, slot_of_canon, slot_of_canon_tmp;
+	DevArray<uint32_t> sort_order; // replicas by size, largest first
 	// rebuild
 	DevArray<uint32_t> keys, vals, keys_s, vals_s, rkeys, rvals, rkeys_s, rvals_s, codes, perm;
 	DevArray<uint8_t> sort_scratch, geom;
