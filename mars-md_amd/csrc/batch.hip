@@ -718,6 +718,10 @@ BatchImpl::BatchImpl(const marsmd_batch_desc& d) : tables(new NbTables{}) {
 
 BatchImpl::~BatchImpl() {
 	clear_graphs();
+	if (readback)
+		(void)hipHostFree(readback);
+	if (staging)
+		(void)hipHostFree(staging);
 	if (capture_stream)
 		(void)hipStreamDestroy(capture_stream);
 	for (hipEvent_t e : timing_events)
@@ -913,10 +917,17 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 	num_fine = fine_total;
 	num_home = home_total;
 	num_items = items[R];
-	geom.resize(R * sizeof(RepGeom));
-	MMD_HIP(hipMemcpyAsync(geom.data(), geom_host.data(), R * sizeof(RepGeom), hipMemcpyHostToDevice, s));
+	// both uploads from pinned memory, so that they queue behind the running steps
+	// without the host waiting for them (the last build's copies ended before its wait)
+	const size_t gbytes = R * sizeof(RepGeom);
+	if (!staging)
+		MMD_HIP(hipHostMalloc(reinterpret_cast<void**>(&staging), gbytes + 4 * (R + 1), hipHostMallocDefault));
+	std::memcpy(staging, geom_host.data(), gbytes);
+	std::memcpy(staging + gbytes, items.data(), 4 * (R + 1));
+	geom.resize(gbytes);
+	MMD_HIP(hipMemcpyAsync(geom.data(), staging, gbytes, hipMemcpyHostToDevice, s));
 	item_off.resize(R + 1);
-	MMD_HIP(hipMemcpyAsync(item_off.data(), items.data(), 4 * (R + 1), hipMemcpyHostToDevice, s));
+	MMD_HIP(hipMemcpyAsync(item_off.data(), staging + gbytes, 4 * (R + 1), hipMemcpyHostToDevice, s));
 	const RepGeom* gd = reinterpret_cast<const RepGeom*>(geom.data());
 	// sort: Morton key (stable; ties by canonical index), then replica (stable)
 	keys.resize(total);
@@ -998,9 +1009,17 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 		k_b_count_stats<<<std::min<unsigned>(grid_for(total, 256), 240), 256, 0, s>>>(
 			counts.data(), total, reinterpret_cast<unsigned long long*>(scalars.data() + 4), scalars.data() + 2);
 		MMD_LAUNCH_CHECK();
-		uint32_t host[6] = {0, 0, 0, 0, 0, 0};
-		MMD_HIP(hipMemcpyAsync(host, scalars.data(), 24, hipMemcpyDeviceToHost, s));
+		// Pairlist::grow_pair_capacity (Pairlist.h:244-251) bookkeeping per replica, read
+		// back with the walk's statistics: one wait for both (into pinned memory)
+		k_b_rep_pairs<<<unsigned(R), 256, 0, s>>>(counts.data(), gd, rep_pairs.data());
+		MMD_LAUNCH_CHECK();
+		if (!readback)
+			MMD_HIP(hipHostMalloc(reinterpret_cast<void**>(&readback), 32 + 8 * R, hipHostMallocDefault));
+		MMD_HIP(hipMemcpyAsync(readback, scalars.data(), 24, hipMemcpyDeviceToHost, s));
+		MMD_HIP(hipMemcpyAsync(readback + 4, rep_pairs.data(), 8 * R, hipMemcpyDeviceToHost, s));
 		MMD_HIP(hipStreamSynchronize(s));
+		uint32_t host[6];
+		std::memcpy(host, readback, 24);
 		if (host[0] & 1u)
 			raise(MARSMD_ERR_VALUE, "batch: a particle left its replica's box (positions must stay wrapped)");
 		if (host[0] & 2u)
@@ -1011,12 +1030,8 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 			break;
 		cap = (max_neighbors + max_neighbors / 8 + 4 + 3) & ~3u;
 	}
-	// Pairlist::grow_pair_capacity (Pairlist.h:244-251) bookkeeping per replica: it sets
-	// the reference grid of the next build
-	k_b_rep_pairs<<<unsigned(R), 256, 0, s>>>(counts.data(), gd, rep_pairs.data());
-	MMD_LAUNCH_CHECK();
-	MMD_HIP(hipMemcpyAsync(rep_pairs_host.data(), rep_pairs.data(), 8 * R, hipMemcpyDeviceToHost, s));
-	MMD_HIP(hipStreamSynchronize(s));
+	// the pair counts set the reference grid of the next build
+	std::memcpy(rep_pairs_host.data(), readback + 4, 8 * R);
 	for (uint64_t r = 0; r < R; ++r) {
 		const uint64_t np = rep_pairs_host[r] / 2;
 		if (np > ref_max_pairs[r])
@@ -1032,11 +1047,11 @@ bool BatchImpl::needs_rebuild(float skin, hipStream_t s) {
 		pos[cur].data(), snap.data(), total, rep_of_slot.data(), reinterpret_cast<const RepGeom*>(geom.data()),
 		scalars.data() + 1);
 	MMD_LAUNCH_CHECK();
-	uint32_t bits = 0;
-	MMD_HIP(hipMemcpyAsync(&bits, scalars.data() + 1, 4, hipMemcpyDeviceToHost, s));
+	// into the pinned word after the walk's statistics (a build has allocated it)
+	MMD_HIP(hipMemcpyAsync(readback + 3, scalars.data() + 1, 4, hipMemcpyDeviceToHost, s));
 	MMD_HIP(hipStreamSynchronize(s));
 	float d2;
-	std::memcpy(&d2, &bits, 4);
+	std::memcpy(&d2, readback + 3, 4);
 	return std::sqrt(d2) > skin * 0.5f;
 }
 

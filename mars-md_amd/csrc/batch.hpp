@@ -50,6 +50,8 @@ struct BatchImpl {
 	std::vector<uint32_t> items_host;
 	std::vector<uint64_t> ref_max_pairs; // per replica: Pairlist::max_pairs_ bookkeeping
 	std::vector<uint64_t> rep_pairs_host;
+	uint8_t* staging = nullptr;   // pinned: the geometry and item offsets of a build's uploads
+	uint64_t* readback = nullptr; // pinned: the walk's statistics (3 words), then the replicas' pair counts
 	DevArray<unsigned long long> rep_pairs;
 	int rep_bits = 1;
 	float built_cutoff = 0.0f;
