@@ -912,8 +912,20 @@ void BatchImpl::build(float pairlist_cutoff, bool relabel, hipStream_t s) {
 	const float rc = pairlist_cutoff;
 	const float rc2 = rc * rc;
 	std::vector<uint32_t>& items = items_host;
-	uint64_t fine_total = 0, home_total = 0;
-	plan_geometry(rc, geom_host, items, fine_total, home_total);
+	// the plan depends on the cutoff, the cell configuration and the pair bookkeeping
+	// (boxes and sizes are fixed): planned again only when one of them changed, which
+	// keeps it off the GPU's idle time when a run starts with a rebuild
+	if (!(plan.valid && plan.rc == rc && plan.target == cell_target && plan.fine == cell_fine &&
+		  plan.ref == ref_max_pairs)) {
+		plan.valid = false; // until planned whole
+		plan_geometry(rc, geom_host, items, plan.fine_total, plan.home_total);
+		plan.valid = true;
+		plan.rc = rc;
+		plan.target = cell_target;
+		plan.fine = cell_fine;
+		plan.ref = ref_max_pairs;
+	}
+	const uint64_t fine_total = plan.fine_total, home_total = plan.home_total;
 	num_fine = fine_total;
 	num_home = home_total;
 	num_items = items[R];

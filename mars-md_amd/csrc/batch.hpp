@@ -48,6 +48,13 @@ struct BatchImpl {
 	DevArray<int4> stencils;
 	size_t stencils_uploaded = 0;
 	std::vector<uint32_t> items_host;
+	struct Plan { // what geom_host and items_host were planned from
+		bool valid = false;
+		float rc = 0.0f;
+		std::vector<int> target, fine;
+		std::vector<uint64_t> ref;
+		uint64_t fine_total = 0, home_total = 0;
+	} plan;
 	std::vector<uint64_t> ref_max_pairs; // per replica: Pairlist::max_pairs_ bookkeeping
 	std::vector<uint64_t> rep_pairs_host;
 	uint8_t* staging = nullptr;   // pinned: the geometry and item offsets of a build's uploads
