@@ -9,8 +9,9 @@
 // relabelled at every reorder as Patch::reorder_particles does, by a sort whose ties
 // fall to the canonical index as the reference's stable sort; its pair set is its own
 // ZOrderPairlist's. Only the scheduling is shared: replica r owns slots
-// [off_r, off_r + n_r), every sort is stable within replicas (Morton key first, then
-// the replica id), and each kernel serves all replicas.
+// [off_r, off_r + n_r), every sort is stable within replicas (each replica's range
+// sorted by Morton key in one workgroup; above 20480 particles a global sort by key,
+// then by replica id), and each kernel serves all replicas.
 //
 // Cadence: at a reorder step every replica reorders and rebuilds, as each Patch does.
 // At a rebuild-period step that is not a reorder step the displacements of all
@@ -87,8 +88,9 @@ __global__ void k_b_encode(const float4* __restrict__ pos, uint64_t total, const
 		atomicOr(outside, 1u);
 }
 
-// Second pass key: the replica of each Morton-sorted entry (a slot of that replica),
-// carrying the entry's current slot along.
+// Second pass key of the global sort (replicas above the per-replica sort's size): the
+// replica of each Morton-sorted entry (a slot of that replica), carrying the entry's
+// current slot along.
 __global__ void k_b_rep_keys(const uint32_t* __restrict__ vals_s, uint64_t total,
 							 const uint32_t* __restrict__ rep_of_slot, uint32_t* __restrict__ rkeys,
 							 uint32_t* __restrict__ rvals) {
