@@ -214,12 +214,18 @@ __global__ __launch_bounds__(kSegThreads) void k_b_seg_sort(const RepGeom* __res
 				continue;
 			const bool live = w0 + uint32_t(j) * 64u + uint32_t(lane) < w1;
 			const uint32_t d = (key[j] >> shift) & 1023u;
-			uint64_t peers = __ballot(live);
+			// the peer mask on 32-bit halves, sx all ones where the lane's bit is set: the
+			// lanes that agree on every bit stay (4 VALU a bit instead of 10 on 64-bit masks)
+			const uint64_t lv = __ballot(live);
+			uint32_t plo = uint32_t(lv), phi = uint32_t(lv >> 32);
 #pragma unroll
 			for (int b = 0; b < 10; ++b) {
-				const uint64_t ones = __ballot(live && ((d >> b) & 1u));
-				peers &= ((d >> b) & 1u) ? ones : ~ones;
+				const int32_t sx = int32_t(d << (31 - b)) >> 31;
+				const uint64_t ones = __ballot(sx != 0);
+				plo &= ~(uint32_t(ones) ^ uint32_t(sx));
+				phi &= ~(uint32_t(ones >> 32) ^ uint32_t(sx));
 			}
+			const uint64_t peers = uint64_t(plo) | (uint64_t(phi) << 32);
 			if (live) {
 				// the other half of the word is the sibling wave's: it moves by atomic adds
 				uint32_t* h = &hist[d * kSegStride + col];
