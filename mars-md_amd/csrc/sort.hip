@@ -38,16 +38,20 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // Lanes of this wave holding the same BITS-bit digit (and valid).
+// Built on 32-bit halves, sx all ones where the lane's bit b is set: the lanes that agree
+// on every bit stay (4 VALU a bit instead of 10 on 64-bit masks).
 template<int BITS>
 __device__ __forceinline__ uint64_t digit_peers(uint32_t d, bool valid) {
-	uint64_t peers = __ballot(valid);
+	const uint64_t v = __ballot(valid);
+	uint32_t lo = uint32_t(v), hi = uint32_t(v >> 32);
 #pragma unroll
 	for (int b = 0; b < BITS; ++b) {
-		const bool bit = (d >> b) & 1u;
-		const uint64_t m = __ballot(bit);
-		peers &= bit ? m : ~m;
+		const int32_t sx = int32_t(d << (31 - b)) >> 31;
+		const uint64_t m = __ballot(sx != 0);
+		lo &= ~(uint32_t(m) ^ uint32_t(sx));
+		hi &= ~(uint32_t(m >> 32) ^ uint32_t(sx));
 	}
-	return valid ? peers : 0;
+	return valid ? (uint64_t(lo) | (uint64_t(hi) << 32)) : 0;
 }
 
 // Digit d of a key is (key >> shift) & mask, mask < 2^BITS. A tile's counts do not
