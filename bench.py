@@ -155,15 +155,17 @@ def batch_bytes_per_particle_step(specs) -> float:
 def bench_batch(torch, steps: int, warmup: int, sequential: bool = True) -> dict:
     """Aggregate particle-steps/s of 512 independent BD replicas (N log-uniform in
     [500, 20000], density 1e-4, config-0 table) stepped together at the autotuner's
-    launch (median of REPEATS runs of `steps` steps), and of the same replicas stepped
-    one after another as separate Patches."""
+    launch (median of REPEATS runs of K = max(steps, 100) steps: whole runs of five
+    rebuild periods, so that the rebuild each run starts with on an idle GPU weighs as
+    it does in a long run), and of the same replicas stepped one after another as
+    separate Patches."""
     import marsmd
     from marsmd import configs
     from marsmd.batch import Batch
 
     specs = configs.config3()
     total = sum(s.n for s in specs)
-    K = max(steps, 20)
+    K = max(steps, 100)
     b = Batch(specs)
     # warmup: the cell configuration per size bucket (timed on the walk), then the launch
     # configuration (timed on real steps), both the autotuners' real steps
